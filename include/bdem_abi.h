@@ -1,0 +1,165 @@
+/*
+ * bdem_abi.h - C ABI of the MI355X DEM step library (libbdem.so).
+ *
+ * The entry points replace the reference's device layer, the C++ free
+ * functions of Device/DeviceInterface.h:16-95 that the KDevice facade
+ * (Interface/KDevice.h:17-99) forwards to.  Differences, by design:
+ *   - state lives in an opaque context (the reference keeps process globals,
+ *     Device/DeviceInterface.cu:52-186), so several simulations can coexist;
+ *   - every call returns a status code instead of calling exit(1)
+ *     (Device/DeviceInterface.cu:1891-1899 and after every launch);
+ *   - bdem_update_sim is asynchronous on the context's stream; results are
+ *     visible after bdem_sync or any bdem_get_* call;
+ *   - no allocation happens per step (the reference allocates and frees
+ *     about a dozen buffers per step, Device/DeviceInterface.cu:1929-2057).
+ * Units and array layouts are the reference's: cm-kg-s, AoS float3
+ * (12 bytes) positions/velocities, Quaterion {w,x,y,z}.
+ *
+ * A C++ shim with the reference's own names and signatures
+ * (Device_Set_SimulationData, Device_DEM_UpdateSim, ...) is declared in
+ * bdem_device_compat.h.
+ */
+#ifndef BDEM_ABI_H_
+#define BDEM_ABI_H_
+
+#include "bdem_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define BDEM_OK            0
+#define BDEM_EINVAL       -1   /* bad argument or configuration */
+#define BDEM_EHIP         -2   /* HIP runtime error (message in bdem_last_error) */
+#define BDEM_ENOTSUP      -3   /* entry point without a sphere-path implementation */
+#define BDEM_ESTATE       -4   /* called in the wrong order (e.g. step before setup) */
+#define BDEM_ENOMEM       -5   /* device allocation failed */
+
+/* Warning bits reported by bdem_get_status (kernels never abort). */
+#define BDEM_WARN_OUT_OF_GRID    1u  /* a live particle left the hash grid */
+#define BDEM_WARN_LIST_OVERFLOW  2u  /* > 32 contacts on a particle (reference list size) */
+#define BDEM_WARN_HIST_OVERFLOW  4u  /* more contacts than tangential-history slots */
+
+typedef struct bdem_ctx bdem_ctx;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+/* Create a context on HIP device `device`.  No reference counterpart (the
+ * reference selects device 0 inside Device_Set_SimulationData,
+ * Device/DeviceInterface.cu:1566). */
+int bdem_create(bdem_ctx** out, int device);
+/* Device_Clean, Device/DeviceInterface.h:76 (frees everything the context owns). */
+int bdem_destroy(bdem_ctx* ctx);
+/* Text of the last error on this context (never NULL). */
+const char* bdem_last_error(const bdem_ctx* ctx);
+
+/* ---- setup -------------------------------------------------------------- */
+/* Device_Set_SimulationData, Device/DeviceInterface.h:16-19 (defined
+ * Device/DeviceInterface.cu:1499-1816).  Allocates for sim->Num_Particles,
+ * uploads the object tables and sets every particle's velocity to
+ * init->velocity[0] (zero when init->use_file), angular velocity to zero.
+ * Positions come from bdem_set_pos* (the reference's grid fillers are not
+ * part of this library). */
+int bdem_set_simulation_data(bdem_ctx* ctx, const bdem_WOBJ* wobj, const bdem_POBJ* pobj,
+                             const bdem_DOBJ* dobj, const bdem_SimulationInfo* sim, int num_wobj,
+                             int num_pobj, int num_dobj, const bdem_InitConfig* init);
+/* Extension parameters (contact law, cones, per-particle radius); call after
+ * bdem_set_simulation_data.  No reference counterpart. */
+int bdem_set_ext_params(bdem_ctx* ctx, const bdem_ext_params* ext);
+
+/* ---- state in ----------------------------------------------------------- */
+/* Device_Set_Pos, Device/DeviceInterface.h:22-23 (ornt may be NULL for spheres). */
+int bdem_set_pos(bdem_ctx* ctx, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                 const unsigned int* ptype, const int* pid);
+/* Device_Set_Pos_Velocity, Device/DeviceInterface.h:26-28 */
+int bdem_set_pos_velocity(bdem_ctx* ctx, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                          const bdem_float3* vel, const bdem_float3* avel,
+                          const unsigned int* ptype, const int* pid);
+/* Device_Set_System_State, Device/DeviceInterface.h:31-33 (acc is accepted and
+ * ignored, as Set_PSystem_State ignores it, GPU_Memory_Transfer.cuh:397-440). */
+int bdem_set_system_state(bdem_ctx* ctx, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                          const bdem_float3* vel, const bdem_float3* avel, const bdem_float3* acc,
+                          const unsigned int* ptype, const int* pid);
+/* Per-particle radius (cm) and mass (kg) in the current particle order;
+ * requires ext->per_particle_radius.  No reference counterpart (the
+ * reference takes both from the particle type). */
+int bdem_set_radii(bdem_ctx* ctx, const float* radius, const float* mass);
+/* Device_Set_Tallys, Device/DeviceInterface.h:36 */
+int bdem_set_tallys(bdem_ctx* ctx, const float* tallys);
+
+/* ---- state out (synchronous) -------------------------------------------- */
+/* Device_Get_System_State, Device/DeviceInterface.h:42-47: all
+ * Num_Particles slots in device order; acc (may be NULL) is zero. */
+int bdem_get_system_state(bdem_ctx* ctx, bdem_float3* pos, bdem_Quaterion* ornt, bdem_float3* vel,
+                          bdem_float3* avel, bdem_float3* acc, unsigned int* ptype, int* pid);
+/* Device_Get_P_Positions, Device/DeviceInterface.h:50-51 */
+int bdem_get_positions(bdem_ctx* ctx, bdem_float3* pos, bdem_Quaterion* ornt, unsigned int* ptype,
+                       int* pid);
+/* Device_Get_P_Velocity, Device/DeviceInterface.h:54-55 (avel written when flag == 1) */
+int bdem_get_velocity(bdem_ctx* ctx, bdem_float3* vel, bdem_float3* avel, int flag,
+                      unsigned int* ptype, int* pid);
+/* Device_Get_W_Forces, Device/DeviceInterface.h:61: wall force of the last
+ * step per particle (recorded only after bdem_record_wall_forces(ctx, 1));
+ * contact points are not recorded and are returned as zero. */
+int bdem_get_wall_forces(bdem_ctx* ctx, bdem_float3* wall_forces, bdem_float3* contact_points);
+int bdem_record_wall_forces(bdem_ctx* ctx, int on);
+/* Device_Get_Energy, Device/DeviceInterface.h:63 (three tallies, scaled 1E-4
+ * like GetEnergy, GPU_Memory_Transfer.cuh:603-619). */
+int bdem_get_energy(bdem_ctx* ctx, float* energy);
+/* Kernel warnings accumulated since the last call (BDEM_WARN_* bits); clears them. */
+int bdem_get_status(bdem_ctx* ctx, unsigned int* warnings);
+/* Particles removed by discharge checks so far. */
+int bdem_get_kill_count(bdem_ctx* ctx, int* killed);
+int bdem_num_particles(const bdem_ctx* ctx);
+long long bdem_num_cells(const bdem_ctx* ctx);
+
+/* ---- actions -------------------------------------------------------------- */
+/* Device_Update_DObjectPostions, Device/DeviceInterface.h:72 */
+int bdem_update_dobject_positions(bdem_ctx* ctx, const bdem_DOBJ* dobj);
+/* Device_Update_WorldObjectPostions, Device/DeviceInterface.h:73 */
+int bdem_update_world_object_positions(bdem_ctx* ctx, const bdem_WOBJ* wobj);
+/* Device_OpenHatch, Device/DeviceInterface.h:75: upload, then drop the last
+ * world object (Hatch, GPU_Memory_Transfer.cuh:703-706). */
+int bdem_open_hatch(bdem_ctx* ctx, const bdem_WOBJ* wobj, int num_wobj);
+/* Device_Reset_Energy, Device/DeviceInterface.h:77 */
+int bdem_reset_energy(bdem_ctx* ctx, int num_values);
+/* drum_rotation, Device/DeviceInterface.h:80 */
+int bdem_drum_rotation(bdem_ctx* ctx, int cylinder_rotation);
+/* Start_Bin, Device/DeviceInterface.h:81 (a host flag in the reference) */
+int bdem_start_bin(bdem_ctx* ctx);
+/* Device_AddForce / SubForce / ZeroForce, Device/DeviceInterface.h:83-85:
+ * interactive no-ops in the reference (they change a device global no kernel
+ * reads); accepted and ignored. */
+int bdem_add_force(bdem_ctx* ctx);
+int bdem_sub_force(bdem_ctx* ctx);
+int bdem_zero_force(bdem_ctx* ctx);
+
+/* ---- the hot path --------------------------------------------------------- */
+/* Device_DEM_UpdateSim, Device/DeviceInterface.h:95 (defined
+ * Device/DeviceInterface.cu:1825-2513): one step, asynchronous. */
+int bdem_update_sim(bdem_ctx* ctx);
+/* nsteps consecutive bdem_update_sim calls. */
+int bdem_step(bdem_ctx* ctx, int nsteps);
+/* Wait for the context's stream. */
+int bdem_sync(bdem_ctx* ctx);
+
+/* ---- stage outputs for parity checks (synchronous; state unchanged) -------- */
+/* Keys and identity index of the current state (Kernel_SpatialDecomp_CalcPHash). */
+int bdem_debug_hash(bdem_ctx* ctx, unsigned int* keys, int* idx);
+/* Keys and index after the stable sort (SpatialDecomp_ThrustSort_ByHash). */
+int bdem_debug_sorted(bdem_ctx* ctx, unsigned int* keys, int* idx);
+/* cellStart / cellEnd of the sorted state (Kernel_SpatialDecomp_BinData),
+ * bdem_num_cells entries each, 0xFFFFFFFF for empty cells. */
+int bdem_debug_cells(bdem_ctx* ctx, unsigned int* cell_start, unsigned int* cell_end);
+/* Broad_List (32 per particle) and NN_NUM of the sorted state
+ * (Kernel_BroadCollisionDetection_NonSymmetry). */
+int bdem_debug_contacts(bdem_ctx* ctx, unsigned int* list32, unsigned int* nn);
+/* Pair force (with gravity), pair torque, wall force, wall torque of the
+ * sorted state, N x 3 floats each; sorted_pid receives the P_ID order. */
+int bdem_debug_forces(bdem_ctx* ctx, float* fpp, float* tpp, float* fw, float* tw, int* sorted_pid);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* BDEM_ABI_H_ */

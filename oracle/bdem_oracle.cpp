@@ -1,0 +1,183 @@
+/*
+ * bdem_oracle.cpp - extern "C" surface of the CPU restatement, for the
+ * tests and the CPU-baseline leg of the benchmark (loaded with ctypes).
+ *
+ * TEST INFRASTRUCTURE ONLY: the product library never links this.
+ *
+ * Handles carry their precision (32 = float parity definition, 64 = double
+ * accuracy reference).  Outputs are returned as double so that both
+ * precisions share one calling convention; float values convert exactly.
+ * Stage queries (hash/sorted/cells/contacts/forces) work on a copy and leave
+ * the state untouched, like the library's debug entry points.
+ */
+#include "bdem_oracle.hpp"
+#include <chrono>
+#include <memory>
+
+using namespace bdem_oracle;
+
+namespace {
+struct Handle {
+  int bits;
+  std::unique_ptr<Oracle<float>> f;
+  std::unique_ptr<Oracle<double>> d;
+};
+
+template <class R>
+void put3(const std::vector<V3<R>>& v, double* out) {
+  for (size_t i = 0; i < v.size(); ++i) {
+    out[3 * i] = double(v[i].x); out[3 * i + 1] = double(v[i].y); out[3 * i + 2] = double(v[i].z);
+  }
+}
+
+template <class R>
+int get_state(Oracle<R>& o, double* p, double* v, double* w, double* q, uint32_t* t, int* id) {
+  if (p) put3(o.pos, p);
+  if (v) put3(o.vel, v);
+  if (w) put3(o.avel, w);
+  for (int i = 0; i < o.N; ++i) {
+    if (q) {
+      q[4 * i] = double(o.quat[i].w); q[4 * i + 1] = double(o.quat[i].x);
+      q[4 * i + 2] = double(o.quat[i].y); q[4 * i + 3] = double(o.quat[i].z);
+    }
+    if (t) t[i] = o.ptype[i];
+    if (id) id[i] = o.pid[i];
+  }
+  return 0;
+}
+}  // namespace
+
+#define WITH(h, ...)                                        \
+  do {                                                      \
+    Handle* H_ = static_cast<Handle*>(h);                   \
+    if (!H_) return -1;                                     \
+    if (H_->bits == 32) { auto& o = *H_->f; __VA_ARGS__; }  \
+    else { auto& o = *H_->d; __VA_ARGS__; }                 \
+  } while (0)
+
+extern "C" {
+
+void* bo_create(int bits, const bdem_SimulationInfo* sim, const bdem_WOBJ* w, int nw,
+                const bdem_POBJ* p, int np, const bdem_ext_params* ext) {
+  if (bits != 32 && bits != 64) return nullptr;
+  Handle* h = new Handle();
+  h->bits = bits;
+  if (bits == 32) {
+    h->f.reset(new Oracle<float>());
+    h->f->configure(sim, w, nw, p, np, ext);
+  } else {
+    h->d.reset(new Oracle<double>());
+    h->d->configure(sim, w, nw, p, np, ext);
+  }
+  return h;
+}
+
+void bo_destroy(void* h) { delete static_cast<Handle*>(h); }
+
+int bo_set_state(void* h, int n, const float* pos, const float* vel, const float* avel,
+                 const float* quat, const uint32_t* ptype, const int* pid, const float* radius,
+                 const float* mass) {
+  WITH(h, o.set_state(n, pos, vel, avel, quat, ptype, pid, radius, mass));
+  return 0;
+}
+
+int bo_get_state(void* h, double* pos, double* vel, double* avel, double* quat, uint32_t* ptype,
+                 int* pid) {
+  WITH(h, return get_state(o, pos, vel, avel, quat, ptype, pid));
+  return 0;
+}
+
+int bo_num_particles(void* h) { WITH(h, return o.N); return 0; }
+long long bo_num_cells(void* h) { WITH(h, return (long long)o.C); return 0; }
+
+int bo_hash(void* h, uint32_t* keys, int* idx) {
+  WITH(h, {
+    auto c = o;
+    c.compute_hash();
+    for (int i = 0; i < c.N; ++i) { keys[i] = c.keys[i]; idx[i] = c.idx[i]; }
+  });
+  return 0;
+}
+
+int bo_sorted(void* h, uint32_t* keys, int* idx) {
+  WITH(h, {
+    auto c = o;
+    c.compute_hash();
+    c.sort_by_key();
+    for (int i = 0; i < c.N; ++i) { keys[i] = c.keys[i]; idx[i] = c.idx[i]; }
+  });
+  return 0;
+}
+
+int bo_cells(void* h, uint32_t* cstart, uint32_t* cend) {
+  WITH(h, {
+    auto c = o;
+    c.prepare();
+    for (uint32_t i = 0; i < c.C; ++i) { cstart[i] = c.cell_start[i]; cend[i] = c.cell_end[i]; }
+  });
+  return 0;
+}
+
+int bo_contacts(void* h, uint32_t* list32, uint32_t* nn) {
+  WITH(h, {
+    auto c = o;
+    c.prepare();
+    std::vector<uint32_t> l, n;
+    c.contact_list(l, n);
+    std::copy(l.begin(), l.end(), list32);
+    std::copy(n.begin(), n.end(), nn);
+  });
+  return 0;
+}
+
+/* Forces on the sorted state: pair force (with gravity), pair torque, wall
+ * force, wall torque; each N x 3.  Also returns the sorted P_ID order. */
+int bo_forces(void* h, double* fpp, double* tpp, double* fw, double* tw, int* sorted_pid) {
+  WITH(h, {
+    auto c = o;
+    c.prepare();
+    c.compute_forces();
+    put3(c.f_pp, fpp); put3(c.t_pp, tpp); put3(c.f_w, fw); put3(c.t_w, tw);
+    if (sorted_pid) for (int i = 0; i < c.N; ++i) sorted_pid[i] = c.pid[i];
+  });
+  return 0;
+}
+
+int bo_step(void* h, int nsteps) {
+  WITH(h, { for (int s = 0; s < nsteps; ++s) o.step(); });
+  return 0;
+}
+
+/* Wall-clock seconds for nsteps (the CPU-baseline leg). */
+double bo_time_steps(void* h, int nsteps) {
+  auto t0 = std::chrono::steady_clock::now();
+  bo_step(h, nsteps);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+int bo_set_active_world_objects(void* h, int n) { WITH(h, o.num_wobj_active = n); return 0; }
+int bo_set_drum_rotation(void* h, int on) { WITH(h, o.cylinder_rotation = on != 0); return 0; }
+
+/* Bit 0: live particle outside the grid; bit 1: history overflow. */
+int bo_warnings(void* h) {
+  WITH(h, return (o.warn_out_of_grid ? 1 : 0) | (o.warn_hist_overflow ? 2 : 0));
+  return 0;
+}
+
+/* Pair-history records of the particle with P_ID p: count, then (id, xi) rows. */
+int bo_history(void* h, int p, int* ids, double* xi) {
+  WITH(h, {
+    if (p < 0 || p >= int(o.hcount.size())) return -1;
+    int c = o.hcount[p];
+    for (int k = 0; k < c; ++k) {
+      const auto& r = o.hist[size_t(p) * HMAX + k];
+      ids[k] = r.id;
+      xi[3 * k] = double(r.xi.x); xi[3 * k + 1] = double(r.xi.y); xi[3 * k + 2] = double(r.xi.z);
+    }
+    return c;
+  });
+  return 0;
+}
+
+}  // extern "C"

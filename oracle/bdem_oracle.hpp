@@ -1,0 +1,841 @@
+/*
+ * bdem_oracle.hpp - CPU restatement of the reference GPU step loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library: only the tests, the smoke check and the CPU-baseline leg of the
+ * benchmark load it.  Nothing in the product links or calls it.
+ *
+ * One template, instantiated for float (the parity definition, built with
+ * -ffp-contract=off -fno-fast-math so every operation is a single IEEE
+ * rounding in source order) and for double (the accuracy reference).
+ *
+ * Stage order follows Device_DEM_UpdateSim (Device/DeviceInterface.cu:1825-2513):
+ *   hash -> stable sort -> reorder -> cell bounds -> contact list ->
+ *   pair force (+gravity) -> wall force -> symplectic Euler -> discharge check.
+ * Each function names the reference lines whose arithmetic it restates.
+ *
+ * Conventions shared with the HIP library (defined there, restated here):
+ *   - keys: in-grid live particles get the reference hash; a live particle
+ *     outside the grid gets KEY_OUT = C, a dead one (P_ID < 0) KEY_DEAD = C+1,
+ *     so both sort behind every cell and are invisible to the cell tables.
+ *   - the 27-cell stencil is clamped to the grid (identical to the reference
+ *     for in-grid particles, Collision_Detection_BroadPhase.cuh:428-459).
+ *   - every live particle is processed each step; dead ones are inert.
+ *   - history laws keep tangential springs keyed by the partner's P_ID.
+ */
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <numeric>
+#include "bdem_types.h"
+
+namespace bdem_oracle {
+
+constexpr int LIST_STRIDE = 32;   /* Broad_List stride, Collision_Detection_BroadPhase.cuh:481 */
+constexpr int HMAX = 16;          /* pair history records per particle */
+constexpr int WMAX = 4;           /* wall history records per particle */
+
+template <class R> struct V3 {
+  R x, y, z;
+};
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, V3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+/* Utilities/Device/Functions.cuh:198-201: x*x + y*y + z*z, left to right */
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* Utilities/Device/Functions.cuh:370-373 */
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline float rsqrt_(float v) { return std::sqrt(v); }
+inline double rsqrt_(double v) { return std::sqrt(v); }
+/* Utilities/Device/Functions.cuh:241-244: sqrtf(dot(v,v)) */
+template <class R> inline R length(V3<R> a) { return rsqrt_(dot(a, a)); }
+template <class R> inline V3<R> cvt(bdem_float3 f) { return {R(f.x), R(f.y), R(f.z)}; }
+
+template <class R> struct Q4 { R w, x, y, z; };
+
+/* Quaternion product, Utilities/Device/Opertators.cuh:866-874 */
+template <class R> inline Q4<R> qmul(Q4<R> a, Q4<R> b) {
+  Q4<R> q;
+  q.x = (a.x * b.w) + (a.y * b.z) - (a.z * b.y) + (a.w * b.x);
+  q.y = -(a.x * b.z) + (a.y * b.w) + (a.z * b.x) + (a.w * b.y);
+  q.z = (a.x * b.y) - (a.y * b.x) + (a.z * b.w) + (a.w * b.z);
+  q.w = -(a.x * b.x) - (a.y * b.y) - (a.z * b.z) + (a.w * b.w);
+  return q;
+}
+/* normaliseD, Utilities/Device/Functions.cuh:399-420 */
+template <class R> inline Q4<R> qnormalise(Q4<R> a) {
+  R s = a.w * a.w + a.x * a.x + a.y * a.y + a.z * a.z;
+  R n = s > R(0) ? R(1) / rsqrt_(s) : R(0);
+  return Q4<R>{a.w * n, a.x * n, a.y * n, a.z * n};
+}
+
+template <class R> struct PairHist { int id; V3<R> xi; };
+
+/* Per type-pair constants for the history laws (host-derived, see derive_laws). */
+template <class R> struct LawPair {
+  R kn, kt, beta, mu;     /* LSD */
+  R estar, gstar;         /* Hertz-Mindlin */
+};
+
+template <class R>
+class Oracle {
+ public:
+  /* --- configuration (the constant tables of GPU_Memory_Transfer.cuh:78-85) --- */
+  bdem_SimulationInfo sim{};
+  std::vector<bdem_WOBJ> wobj;
+  std::vector<bdem_POBJ> pobj;
+  bdem_ext_params ext{};
+  bool has_ext = false;
+  int num_wobj_active = 0;           /* Num_WorldObjects, decremented by Hatch */
+  bool cylinder_rotation = false;    /* is_cylinder_rotation (drum_rotation) */
+  uint32_t C = 0;                    /* number of grid cells */
+  LawPair<R> law[8][8]{};
+  LawPair<R> wlaw[8]{};
+
+  /* --- state, in current (sorted) order --- */
+  int N = 0;
+  std::vector<V3<R>> pos, vel, avel;
+  std::vector<Q4<R>> quat;
+  std::vector<R> radius, mass, inv_inertia;
+  std::vector<uint32_t> ptype;
+  std::vector<int> pid;
+  /* history, indexed by P_ID */
+  std::vector<int> hcount, wcount;
+  std::vector<PairHist<R>> hist, whist;
+
+  /* --- per-step work arrays --- */
+  std::vector<uint32_t> keys;
+  std::vector<int> idx;
+  std::vector<uint32_t> cell_start, cell_end;   /* reference format, 0xFFFFFFFF empty */
+  std::vector<V3<R>> f_pp, t_pp, f_w, t_w;
+  int stepc = 0;
+  long long steps_done = 0;
+  int warn_out_of_grid = 0;
+  int warn_list_overflow = 0;
+  int warn_hist_overflow = 0;
+
+  void configure(const bdem_SimulationInfo* s, const bdem_WOBJ* w, int nw, const bdem_POBJ* p,
+                 int np, const bdem_ext_params* e) {
+    sim = *s;
+    wobj.assign(w, w + nw);
+    pobj.assign(p, p + np);
+    num_wobj_active = nw;
+    has_ext = e != nullptr;
+    if (e) ext = *e; else std::memset(&ext, 0, sizeof(ext));
+    C = uint32_t(sim.num_NNCells.x) * uint32_t(sim.num_NNCells.y) * uint32_t(sim.num_NNCells.z);
+    derive_laws();
+  }
+
+  int law_kind() const { return has_ext ? ext.contact_law : BDEM_LAW_REFERENCE; }
+
+  /* Pair and wall constants of the history laws.  Restated from the
+   * textbook forms (no reference counterpart):
+   *   beta = ln e / sqrt(ln^2 e + pi^2)
+   *   Hertz: 1/E* = (1-nu_i^2)/E_i + (1-nu_j^2)/E_j,
+   *          1/G* = 2(2-nu_i)(1+nu_i)/E_i + 2(2-nu_j)(1+nu_j)/E_j
+   * Pair values average e, mu, kn, kt/kn of the two types.  Computed in
+   * double, stored in R (the HIP library stores the same float values). */
+  void derive_laws() {
+    const double pi = 3.14159265358979323846;
+    for (int a = 0; a < 8; ++a) {
+      for (int b = 0; b < 8; ++b) {
+        const bdem_material& ma = ext.material[a];
+        const bdem_material& mb = ext.material[b];
+        double e = 0.5 * (double(ma.restitution) + double(mb.restitution));
+        double beta = 0.0;
+        if (e > 0.0 && e < 1.0) beta = std::log(e) / std::sqrt(std::log(e) * std::log(e) + pi * pi);
+        double kn = 0.5 * (double(ma.kn) + double(mb.kn));
+        double ktr = 0.5 * (double(ma.kt_over_kn) + double(mb.kt_over_kn));
+        double mu = 0.5 * (double(ma.mu) + double(mb.mu));
+        double ea = ma.youngs, eb = mb.youngs, na = ma.poisson, nb = mb.poisson;
+        double estar = 0.0, gstar = 0.0;
+        if (ea > 0 && eb > 0) {
+          estar = 1.0 / ((1.0 - na * na) / ea + (1.0 - nb * nb) / eb);
+          gstar = 1.0 / (2.0 * (2.0 - na) * (1.0 + na) / ea + 2.0 * (2.0 - nb) * (1.0 + nb) / eb);
+        }
+        law[a][b] = LawPair<R>{R(float(kn)), R(float(kn * ktr)), R(float(beta)), R(float(mu)),
+                               R(float(estar)), R(float(gstar))};
+      }
+      const bdem_material& ma = ext.material[a];
+      double e = ma.restitution;
+      double beta = 0.0;
+      if (e > 0.0 && e < 1.0) beta = std::log(e) / std::sqrt(std::log(e) * std::log(e) + pi * pi);
+      double ew = ext.wall_youngs > 0 ? ext.wall_youngs : ma.youngs;
+      double nw = ext.wall_youngs > 0 ? ext.wall_poisson : ma.poisson;
+      double estar = 0.0, gstar = 0.0;
+      if (ma.youngs > 0 && ew > 0) {
+        estar = 1.0 / ((1.0 - ma.poisson * ma.poisson) / ma.youngs + (1.0 - nw * nw) / ew);
+        gstar = 1.0 / (2.0 * (2.0 - ma.poisson) * (1.0 + ma.poisson) / ma.youngs +
+                       2.0 * (2.0 - nw) * (1.0 + nw) / ew);
+      }
+      wlaw[a] = LawPair<R>{R(ma.kn), R(float(double(ma.kn) * ma.kt_over_kn)), R(float(beta)),
+                           R(ma.mu), R(float(estar)), R(float(gstar))};
+    }
+  }
+
+  /* Device_Set_System_State / Set_PSystem_State (GPU_Memory_Transfer.cuh:397-440).
+   * radius_in/mass_in are the per-particle extension; without them the type
+   * table supplies both, exactly as the reference does. */
+  void set_state(int n, const float* p3, const float* v3, const float* w3, const float* q4,
+                 const uint32_t* t, const int* id, const float* radius_in, const float* mass_in) {
+    N = n;
+    pos.resize(n); vel.resize(n); avel.resize(n); quat.resize(n);
+    radius.resize(n); mass.resize(n); inv_inertia.resize(n); ptype.resize(n); pid.resize(n);
+    for (int i = 0; i < n; ++i) {
+      pos[i] = mk<R>(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
+      vel[i] = v3 ? mk<R>(v3[3 * i], v3[3 * i + 1], v3[3 * i + 2]) : mk<R>(0, 0, 0);
+      avel[i] = w3 ? mk<R>(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2]) : mk<R>(0, 0, 0);
+      quat[i] = q4 ? Q4<R>{R(q4[4 * i]), R(q4[4 * i + 1]), R(q4[4 * i + 2]), R(q4[4 * i + 3])}
+                   : Q4<R>{1, 0, 0, 0};
+      ptype[i] = t[i];
+      pid[i] = id[i];
+      const bdem_POBJ& po = pobj[t[i]];
+      if (radius_in) {
+        radius[i] = R(radius_in[i]);
+        mass[i] = R(mass_in[i]);
+        /* per-particle sphere inverse inertia, same float formula as
+         * main.cpp:615-616 evaluated per particle */
+        inv_inertia[i] = R(1) / (R(0.4f) * radius[i] * radius[i] * mass[i]);
+      } else {
+        radius[i] = R(po.radius);
+        mass[i] = R(po.mass);
+        inv_inertia[i] = R(po.InertiaT[0]);
+      }
+    }
+    int maxid = 0;
+    for (int i = 0; i < n; ++i) maxid = std::max(maxid, pid[i] + 1);
+    hcount.assign(std::max(maxid, n), 0);
+    wcount.assign(std::max(maxid, n), 0);
+    hist.assign(size_t(std::max(maxid, n)) * HMAX, PairHist<R>{-1, mk<R>(0, 0, 0)});
+    whist.assign(size_t(std::max(maxid, n)) * WMAX, PairHist<R>{-1, mk<R>(0, 0, 0)});
+    f_pp.assign(n, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp;
+  }
+
+  /* ------------------------------------------------------------------ */
+  /* a1: grid position and hash                                          */
+  /* SpatialDecomp_CalcGridPos  Collision_Detection_BroadPhase.cuh:20-29 */
+  /* SpatialDecomp_CalcGridHash Collision_Detection_BroadPhase.cuh:42-50 */
+  /* ------------------------------------------------------------------ */
+  void grid_pos(V3<R> p, int g[3]) const {
+    g[0] = int(std::floor((p.x - R(sim.worldOrigin.x)) / R(sim.cellSize.x)));
+    g[1] = int(std::floor((p.y - R(sim.worldOrigin.y)) / R(sim.cellSize.y)));
+    g[2] = int(std::floor((p.z - R(sim.worldOrigin.z)) / R(sim.cellSize.z)));
+  }
+  uint32_t hash_of(int gx, int gy, int gz) const {
+    /* (x + z*Nz) + (y*Nx)*Nz in 32-bit two's-complement arithmetic */
+    uint32_t nx = uint32_t(sim.num_NNCells.x), nz = uint32_t(sim.num_NNCells.z);
+    return (uint32_t(gx) + uint32_t(gz) * nz) + (uint32_t(gy) * nx) * nz;
+  }
+  bool in_grid(const int g[3]) const {
+    return g[0] >= 0 && g[1] >= 0 && g[2] >= 0 && g[0] < sim.num_NNCells.x &&
+           g[1] < sim.num_NNCells.y && g[2] < sim.num_NNCells.z;
+  }
+  uint32_t key_of(int i) const {
+    if (pid[i] < 0) return C + 1;                  /* KEY_DEAD */
+    int g[3];
+    grid_pos(pos[i], g);
+    if (!in_grid(g)) return C;                     /* KEY_OUT */
+    uint32_t h = hash_of(g[0], g[1], g[2]);
+    return h < C ? h : C;                          /* aliasing past the table: KEY_OUT */
+  }
+  /* Kernel_SpatialDecomp_CalcPHash, Collision_Detection_BroadPhase.cuh:75-100 */
+  void compute_hash() {
+    keys.resize(N); idx.resize(N);
+    for (int i = 0; i < N; ++i) {
+      keys[i] = key_of(i);
+      idx[i] = i;
+      if (keys[i] == C) warn_out_of_grid = 1;
+    }
+  }
+
+  /* a2: thrust::stable_sort_by_key (Collision_Detection_BroadPhase.cuh:110-117):
+   * ascending key, ties keep ascending original index. */
+  void sort_by_key() {
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return keys[a] < keys[b]; });
+    std::vector<uint32_t> k2(N);
+    std::vector<int> i2(N);
+    for (int i = 0; i < N; ++i) { k2[i] = keys[order[i]]; i2[i] = idx[order[i]]; }
+    keys.swap(k2); idx.swap(i2);
+  }
+
+  /* a3: Kernel_SortArrays + Kernel_SpatialDecomp_ReorderPDA
+   * (Collision_Detection_BroadPhase.cuh:126-268): gather by sorted index. */
+  template <class T> void permute(std::vector<T>& v) const {
+    std::vector<T> t(N);
+    for (int i = 0; i < N; ++i) t[i] = v[idx[i]];
+    v.swap(t);
+  }
+  void reorder() {
+    permute(pos); permute(vel); permute(avel); permute(quat);
+    permute(radius); permute(mass); permute(inv_inertia); permute(ptype); permute(pid);
+  }
+
+  /* a4: cudaMemset(0xFF) + Kernel_SpatialDecomp_BinData
+   * (Device/DeviceInterface.cu:2097-2105, Collision_Detection_BroadPhase.cuh:279-325) */
+  void bin_cells() {
+    cell_start.assign(C, 0xFFFFFFFFu);
+    cell_end.assign(C, 0xFFFFFFFFu);
+    for (int i = 0; i < N; ++i) {
+      uint32_t h = keys[i];
+      if (h >= C) continue;
+      if (i == 0 || keys[i - 1] != h) cell_start[h] = uint32_t(i);
+      if (i == N - 1 || keys[i + 1] != h) cell_end[h] = uint32_t(i + 1);
+    }
+  }
+
+  /* Stencil bounds, Collision_Detection_BroadPhase.cuh:428-459, clamped. */
+  void stencil(const int g[3], int lo[3], int hi[3]) const {
+    const int n[3] = {sim.num_NNCells.x, sim.num_NNCells.y, sim.num_NNCells.z};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::max(g[a] - 1, 0);
+      hi[a] = std::min(g[a] + 1, n[a] - 1);
+    }
+  }
+
+  /* Overlap filter, CollisionDetection_Filter_SphereNNCell
+   * (Collision_Detection_BroadPhase.cuh:369-383): length(pA-pB) - (rA+rB) < 0 */
+  bool overlaps(int i, int j) const {
+    return length(pos[i] - pos[j]) - (radius[i] + radius[j]) < R(0);
+  }
+
+  /* Visit the contacts of particle i in the reference order: y outer, z,
+   * x inner, then ascending sorted index within a cell
+   * (Collision_Detection_BroadPhase.cuh:466-489). */
+  template <class F> int for_each_contact(int i, F&& fn) const {
+    int g[3], lo[3], hi[3];
+    grid_pos(pos[i], g);
+    stencil(g, lo, hi);
+    int count = 0;
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          uint32_t h = hash_of(x, y, z);
+          if (h >= C) continue;
+          uint32_t s = cell_start[h];
+          if (s == 0xFFFFFFFFu) continue;
+          uint32_t e = cell_end[h];
+          for (uint32_t j = s; j < e; ++j) {
+            if (int(j) == i) continue;
+            if (overlaps(i, int(j))) { fn(int(j), count); ++count; }
+          }
+        }
+    return count;
+  }
+
+  /* a5: Kernel_BroadCollisionDetection_NonSymmetry
+   * (Collision_Detection_BroadPhase.cuh:402-513).  The list keeps the first
+   * 32 entries per particle; nn holds the full count, as the reference's
+   * NN_NUM does. */
+  void contact_list(std::vector<uint32_t>& list, std::vector<uint32_t>& nn) {
+    list.assign(size_t(N) * LIST_STRIDE, 0u);
+    nn.assign(N, 0u);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      if (pid[i] < 0) continue;
+      int c = for_each_contact(i, [&](int j, int k) {
+        if (k < LIST_STRIDE) list[size_t(i) * LIST_STRIDE + k] = uint32_t(j);
+      });
+      nn[i] = uint32_t(c);
+    }
+  }
+
+  /* ------------------------------------------------------------------ */
+  /* a6: reference pair law                                              */
+  /* Kernel_ParticleInteraction_Spheres_NonSymmetry DEM_Kernels_Compute.cuh:620-729 */
+  /* Collision_Response_Particle_Particle Collision_Response.cuh:374-494 */
+  /* ------------------------------------------------------------------ */
+  void pair_force_ref(int a, int b, V3<R>& F, V3<R>& T) const {
+    const uint32_t ta = ptype[a], tb = ptype[b];
+    V3<R> relPos = pos[a] - pos[b];
+    V3<R> RVel = vel[a] - vel[b];
+    V3<R> N3 = relPos / length(relPos);
+    R dis = std::fabs(length(relPos) - (radius[a] + radius[b])) + R(0.000010f);
+    V3<R> RA = (R(-1.0f) * N3) * radius[a];
+    V3<R> RB = N3 * radius[b];
+    const bdem_PP_Parms& pp = pobj[ta].PP[tb];
+    V3<R> Vnormal = dot(RVel, N3) * N3;
+    R Fnormal = R(pp.Kn) * dis;
+    V3<R> Fdiss = R(pp.Cn) * Vnormal;
+    V3<R> Ft = Fnormal * N3 - Fdiss;
+    V3<R> V_CPA = vel[a] + cross(avel[a], RA);
+    V3<R> V_CPB = vel[b] + cross(avel[b], RB);
+    V3<R> V_CPNA = dot(V_CPA, N3) * N3;
+    V3<R> V_CPNB = dot(V_CPB, N3) * N3;
+    V3<R> Vt_A = V_CPA - V_CPNA;
+    V3<R> Vt_B = V_CPB - V_CPNB;
+    V3<R> Vt = Vt_A - Vt_B;
+    R mag = dot(Vt, Vt);
+    V3<R> Tang = mk<R>(0, 0, 0);
+    if (mag > R(0)) {
+      mag = rsqrt_(mag);
+      R Velf = dot(RVel, RVel);
+      Velf = Velf > R(0) ? rsqrt_(Velf) : R(1.0f);
+      R tan_f = (((R(0.50f) * (mass[a] + mass[b])) / R(sim.InitalDelta_t)) * R(0.050f)) *
+                R(pp.Fric_kinetic) * (mag / Velf);
+      Tang = (R(-1.0f) * tan_f) * (Vt / mag);
+    }
+    Ft = Ft + Tang;
+    F = F + Ft;
+    if (sim.Rotation) T = T + cross(RA, Ft);
+  }
+
+  /* ------------------------------------------------------------------ */
+  /* History laws (extensions; textbook forms, no reference counterpart)  */
+  /* ------------------------------------------------------------------ */
+  /* Carry a tangential spring onto the current tangent plane, keeping its
+   * length, then add vt*dt. */
+  V3<R> rotate_spring(V3<R> xi_old, V3<R> n, V3<R> vt) const {
+    V3<R> xi = xi_old - dot(xi_old, n) * n;
+    R m_old = dot(xi_old, xi_old);
+    R m_new = dot(xi, xi);
+    if (m_new > R(0)) xi = xi * rsqrt_(m_old / m_new);
+    return xi + vt * R(sim.InitalDelta_t);
+  }
+
+  /* Normal/tangential law shared by pairs and walls.
+   *   n      unit normal pointing at particle a
+   *   delta  overlap (>0)
+   *   vc     contact-point velocity of a relative to the partner
+   *   rstar, mstar  effective radius and mass
+   *   lp     per type-pair constants; xi in/out tangential spring
+   * Returns the force on a. */
+  V3<R> history_law(V3<R> n, R delta, V3<R> vc, R rstar, R mstar, const LawPair<R>& lp,
+                    V3<R>& xi) const {
+    R vn = dot(vc, n);
+    V3<R> vt = vc - vn * n;
+    xi = rotate_spring(xi, n, vt);
+    R fn, kt, gt;
+    if (law_kind() == BDEM_LAW_LSD_COULOMB) {
+      R cn = R(-2.0f) * lp.beta * rsqrt_(lp.kn * mstar);
+      fn = lp.kn * delta - cn * vn;
+      kt = lp.kt;
+      gt = R(0);
+    } else {
+      R srd = rsqrt_(rstar * delta);
+      R sn = R(2.0f) * lp.estar * srd;
+      R st = R(8.0f) * lp.gstar * srd;
+      R gn = R(-1.8257418583505538f) * lp.beta * rsqrt_(sn * mstar);   /* 2 sqrt(5/6) */
+      gt = R(-1.8257418583505538f) * lp.beta * rsqrt_(st * mstar);
+      fn = (R(4.0f) / R(3.0f)) * lp.estar * srd * delta - gn * vn;
+      kt = st;
+    }
+    if (fn < R(0)) fn = R(0);
+    V3<R> ft = R(-1.0f) * kt * xi - gt * vt;
+    R ft2 = dot(ft, ft);
+    R fc = lp.mu * fn;
+    if (ft2 > fc * fc) {
+      R s = fc / rsqrt_(ft2);
+      ft = ft * s;
+      xi = (R(-1.0f) / kt) * ft;
+    }
+    return fn * n + ft;
+  }
+
+  /* Find the spring for partner id in the previous records of particle
+   * `owner` (pair or wall table), 0 if absent. */
+  V3<R> find_spring(const std::vector<PairHist<R>>& tab, const std::vector<int>& cnt,
+                    int stride, int owner, int id) const {
+    if (owner < 0 || owner >= int(cnt.size())) return mk<R>(0, 0, 0);
+    int c = cnt[owner];
+    for (int k = 0; k < c; ++k) {
+      const PairHist<R>& h = tab[size_t(owner) * stride + k];
+      if (h.id == id) return h.xi;
+    }
+    return mk<R>(0, 0, 0);
+  }
+
+  void pair_force_hist(int a, int b, V3<R>& F, V3<R>& T, std::vector<PairHist<R>>& out,
+                       const std::vector<PairHist<R>>& hin, const std::vector<int>& hcin) const {
+    V3<R> rel = pos[a] - pos[b];
+    R d = length(rel);
+    V3<R> n = rel / d;
+    R delta = (radius[a] + radius[b]) - d;
+    V3<R> la = (R(-1.0f) * n) * radius[a];
+    V3<R> lb = n * radius[b];
+    V3<R> vc = (vel[a] + cross(avel[a], la)) - (vel[b] + cross(avel[b], lb));
+    R rstar = (radius[a] * radius[b]) / (radius[a] + radius[b]);
+    R mstar = (mass[a] * mass[b]) / (mass[a] + mass[b]);
+    V3<R> xi = find_spring(hin, hcin, HMAX, pid[a], pid[b]);
+    V3<R> f = history_law(n, delta, vc, rstar, mstar, law[ptype[a]][ptype[b]], xi);
+    F = F + f;
+    if (sim.Rotation) T = T + cross(la, f);
+    if (int(out.size()) < HMAX) out.push_back(PairHist<R>{pid[b], xi});
+  }
+
+  /* ------------------------------------------------------------------ */
+  /* a8: walls (WorldObject_InteractionParticle DEM_Kernels_Compute.cuh:254-610) */
+  /* ------------------------------------------------------------------ */
+  struct Coll { bool hit; R dis; V3<R> cp; V3<R> n; };
+
+  /* is_PointOnSurface, Device/Utilities.cuh:20-53 */
+  bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, V3<R> point) const {
+    R area = R(0);
+    for (int i = 0; i < s.num_vertex - 1; ++i) {
+      V3<R> V1 = point - cvt<R>(w.vertex[s.vertex_Order[i]]);
+      V3<R> V2 = point - cvt<R>(w.vertex[s.vertex_Order[i + 1]]);
+      area = area + R(0.500000f) * length(cross(V1, V2));
+    }
+    V3<R> V1 = point - cvt<R>(w.vertex[s.vertex_Order[s.num_vertex - 1]]);
+    V3<R> V2 = point - cvt<R>(w.vertex[s.vertex_Order[0]]);
+    area = area + R(0.500000f) * length(cross(V1, V2));
+    return (std::fabs(area - R(s.area)) / R(s.area)) * R(100.0000f) <= R(s.Tol);
+  }
+
+  /* Collision_Detection_Sphere_Surface / _Plane (Collision_Detection_Spheres.cuh:6-100) */
+  Coll sphere_plane(int i, V3<R> nrm, V3<R> cen, const bdem_Surface* s, const bdem_WOBJ* w) const {
+    Coll c{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    R r = radius[i];
+    V3<R> Pt = pos[i] - r * nrm;
+    R dis = dot(nrm, Pt - cen);
+    if (dis < R(0.000000f) && dis > -r) {
+      dis = std::fabs(dis);
+      if (s) {
+        V3<R> point = Pt + nrm * dis;
+        if (!point_on_surface(*s, *w, point)) return c;
+      }
+      c.hit = true; c.dis = dis; c.cp = Pt; c.n = nrm;
+    }
+    return c;
+  }
+
+  /* Collision_Detection_Sphere_Cylinder (Collision_Detection_Spheres.cuh:234-265) */
+  Coll sphere_cylinder(int i, const bdem_Macro_Cylinder& cyl) const {
+    Coll c{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    V3<R> Pvec = pos[i] * cvt<R>(cyl.AXIS) - cvt<R>(cyl.center_bot_cap);
+    R Pd = length(Pvec) + radius[i];
+    if (Pd > R(cyl.radius)) {
+      V3<R> n = R(-1.0f) * Pvec;
+      n = n * (R(1.0f) / length(n));
+      c.n = n;
+      c.cp = pos[i] - n * radius[i];
+      c.dis = std::fabs(Pd - R(cyl.radius));
+      c.hit = true;
+    }
+    return c;
+  }
+
+  /* Conical frustum wall (extension).  Closest point of the meridian
+   * segment (s0,r0)-(s1,r1) to the particle centre; contact when closer than
+   * the radius.  Normal points from the wall to the centre. */
+  Coll sphere_cone(int i, const bdem_cone& cn) const {
+    Coll c{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    V3<R> ax = cvt<R>(cn.axis);
+    V3<R> d = pos[i] - cvt<R>(cn.base);
+    R s = dot(d, ax);
+    V3<R> q = d - s * ax;
+    R rho = length(q);
+    if (!(rho > R(0))) return c;
+    R ds = R(cn.s1) - R(cn.s0), dr = R(cn.r1) - R(cn.r0);
+    R t = ((s - R(cn.s0)) * ds + (rho - R(cn.r0)) * dr) / (ds * ds + dr * dr);
+    t = std::min(std::max(t, R(0)), R(1));
+    R sc = R(cn.s0) + t * ds, rc = R(cn.r0) + t * dr;
+    R es = s - sc, er = rho - rc;
+    R dist = rsqrt_(es * es + er * er);
+    if (dist < radius[i] && dist > R(0)) {
+      V3<R> qh = q / rho;
+      V3<R> n = (es / dist) * ax + (er / dist) * qh;
+      c.hit = true;
+      c.dis = radius[i] - dist;
+      c.n = n;
+      c.cp = pos[i] - n * radius[i];
+    }
+    return c;
+  }
+
+  /* Collision_Response_Particle_StaticSurface (Collision_Response.cuh:4-109) */
+  void resp_static(int i, const Coll& c, V3<R>& F, V3<R>& T) const {
+    const bdem_POBJ& P = pobj[ptype[i]];
+    V3<R> PVel = vel[i];
+    V3<R> Rc = c.cp - pos[i];
+    V3<R> Vnormal = dot(PVel, c.n) * c.n;
+    R Fnormal = R(P.surface_Kn) * c.dis;
+    V3<R> Fdiss = R(P.surface_Cn) * Vnormal;
+    V3<R> Ftr = Fnormal * c.n - Fdiss;
+    V3<R> Fdir = Ftr / length(Ftr);
+    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    V3<R> Vt = PVel - Vnormal;
+    if (sim.Rotation) Vt = Vt + cross(avel[i], Rc);
+    R mag = dot(Vt, Vt);
+    V3<R> Tang = mk<R>(0, 0, 0);
+    if (mag > R(0)) {
+      mag = rsqrt_(mag);
+      R Velf = dot(PVel, PVel);
+      Velf = Velf > R(0) ? rsqrt_(Velf) : R(1.0f);
+      R tan_f = (R(0.050f) * (mass[i] / R(sim.InitalDelta_t))) *
+                (R(P.surface_Fric_static) * (mag / Velf));
+      Tang = (R(-1.0f) * tan_f) * (Vt / mag);
+    }
+    Ftr = Ftr + Tang;
+    F = F + Ftr;
+    if (sim.Rotation) T = T + cross(Rc, Ftr);
+  }
+
+  /* Collision_Response_Particle_MovingSurface (Collision_Response.cuh:116-233) */
+  V3<R> resp_moving(int i, const Coll& c, V3<R> Vs, V3<R>& Tout) const {
+    const bdem_POBJ& P = pobj[ptype[i]];
+    V3<R> Rc = c.cp - pos[i];
+    V3<R> RVel = vel[i] - Vs;
+    V3<R> Vn = dot(RVel, c.n) * c.n;
+    R Fnormal = R(P.surface_Kn) * c.dis;
+    V3<R> Fdiss = R(P.surface_Cn) * Vn;
+    V3<R> Ftr = Fnormal * c.n - Fdiss;
+    V3<R> Fdir = Ftr / length(Ftr);
+    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    V3<R> V_CPA = vel[i] + cross(avel[i], Rc);
+    V3<R> V_CPNA = dot(V_CPA, c.n) * c.n;
+    V3<R> Vt_A = V_CPA - V_CPNA;
+    V3<R> Vt_B = Vs - dot(Vs, c.n) * c.n;
+    V3<R> Vt = Vt_A - Vt_B;
+    R mag = dot(Vt, Vt);
+    V3<R> Tang = mk<R>(0, 0, 0);
+    if (mag > R(0)) {
+      mag = rsqrt_(mag);
+      R Velf = dot(RVel, RVel);
+      Velf = Velf > R(0) ? rsqrt_(Velf) : R(1.0f);
+      R tan_f = R(P.surface_Fric_kinetic) * (mag / Velf);
+      Tang = ((R(-1.0f) * tan_f) * (Vt / mag)) * (mass[i] / R(sim.InitalDelta_t)) * R(0.050f);
+    }
+    Ftr = Ftr + Tang;
+    Tout = sim.Rotation ? cross(Rc, Ftr) : mk<R>(0, 0, 0);
+    return Ftr;
+  }
+
+  /* Collision_Response_EndPlate_Particle (Collision_Response.cuh:504-526) */
+  V3<R> resp_endplate(int i, const Coll& c) const {
+    const bdem_POBJ& P = pobj[ptype[i]];
+    V3<R> Vn = dot(vel[i], c.n) * c.n;
+    V3<R> Fdiss = R(P.surface_Cn) * Vn;
+    R Fn = R(P.surface_Kn) * c.dis;
+    return Fn * c.n - Fdiss;
+  }
+
+  /* Drum surface velocity, DEM_Kernels_Compute.cuh:375-378 and 449-452 */
+  V3<R> drum_velocity(V3<R> cp) const {
+    R Rd = R(wobj[0].cyl_geo.radius);
+    return cross(mk<R>(R(0.0), R(0.0), -R(sim.MillRotVel)), cp - mk<R>(Rd, Rd, R(0.0f)));
+  }
+
+  /* History-law wall contact: effective radius r, mass m, wall velocity vw. */
+  void wall_hist(int i, const Coll& c, V3<R> vw, int wall_id, V3<R>& F, V3<R>& T,
+                 std::vector<PairHist<R>>& wout) const {
+    V3<R> la = c.cp - pos[i];
+    V3<R> vc = (vel[i] + cross(avel[i], la)) - vw;
+    V3<R> xi = find_spring(whist, wcount, WMAX, pid[i], wall_id);
+    V3<R> f = history_law(c.n, c.dis, vc, radius[i], mass[i], wlaw[ptype[i]], xi);
+    F = F + f;
+    if (sim.Rotation) T = T + cross(la, f);
+    if (int(wout.size()) < WMAX) wout.push_back(PairHist<R>{wall_id, xi});
+  }
+
+  /* Wall force and torque on particle i.  Reference law reproduces the
+   * accumulate/overwrite pattern of DEM_Kernels_Compute.cuh:283-604: plane
+   * hits add, cylinder and cap hits assign the force (torques add). */
+  void wall_forces(int i, V3<R>& F, V3<R>& T, std::vector<PairHist<R>>* wout) const {
+    F = mk<R>(0, 0, 0);
+    T = mk<R>(0, 0, 0);
+    const bool use_hist = law_kind() != BDEM_LAW_REFERENCE;
+    const bool moving_planes = sim.Rotating_WObject && cylinder_rotation;
+    for (int k = 0; k < num_wobj_active; ++k) {
+      const bdem_WOBJ& w = wobj[k];
+      if (w.surface_type == BDEM_SURFACE_PLANE) {
+        for (int s = 0; s < w.num_surfaces; ++s) {
+          const bdem_Surface& S = w.surfaces[s];
+          Coll c = (S.area == -1.0f) ? sphere_plane(i, cvt<R>(S.normal), cvt<R>(S.centroid), nullptr, nullptr)
+                                     : sphere_plane(i, cvt<R>(S.normal), cvt<R>(S.centroid), &S, &w);
+          if (!c.hit) continue;
+          if (use_hist) {
+            V3<R> vw = moving_planes ? drum_velocity(c.cp) : mk<R>(0, 0, 0);
+            wall_hist(i, c, vw, k * 16 + s, F, T, *wout);
+          } else if (moving_planes) {
+            V3<R> tq;
+            F = F + resp_moving(i, c, drum_velocity(c.cp), tq);
+            T = T + tq;
+          } else {
+            resp_static(i, c, F, T);
+          }
+        }
+      } else if (w.surface_type == BDEM_SURFACE_CYLINDER) {
+        const bdem_Macro_Cylinder& cyl = w.cyl_geo;
+        Coll c = sphere_cylinder(i, cyl);
+        if (c.hit) {
+          if (use_hist) {
+            V3<R> vw = cylinder_rotation ? drum_velocity(c.cp) : mk<R>(0, 0, 0);
+            wall_hist(i, c, vw, 128 + k, F, T, *wout);
+          } else if (cylinder_rotation) {
+            V3<R> tq;
+            F = resp_moving(i, c, drum_velocity(c.cp), tq);
+            T = T + tq;
+          } else {
+            V3<R> f = mk<R>(0, 0, 0), tq = mk<R>(0, 0, 0);
+            resp_static(i, c, f, tq);
+            F = f;
+            T = T + tq;
+          }
+        }
+        if (cyl.has_top_cap) {
+          Coll cc = sphere_plane(i, cvt<R>(cyl.normal_top_cap), cvt<R>(cyl.center_top_cap), nullptr, nullptr);
+          if (cc.hit) {
+            if (use_hist) wall_hist(i, cc, mk<R>(0, 0, 0), 144 + k, F, T, *wout);
+            else F = resp_endplate(i, cc);
+          }
+        }
+        if (cyl.has_bot_cap) {
+          Coll cc = sphere_plane(i, cvt<R>(cyl.normal_bot_cap), cvt<R>(cyl.center_bot_cap), nullptr, nullptr);
+          if (cc.hit) {
+            if (use_hist) wall_hist(i, cc, mk<R>(0, 0, 0), 160 + k, F, T, *wout);
+            else F = resp_endplate(i, cc);
+          }
+        }
+      }
+    }
+    if (has_ext) {
+      for (int k = 0; k < ext.num_cones; ++k) {
+        Coll c = sphere_cone(i, ext.cone[k]);
+        if (!c.hit) continue;
+        if (use_hist) {
+          wall_hist(i, c, mk<R>(0, 0, 0), 176 + k, F, T, *wout);
+        } else {
+          resp_static(i, c, F, T);
+        }
+      }
+    }
+  }
+
+  /* Pair forces (including gravity, DEM_Kernels_Compute.cuh:654) and wall
+   * forces of every live particle on the sorted state. */
+  void compute_forces() {
+    f_pp.assign(N, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp;
+    const bool use_hist = law_kind() != BDEM_LAW_REFERENCE;
+    std::vector<int> hc_new, wc_new;
+    std::vector<PairHist<R>> h_new, w_new;
+    if (use_hist) {
+      hc_new.assign(hcount.size(), 0);
+      wc_new.assign(wcount.size(), 0);
+      h_new.assign(hist.size(), PairHist<R>{-1, mk<R>(0, 0, 0)});
+      w_new.assign(whist.size(), PairHist<R>{-1, mk<R>(0, 0, 0)});
+    }
+    int hov = 0;
+#pragma omp parallel for schedule(static) reduction(| : hov)
+    for (int i = 0; i < N; ++i) {
+      if (pid[i] < 0) continue;
+      V3<R> F = cvt<R>(sim.Force_Field) * mass[i];
+      V3<R> T = mk<R>(0, 0, 0);
+      std::vector<PairHist<R>> out, wout;
+      int n = for_each_contact(i, [&](int j, int) {
+        if (use_hist) pair_force_hist(i, j, F, T, out, hist, hcount);
+        else pair_force_ref(i, j, F, T);
+      });
+      if (use_hist && n > HMAX) hov = 1;
+      if (!sim.Rotation) T = mk<R>(0, 0, 0);
+      f_pp[i] = F;
+      t_pp[i] = T;
+      V3<R> Fw, Tw;
+      wall_forces(i, Fw, Tw, &wout);
+      f_w[i] = Fw;
+      t_w[i] = Tw;
+      if (use_hist) {
+        int p = pid[i];
+        hc_new[p] = int(out.size());
+        for (size_t k = 0; k < out.size(); ++k) h_new[size_t(p) * HMAX + k] = out[k];
+        wc_new[p] = int(wout.size());
+        for (size_t k = 0; k < wout.size(); ++k) w_new[size_t(p) * WMAX + k] = wout[k];
+      }
+    }
+    if (hov) warn_hist_overflow = 1;
+    if (use_hist) {
+      hcount.swap(hc_new); hist.swap(h_new);
+      wcount.swap(wc_new); whist.swap(w_new);
+    }
+  }
+
+  /* a10: Integrate_Euler_NonSymmetry (Integration.cuh:6-184), spheres. */
+  void integrate() {
+    const R dt = R(sim.InitalDelta_t);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+      if (pid[i] < 0) continue;
+      V3<R> acc = f_w[i] + f_pp[i];
+      acc = acc * (R(1.0f) / mass[i]);
+      vel[i] = vel[i] + acc * dt;
+      pos[i] = pos[i] + vel[i] * dt;
+      if (sim.Mode2D) vel[i].z = R(0);
+      V3<R> fl = t_pp[i] + t_w[i];
+      if (sim.Rotation) {
+        if (dot(fl, fl) > R(0)) {
+          V3<R> aa = inv_inertia[i] * fl;
+          avel[i] = avel[i] + aa * dt;
+          avel[i] = avel[i] - avel[i] * R(sim.Roll_Res);
+        }
+        if (sim.sphere_orient) {
+          V3<R> om = avel[i];
+          R mo = dot(om, om);
+          Q4<R> dQ{1, 0, 0, 0};
+          if (mo > R(0.000000f)) {
+            mo = rsqrt_(mo);
+            V3<R> dir = om / mo;
+            dQ.w = std::cos(R(0.5000f) * mo * dt);
+            R sv = std::sin(R(0.5000f) * mo * dt);
+            dQ.x = sv * dir.x; dQ.y = sv * dir.y; dQ.z = sv * dir.z;
+          }
+          Q4<R> Q = qnormalise(qmul(dQ, quat[i]));
+          quat[i] = (Q.w <= R(-1.00000f)) ? dQ : Q;
+          if (sim.Mode2D) { quat[i].x = R(0); quat[i].y = R(0); }
+        }
+        if (sim.Mode2D) { avel[i].x = R(0); avel[i].y = R(0); }
+      }
+    }
+  }
+
+  /* a11: Remove_Silo_Discharged_Particles (GPU_Memory_Transfer.cuh:650-664),
+   * every kill_check_interval steps (Device/DeviceInterface.cu:2466-2499). */
+  int kill_discharged() {
+    int killed = 0;
+    for (int i = 0; i < N; ++i) {
+      if (pid[i] == -1) continue;
+      if (pos[i].y < R(sim.Silo_Kill_Height)) {
+        pos[i] = cvt<R>(sim.max_size);
+        pid[i] = -1;
+        ++killed;
+      }
+    }
+    return killed;
+  }
+
+  /* Hash, sort, reorder and bin: the broad-phase front half of a step. */
+  void prepare() {
+    compute_hash();
+    sort_by_key();
+    reorder();
+    bin_cells();
+  }
+
+  /* One Device_DEM_UpdateSim. */
+  void step() {
+    prepare();
+    compute_forces();
+    integrate();
+    if (sim.Kill_Particles) {
+      ++stepc;
+      int interval = (has_ext && ext.kill_check_interval > 0) ? ext.kill_check_interval : 1000;
+      if (stepc >= interval) {
+        kill_discharged();
+        stepc = 0;
+      }
+    }
+    ++steps_done;
+  }
+};
+
+}  // namespace bdem_oracle

@@ -1,0 +1,148 @@
+"""ctypes wrapper of the CPU restatement (libbdem_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, the smoke check and the
+CPU-baseline leg of the benchmark, never by the product.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbdem_oracle.so")
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle not built at {LIB_PATH}: run make -C oracle")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    dp = C.POINTER(C.c_double)
+    up = C.POINTER(C.c_uint)
+    ip = C.POINTER(C.c_int)
+    lib.bo_create.restype = P
+    lib.bo_create.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]
+    lib.bo_destroy.argtypes = [P]
+    lib.bo_set_state.argtypes = [P, C.c_int, fp, fp, fp, fp, up, ip, fp, fp]
+    lib.bo_get_state.argtypes = [P, dp, dp, dp, dp, up, ip]
+    lib.bo_num_cells.restype = C.c_longlong
+    lib.bo_num_cells.argtypes = [P]
+    lib.bo_hash.argtypes = [P, up, ip]
+    lib.bo_sorted.argtypes = [P, up, ip]
+    lib.bo_cells.argtypes = [P, up, up]
+    lib.bo_contacts.argtypes = [P, up, up]
+    lib.bo_forces.argtypes = [P, dp, dp, dp, dp, ip]
+    lib.bo_step.argtypes = [P, C.c_int]
+    lib.bo_time_steps.restype = C.c_double
+    lib.bo_time_steps.argtypes = [P, C.c_int]
+    lib.bo_set_active_world_objects.argtypes = [P, C.c_int]
+    lib.bo_set_drum_rotation.argtypes = [P, C.c_int]
+    lib.bo_warnings.argtypes = [P]
+    lib.bo_history.argtypes = [P, C.c_int, ip, dp]
+    _lib = lib
+    return lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Oracle:
+    """CPU restatement of one simulation in float (bits=32) or double (bits=64)."""
+
+    def __init__(self, deck, bits=32):
+        import ctypes
+        self.lib = load()
+        self.deck = deck
+        self._w = (deck["wobj"][0].__class__ * len(deck["wobj"]))(*deck["wobj"])
+        self._p = (deck["pobj"][0].__class__ * len(deck["pobj"]))(*deck["pobj"])
+        ext = deck.get("ext")
+        self.h = self.lib.bo_create(bits, ctypes.byref(deck["sim"]), self._w, len(deck["wobj"]),
+                                    self._p, len(deck["pobj"]),
+                                    ctypes.byref(ext) if ext is not None else None)
+        self.n = deck["sim"].Num_Particles
+        self.bits = bits
+        self.set_state(deck)
+
+    def set_state(self, deck):
+        f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        self._keep = [f(deck["pos"]), f(deck["vel"]), f(deck.get("avel")), f(deck.get("quat")),
+                      np.ascontiguousarray(deck["ptype"], np.uint32),
+                      np.ascontiguousarray(deck["pid"], np.int32), f(deck.get("radius")),
+                      f(deck.get("mass"))]
+        k = self._keep
+        self.lib.bo_set_state(self.h, self.n, _p(k[0], C.c_float), _p(k[1], C.c_float),
+                              _p(k[2], C.c_float), _p(k[3], C.c_float), _p(k[4], C.c_uint),
+                              _p(k[5], C.c_int), _p(k[6], C.c_float), _p(k[7], C.c_float))
+
+    def close(self):
+        if self.h:
+            self.lib.bo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def state(self):
+        n = self.n
+        pos = np.zeros((n, 3)); vel = np.zeros((n, 3)); avel = np.zeros((n, 3)); quat = np.zeros((n, 4))
+        pt = np.zeros(n, np.uint32); pid = np.zeros(n, np.int32)
+        self.lib.bo_get_state(self.h, _p(pos, C.c_double), _p(vel, C.c_double), _p(avel, C.c_double),
+                              _p(quat, C.c_double), _p(pt, C.c_uint), _p(pid, C.c_int))
+        return dict(pos=pos, vel=vel, avel=avel, quat=quat, ptype=pt, pid=pid)
+
+    def num_cells(self):
+        return int(self.lib.bo_num_cells(self.h))
+
+    def hash(self):
+        k = np.zeros(self.n, np.uint32); i = np.zeros(self.n, np.int32)
+        self.lib.bo_hash(self.h, _p(k, C.c_uint), _p(i, C.c_int))
+        return k, i
+
+    def sorted(self):
+        k = np.zeros(self.n, np.uint32); i = np.zeros(self.n, np.int32)
+        self.lib.bo_sorted(self.h, _p(k, C.c_uint), _p(i, C.c_int))
+        return k, i
+
+    def cells(self):
+        nc = self.num_cells()
+        s = np.zeros(nc, np.uint32); e = np.zeros(nc, np.uint32)
+        self.lib.bo_cells(self.h, _p(s, C.c_uint), _p(e, C.c_uint))
+        return s, e
+
+    def contacts(self):
+        l = np.zeros(self.n * 32, np.uint32); nn = np.zeros(self.n, np.uint32)
+        self.lib.bo_contacts(self.h, _p(l, C.c_uint), _p(nn, C.c_uint))
+        return l.reshape(self.n, 32), nn
+
+    def forces(self):
+        out = [np.zeros((self.n, 3)) for _ in range(4)]
+        pid = np.zeros(self.n, np.int32)
+        self.lib.bo_forces(self.h, *[_p(a, C.c_double) for a in out], _p(pid, C.c_int))
+        return out[0], out[1], out[2], out[3], pid
+
+    def step(self, n=1):
+        self.lib.bo_step(self.h, int(n))
+
+    def time_steps(self, n):
+        return float(self.lib.bo_time_steps(self.h, int(n)))
+
+    def set_active_world_objects(self, k):
+        self.lib.bo_set_active_world_objects(self.h, int(k))
+
+    def warnings(self):
+        return int(self.lib.bo_warnings(self.h))
+
+    def history(self, pid):
+        ids = np.zeros(16, np.int32); xi = np.zeros((16, 3))
+        c = self.lib.bo_history(self.h, int(pid), _p(ids, C.c_int), _p(xi, C.c_double))
+        return ids[:max(c, 0)], xi[:max(c, 0)]

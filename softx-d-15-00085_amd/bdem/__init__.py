@@ -1,0 +1,8 @@
+"""Python binding of the MI355X DEM step library (libbdem.so).
+
+types      ctypes mirrors of the reference's boundary structs
+lib        the C ABI and a KDevice-style Device wrapper
+scenarios  seeded synthetic decks (box fill, silo, dense random packs)
+"""
+from . import types, lib, scenarios  # noqa: F401
+from .lib import Device, load  # noqa: F401

@@ -1,0 +1,243 @@
+"""ctypes binding of libbdem.so (include/bdem_abi.h).
+
+`Device` mirrors the reference's KDevice facade (Interface/KDevice.h:17-99):
+one object per simulation, constructed from the object tables, with
+IDevice_*-style methods mapped onto the C ABI.  Arrays are numpy, in the
+reference layout (AoS float3, Quaterion w,x,y,z).  There is no CPU fallback:
+constructing a Device without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import types as T
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libbdem.so")
+
+BDEM_OK = 0
+WARN_OUT_OF_GRID = 1
+WARN_LIST_OVERFLOW = 2
+WARN_HIST_OVERFLOW = 4
+
+_lib = None
+
+_P = C.c_void_p
+_f3p = C.POINTER(T.Float3)
+_qp = C.POINTER(T.Quaterion)
+_up = C.POINTER(C.c_uint)
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+
+_PROTOS = {
+    "bdem_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "bdem_destroy": (C.c_int, [_P]),
+    "bdem_last_error": (C.c_char_p, [_P]),
+    "bdem_set_simulation_data": (C.c_int, [_P, C.POINTER(T.WOBJ), C.POINTER(T.POBJ), C.POINTER(T.DOBJ),
+                                           C.POINTER(T.SimulationInfo), C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(T.InitConfig)]),
+    "bdem_set_ext_params": (C.c_int, [_P, C.POINTER(T.ExtParams)]),
+    "bdem_set_pos": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
+    "bdem_set_pos_velocity": (C.c_int, [_P, _f3p, _qp, _f3p, _f3p, _up, _ip]),
+    "bdem_set_system_state": (C.c_int, [_P, _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip]),
+    "bdem_set_radii": (C.c_int, [_P, _fp, _fp]),
+    "bdem_set_tallys": (C.c_int, [_P, _fp]),
+    "bdem_get_system_state": (C.c_int, [_P, _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip]),
+    "bdem_get_positions": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
+    "bdem_get_velocity": (C.c_int, [_P, _f3p, _f3p, C.c_int, _up, _ip]),
+    "bdem_get_wall_forces": (C.c_int, [_P, _f3p, _f3p]),
+    "bdem_record_wall_forces": (C.c_int, [_P, C.c_int]),
+    "bdem_get_energy": (C.c_int, [_P, _fp]),
+    "bdem_get_status": (C.c_int, [_P, _up]),
+    "bdem_get_kill_count": (C.c_int, [_P, _ip]),
+    "bdem_num_particles": (C.c_int, [_P]),
+    "bdem_num_cells": (C.c_longlong, [_P]),
+    "bdem_update_dobject_positions": (C.c_int, [_P, C.POINTER(T.DOBJ)]),
+    "bdem_update_world_object_positions": (C.c_int, [_P, C.POINTER(T.WOBJ)]),
+    "bdem_open_hatch": (C.c_int, [_P, C.POINTER(T.WOBJ), C.c_int]),
+    "bdem_reset_energy": (C.c_int, [_P, C.c_int]),
+    "bdem_drum_rotation": (C.c_int, [_P, C.c_int]),
+    "bdem_start_bin": (C.c_int, [_P]),
+    "bdem_add_force": (C.c_int, [_P]),
+    "bdem_sub_force": (C.c_int, [_P]),
+    "bdem_zero_force": (C.c_int, [_P]),
+    "bdem_update_sim": (C.c_int, [_P]),
+    "bdem_step": (C.c_int, [_P, C.c_int]),
+    "bdem_sync": (C.c_int, [_P]),
+    "bdem_debug_hash": (C.c_int, [_P, _up, _ip]),
+    "bdem_debug_sorted": (C.c_int, [_P, _up, _ip]),
+    "bdem_debug_cells": (C.c_int, [_P, _up, _up]),
+    "bdem_debug_contacts": (C.c_int, [_P, _up, _up]),
+    "bdem_debug_forces": (C.c_int, [_P, _fp, _fp, _fp, _fp, _ip]),
+}
+
+EXPORTED = tuple(_PROTOS)
+
+
+def load(path=LIB_PATH):
+    """Load libbdem.so and declare every prototype; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libbdem.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class BdemError(RuntimeError):
+    pass
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def _f3(a):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+class Device:
+    """One simulation context on one GPU (KDevice, Interface/KDevice.h:17-99)."""
+
+    def __init__(self, wobj, pobj, sim, init=None, dobj=None, device=0):
+        self.lib = load()
+        self.h = _P()
+        rc = self.lib.bdem_create(C.byref(self.h), int(device))
+        if rc != BDEM_OK:
+            raise BdemError(f"bdem_create failed ({rc})")
+        self._wobj = (T.WOBJ * max(len(wobj), 1))(*wobj)
+        self._pobj = (T.POBJ * len(pobj))(*pobj)
+        self._dobj = (T.DOBJ * max(len(dobj or []), 1))(*(dobj or []))
+        self.sim = sim
+        self.init = init if init is not None else T.InitConfig()
+        self._check(self.lib.bdem_set_simulation_data(
+            self.h, self._wobj, self._pobj, self._dobj, C.byref(sim), len(wobj), len(pobj),
+            len(dobj or []), C.byref(self.init)))
+        self.n = sim.Num_Particles
+
+    def _check(self, rc):
+        if rc != BDEM_OK:
+            msg = self.lib.bdem_last_error(self.h).decode()
+            raise BdemError(f"bdem error {rc}: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.bdem_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup ------------------------------------------------------------
+    def set_ext(self, ext):
+        self._ext = ext
+        self._check(self.lib.bdem_set_ext_params(self.h, C.byref(ext)))
+
+    def set_state(self, pos, ptype, pid, vel=None, avel=None, quat=None):
+        pos = _f3(pos)
+        ptype = np.ascontiguousarray(ptype, dtype=np.uint32)
+        pid = np.ascontiguousarray(pid, dtype=np.int32)
+        q = None if quat is None else np.ascontiguousarray(quat, dtype=np.float32)
+        if vel is None:
+            self._check(self.lib.bdem_set_pos(self.h, _ptr(pos, T.Float3), _ptr(q, T.Quaterion),
+                                              _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
+        else:
+            vel = _f3(vel)
+            avel = _f3(avel if avel is not None else np.zeros_like(vel))
+            self._check(self.lib.bdem_set_pos_velocity(
+                self.h, _ptr(pos, T.Float3), _ptr(q, T.Quaterion), _ptr(vel, T.Float3),
+                _ptr(avel, T.Float3), _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
+
+    def set_radii(self, radius, mass):
+        r = np.ascontiguousarray(radius, dtype=np.float32)
+        m = np.ascontiguousarray(mass, dtype=np.float32)
+        self._check(self.lib.bdem_set_radii(self.h, _ptr(r, C.c_float), _ptr(m, C.c_float)))
+
+    # -- state out --------------------------------------------------------
+    def get_state(self):
+        n = self.n
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        avel = np.zeros((n, 3), np.float32)
+        quat = np.zeros((n, 4), np.float32)
+        ptype = np.zeros(n, np.uint32)
+        pid = np.zeros(n, np.int32)
+        self._check(self.lib.bdem_get_system_state(
+            self.h, _ptr(pos, T.Float3), _ptr(quat, T.Quaterion), _ptr(vel, T.Float3),
+            _ptr(avel, T.Float3), None, _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
+        return dict(pos=pos, vel=vel, avel=avel, quat=quat, ptype=ptype, pid=pid)
+
+    def status(self):
+        w = C.c_uint(0)
+        self._check(self.lib.bdem_get_status(self.h, C.byref(w)))
+        return w.value
+
+    def kill_count(self):
+        k = C.c_int(0)
+        self._check(self.lib.bdem_get_kill_count(self.h, C.byref(k)))
+        return k.value
+
+    def num_cells(self):
+        return int(self.lib.bdem_num_cells(self.h))
+
+    # -- hot path ---------------------------------------------------------
+    def update_sim(self):
+        self._check(self.lib.bdem_update_sim(self.h))
+
+    def step(self, nsteps):
+        self._check(self.lib.bdem_step(self.h, int(nsteps)))
+
+    def sync(self):
+        self._check(self.lib.bdem_sync(self.h))
+
+    def open_hatch(self):
+        self._check(self.lib.bdem_open_hatch(self.h, None, 0))
+
+    def drum_rotation(self, on):
+        self._check(self.lib.bdem_drum_rotation(self.h, 1 if on else 0))
+
+    # -- stage outputs ----------------------------------------------------
+    def debug_hash(self):
+        k = np.zeros(self.n, np.uint32)
+        i = np.zeros(self.n, np.int32)
+        self._check(self.lib.bdem_debug_hash(self.h, _ptr(k, C.c_uint), _ptr(i, C.c_int)))
+        return k, i
+
+    def debug_sorted(self):
+        k = np.zeros(self.n, np.uint32)
+        i = np.zeros(self.n, np.int32)
+        self._check(self.lib.bdem_debug_sorted(self.h, _ptr(k, C.c_uint), _ptr(i, C.c_int)))
+        return k, i
+
+    def debug_cells(self):
+        nc = self.num_cells()
+        s = np.zeros(nc, np.uint32)
+        e = np.zeros(nc, np.uint32)
+        self._check(self.lib.bdem_debug_cells(self.h, _ptr(s, C.c_uint), _ptr(e, C.c_uint)))
+        return s, e
+
+    def debug_contacts(self):
+        lst = np.zeros(self.n * 32, np.uint32)
+        nn = np.zeros(self.n, np.uint32)
+        self._check(self.lib.bdem_debug_contacts(self.h, _ptr(lst, C.c_uint), _ptr(nn, C.c_uint)))
+        return lst.reshape(self.n, 32), nn
+
+    def debug_forces(self):
+        out = [np.zeros((self.n, 3), np.float32) for _ in range(4)]
+        pid = np.zeros(self.n, np.int32)
+        self._check(self.lib.bdem_debug_forces(self.h, *[_ptr(a, C.c_float) for a in out],
+                                               _ptr(pid, C.c_int)))
+        return out[0], out[1], out[2], out[3], pid

@@ -1,0 +1,388 @@
+"""Seeded synthetic decks in the reference's units (cm, kg, s).
+
+Builds the object tables a reference caller would pass to
+Device_Set_SimulationData (Device/DeviceInterface.h:16-19) plus the initial
+particle state:
+
+  box_deck   8,192 monodisperse spheres in a closed 50 cm cube (box fill)
+  silo_deck  1,048,576 polydisperse spheres in a cylindrical silo over a
+             30-degree conical hopper with a closed 6 cm outlet
+  random_deck  small dense random packs for parity tests
+
+Derived quantities follow the reference's host code: wall normals, areas and
+centroids as ReadWObject computes them (Input/KSimulationData.cpp:841-875),
+sphere mass 4/3 pi r^3 rho 1e-6 (Input/KSimulationData.cpp:1096), damping
+from restitution (Set_SpringConstantsParticleKN / Surface_KN,
+Input/KSimulationData.cpp:950-1045), inverse inertia 1/(0.4 r^2 m)
+(main.cpp:615-616), grid size ceil((worldSize - origin)/cell)
+(main.cpp:563-569), max_size (main.cpp:820-822).
+
+Random numbers: splitmix64, one stream per quantity, uniform float
+(z >> 40) * 2**-24.
+"""
+import math
+
+import numpy as np
+
+from . import types as T
+
+GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+STREAM_POS, STREAM_JITTER, STREAM_VEL, STREAM_RADIUS, STREAM_SHAPE, STREAM_MATERIAL = range(6)
+
+
+def splitmix64(seed, stream, n):
+    """n consecutive splitmix64 outputs of stream `stream` of `seed`."""
+    with np.errstate(over="ignore"):
+        base = np.uint64((seed * 64 + stream) & 0xFFFFFFFFFFFFFFFF)
+        z = base + GOLDEN * (np.arange(1, n + 1, dtype=np.uint64))
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, stream, n, lo=0.0, hi=1.0):
+    z = splitmix64(seed, stream, n)
+    u = (z >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+    return lo + (hi - lo) * u
+
+
+# ---------------------------------------------------------------------------
+# reference host derivations
+# ---------------------------------------------------------------------------
+def sphere_mass(radius_cm, density):
+    """Input/KSimulationData.cpp:1096 (double, stored as float)."""
+    return np.float32((4.0 / 3.0) * 2.0 * math.asin(1.0) * radius_cm ** 3 * density * 1e-6)
+
+
+def cn_from_cor(kn, m, cor):
+    """Damping from restitution, Input/KSimulationData.cpp:950-1045 (double)."""
+    if cor >= 1.0:
+        return 0.0
+    if 0.30 <= cor <= 0.90:
+        cor -= 0.038080
+    pi = 2.0 * math.asin(1.0)
+    return -2.0 * math.log(cor) * math.sqrt(kn * m) / math.sqrt(math.log(cor) ** 2 + pi * pi)
+
+
+def inertia_inv(radius, mass):
+    """main.cpp:615-616 in float arithmetic."""
+    r = np.float32(radius)
+    m = np.float32(mass)
+    return np.float32(np.float32(1.0) / (np.float32(np.float32(np.float32(0.40) * r) * r) * m))
+
+
+def sphere_pobj(radius, density, cor, kn, mu, cor_s=None, kn_s=None, mu_s=None):
+    """A sphere POBJ as ReadPObject_Sphere + Set_SpringConstants*_KN build it
+    (ForceSpec 1, same-type pair: reduced mass m/2, Input/KSimulationData.cpp:958)."""
+    p = T.POBJ()
+    p.radius = float(radius)
+    p.mass = float(sphere_mass(radius, density))
+    m = float(np.float32(p.mass))
+    kn_s = kn if kn_s is None else kn_s
+    cor_s = cor if cor_s is None else cor_s
+    mu_s = mu if mu_s is None else mu_s
+    p.PP[0].Kn = kn
+    p.PP[0].Cn = cn_from_cor(float(np.float32(kn)), 1.0 / (1.0 / m + 1.0 / m), cor)
+    p.PP[0].Fric_static = mu
+    p.PP[0].Fric_kinetic = mu
+    p.surface_Kn = kn_s
+    p.surface_Cn = cn_from_cor(float(np.float32(kn_s)), m, cor_s)
+    p.surface_Fric_static = mu_s
+    p.surface_Fric_kinetic = mu_s
+    p.Dsurface_Kn = kn_s
+    p.Dsurface_Cn = p.surface_Cn
+    p.Dsurface_Fric_static = mu_s
+    p.Dsurface_Fric_kinetic = mu_s
+    p.InertiaT[0] = float(inertia_inv(p.radius, p.mass))
+    return p
+
+
+def plane_wobj(vertices, surfaces, tol=1.0):
+    """A planar WOBJ as ReadWObject derives it (Input/KSimulationData.cpp:712-875):
+    quad normal = normalise(cross(V2, V1)), V1 = v1-v0, V2 = v2-v1; area = sum of
+    the two triangle halves; centroid = (1/nv) * sum, all in float arithmetic.
+    `surfaces` are 0-based vertex orders."""
+    f = np.float32
+    w = T.WOBJ()
+    w.surface_type = T.SURFACE_PLANE
+    w.num_vertex = len(vertices)
+    V = np.asarray(vertices, dtype=np.float32)
+    for k, v in enumerate(V):
+        w.vertex[k] = T.f3(*v)
+    w.num_surfaces = len(surfaces)
+    for j, order in enumerate(surfaces):
+        s = w.surfaces[j]
+        s.Tol = tol
+        s.num_vertex = len(order)
+        total = np.zeros(3, np.float32)
+        for q, o in enumerate(order):
+            s.vertex_Order[q] = o
+            total = (total + V[o]).astype(np.float32)
+        if len(order) == 3:
+            V1 = V[order[2]] - V[order[0]]
+            V2 = V[order[1]] - V[order[2]]
+            n = np.cross(V2, V1).astype(np.float32)
+            ln = f(np.sqrt(f(np.dot(n, n))))
+            area = f(f(0.5) * ln)
+        else:
+            V1 = (V[order[1]] - V[order[0]]).astype(np.float32)
+            V2 = (V[order[2]] - V[order[1]]).astype(np.float32)
+            V3 = (V[order[3]] - V[order[2]]).astype(np.float32)
+            V4 = (V[order[0]] - V[order[3]]).astype(np.float32)
+            n = np.cross(V2, V1).astype(np.float32)
+            ln = f(np.sqrt(f(np.dot(n, n))))
+            n2 = np.cross(V3, V4).astype(np.float32)
+            area = f(f(f(0.5) * ln) + f(f(0.5) * f(np.sqrt(f(np.dot(n2, n2))))))
+        nn = (n / ln).astype(np.float32)
+        s.normal = T.f3(*nn)
+        s.area = float(area)
+        c = (f(1.0) / f(len(order))) * total
+        s.centroid = T.f3(*c.astype(np.float32))
+    return w
+
+
+def infinite_plane_wobj(point, normal):
+    """One infinite plane (area == -1 selects Collision_Detection_Sphere_Plane,
+    DEM_Kernels_Compute.cuh:319-330); vertex[0] is the plane point."""
+    w = T.WOBJ()
+    w.surface_type = T.SURFACE_PLANE
+    w.num_vertex = 1
+    w.vertex[0] = T.f3(*point)
+    w.num_surfaces = 1
+    s = w.surfaces[0]
+    s.num_vertex = 0
+    s.area = -1.0
+    s.normal = T.f3(*normal)
+    s.centroid = T.f3(*point)
+    s.Tol = 1.0
+    return w
+
+
+def cylinder_wobj(radius, height, axis_normal, offset, top=False, bot=False):
+    """A macro cylinder as ReadWObject builds it (Input/KSimulationData.cpp:772-835):
+    radius = diameter/2, AXIS = (1,1,1) - n_bot, c_bot = offset + R*(1,1,1 - n_bot)."""
+    w = T.WOBJ()
+    w.surface_type = T.SURFACE_CYLINDER
+    w.num_surfaces = 1
+    g = w.cyl_geo
+    g.radius = float(radius)
+    g.height = float(height)
+    g.has_top_cap = top
+    g.has_bot_cap = bot
+    nb = np.asarray(axis_normal, np.float32)
+    g.normal_bot_cap = T.f3(*nb)
+    g.normal_top_cap = T.f3(*(-nb))
+    axis = (np.ones(3, np.float32) - nb).astype(np.float32)
+    g.AXIS = T.f3(*axis)
+    cb = (np.asarray(offset, np.float32) + np.float32(radius) * axis).astype(np.float32)
+    g.center_bot_cap = T.f3(*cb)
+    ct = (cb + nb * np.float32(height)).astype(np.float32)
+    g.center_top_cap = T.f3(*ct)
+    return w
+
+
+def sim_info(n, dt, gravity, origin, world_size, cell, rotation=True, roll_res=0.0,
+             kill=False, kill_height=0.0, sim_type=0):
+    s = T.SimulationInfo()
+    s.particle_type = 0
+    s.Num_Particles = int(n)
+    s.Num_ParticleObjects = 1
+    s.Rotation = bool(rotation)
+    s.InitalDelta_t = float(dt)
+    s.Force_Field = T.f3(*gravity)
+    s.Roll_Res = float(roll_res)
+    s.worldOrigin = T.f3(*origin)
+    s.cellSize = T.f3(cell, cell, cell)
+    total = np.asarray(world_size, np.float32) - np.asarray(origin, np.float32)
+    ncell = [int(math.ceil(float(np.float32(t) / np.float32(cell)))) for t in total]
+    s.num_NNCells = T.Int3(*ncell)
+    s.max_size = T.f3(*(np.asarray(ncell, np.float32) * np.float32(cell)))
+    s.Kill_Particles = bool(kill)
+    s.Silo_Kill_Height = float(kill_height)
+    s.Simulation_Type = sim_type
+    s.Integration_Type = 0
+    return s
+
+
+def _deck(sim, wobj, pobj, pos, vel, ptype=None, pid=None, radius=None, mass=None, ext=None,
+          name=""):
+    n = len(pos)
+    return dict(
+        name=name, sim=sim, wobj=wobj, pobj=pobj, ext=ext,
+        pos=np.ascontiguousarray(pos, np.float32),
+        vel=np.ascontiguousarray(vel, np.float32),
+        avel=np.zeros((n, 3), np.float32),
+        ptype=np.zeros(n, np.uint32) if ptype is None else ptype.astype(np.uint32),
+        pid=np.arange(n, dtype=np.int32) if pid is None else pid.astype(np.int32),
+        radius=None if radius is None else radius.astype(np.float32),
+        mass=None if mass is None else mass.astype(np.float32),
+    )
+
+
+def box_walls(L):
+    """Six quads of a closed cube [0, L]^3 with the vertex orders of the
+    reference's Container_128 deck (Projects/1.Particles_Box/SurfaceObjects/
+    Container_128.WOBJ), normals pointing inside."""
+    verts = [(0, 0, 0), (0, 0, L), (L, 0, L), (L, 0, 0),
+             (0, L, 0), (0, L, L), (L, L, L), (L, L, 0)]
+    quads = [(3, 2, 1, 0), (4, 0, 1, 5), (5, 1, 2, 6), (6, 2, 3, 7), (7, 3, 0, 4), (4, 5, 6, 7)]
+    return plane_wobj(verts, quads, tol=1.0)
+
+
+def lsd_ext(kn, e, mu, kt_over_kn=2.0 / 7.0):
+    x = T.ExtParams()
+    x.contact_law = T.LAW_LSD_COULOMB
+    for t in range(8):
+        m = x.material[t]
+        m.kn, m.kt_over_kn, m.restitution, m.mu = kn, kt_over_kn, e, mu
+    return x
+
+
+def box_deck(law="ref", seed=1, nx=32, ny=16, nz=16, L=50.0, radius=0.5, density=2500.0,
+             kn=1e5, e=0.6, mu=0.4, dt=1e-5, spacing=1.5, jitter=0.05, vmax=10.0):
+    """Box fill: a seeded jittered lattice in the upper half of a closed cube.
+    Defaults are the 8,192-sphere configuration (r 5 mm, rho 2500, 0.5 m box,
+    kn 1e5, e 0.6, mu 0.4, dt 1e-5, cell 2r, origin -1)."""
+    n = nx * ny * nz
+    gi = np.indices((nx, ny, nz)).reshape(3, -1).T.astype(np.float64)
+    x0 = np.array([1.75, 26.0, 1.75])
+    pos = x0 + gi * spacing
+    pos += uniform(seed, STREAM_JITTER, 3 * n, -jitter, jitter).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+    cell = 2.0 * radius
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (L, L, L), cell,
+                   rotation=True, roll_res=0.0)
+    pobj = [sphere_pobj(radius, density, e, kn, mu)]
+    ext = None if law == "ref" else lsd_ext(kn, e, mu)
+    return _deck(sim, [box_walls(L)], pobj, pos, vel, ext=ext, name=f"box{n}-{law}")
+
+
+def random_deck(n, seed=7, L=12.0, law="ref", radius=0.5, polydisperse=False, rmin=0.2, rmax=0.4,
+                density=2500.0, kn=1e5, e=0.6, mu=0.4, dt=1e-5, vmax=50.0, wmax=20.0,
+                cell=None, origin=-1.0, aliased=False):
+    """Dense random pack (overlaps allowed) in a closed box, random v and omega:
+    exercises every contact and wall branch in one step."""
+    pos = uniform(seed, STREAM_POS, 3 * n, 0.6, L - 0.6).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+    avel = uniform(seed, STREAM_VEL + 8, 3 * n, -wmax, wmax).reshape(n, 3)
+    rmax_ = rmax if polydisperse else radius
+    cell = 2.0 * rmax_ if cell is None else cell
+    ws = (L, L, L) if not aliased else (L, L, L * 0.5)
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (origin, origin, origin), ws, cell, rotation=True,
+                   roll_res=0.001)
+    pobj = [sphere_pobj(radius, density, e, kn, mu)]
+    ext = None
+    rad = mass = None
+    if law != "ref" or polydisperse:
+        ext = T.ExtParams() if law == "ref" else lsd_ext(kn, e, mu)
+        if law == "hm":
+            ext = hm_ext(1e5, 0.3, e, mu)
+    if polydisperse:
+        ext.per_particle_radius = 1
+        rad = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+        mass = np.array([sphere_mass(float(r), density) for r in rad], np.float32)
+    d = _deck(sim, [box_walls(L)], pobj, pos, vel, ext=ext, radius=rad, mass=mass,
+              name=f"random{n}-{law}")
+    d["avel"] = avel.astype(np.float32)
+    return d
+
+
+def hm_ext(E, nu, e, mu):
+    x = T.ExtParams()
+    x.contact_law = T.LAW_HERTZ_MINDLIN
+    for t in range(8):
+        m = x.material[t]
+        m.youngs, m.poisson, m.restitution, m.mu = E, nu, e, mu
+    return x
+
+
+# ---------------------------------------------------------------------------
+# silo
+# ---------------------------------------------------------------------------
+SILO_R = 25.0          # cylinder radius, cm
+SILO_H = 120.0         # cylinder height, cm
+SILO_OUTLET_R = 3.0    # outlet radius (60 mm outlet), cm
+SILO_HALF_ANGLE = 30.0  # hopper half-angle from the axis, degrees
+SILO_AXIS_XZ = 26.0    # axis position in x and z
+SILO_Y0 = 10.0         # outlet (hatch) height
+
+
+def silo_geometry():
+    hop_h = (SILO_R - SILO_OUTLET_R) / math.tan(math.radians(SILO_HALF_ANGLE))
+    return dict(y0=SILO_Y0, y1=SILO_Y0 + hop_h, top=SILO_Y0 + hop_h + SILO_H, hopper_height=hop_h)
+
+
+def silo_deck(n=1048576, seed=2, rmin=0.2, rmax=0.4, density=2600.0, E=1e5, nu=0.3, e=0.5,
+              mu=0.5, dt=5e-6, fill_spacing=None, law="hm"):
+    """Silo discharge deck.  Cylinder R 25 cm (axis y) over a 30-degree conical
+    hopper down to a 6 cm outlet closed by a hatch plane (the last world
+    object, so bdem_open_hatch removes it like Device_OpenHatch).  Initial
+    state: an FCC lattice of spacing 2*rmax + gap filling the hopper and the
+    cylinder bottom-up (a non-overlapping fill), radii U[rmin, rmax] from the
+    seed, zero velocity.  Hertz-Mindlin E in kg cm^-1 s^-2 (1e7 Pa = 1e5)."""
+    g = silo_geometry()
+    d = 2.0 * rmax + 0.01 if fill_spacing is None else fill_spacing
+    pos = _fcc_fill(n, d, rmax, g)
+    rad = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+    mass = np.array((4.0 / 3.0) * math.pi * rad.astype(np.float64) ** 3 * density * 1e-6, np.float32)
+    top = float(pos[:, 1].max()) + 2.0 * rmax
+    cell = 2.0 * rmax
+    world = (2 * SILO_AXIS_XZ, top, 2 * SILO_AXIS_XZ)
+    kill_h = SILO_Y0 - 5.0  # main.cpp:743, hatch height - 5
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (0.0, 0.0, 0.0), world, cell, rotation=True,
+                   roll_res=0.0, kill=True, kill_height=kill_h, sim_type=2)
+    cyl = cylinder_wobj(SILO_R, g["top"] - g["y1"], (0.0, 1.0, 0.0),
+                        (SILO_AXIS_XZ - SILO_R, 0.0, SILO_AXIS_XZ - SILO_R))
+    hatch = infinite_plane_wobj((SILO_AXIS_XZ, SILO_Y0, SILO_AXIS_XZ), (0.0, 1.0, 0.0))
+    pobj = [sphere_pobj(rmax, density, e, 1e5, mu)]
+    ext = hm_ext(E, nu, e, mu) if law == "hm" else T.ExtParams()
+    ext.per_particle_radius = 1
+    ext.num_cones = 1
+    c = ext.cone[0]
+    c.base = T.f3(SILO_AXIS_XZ, 0.0, SILO_AXIS_XZ)
+    c.axis = T.f3(0.0, 1.0, 0.0)
+    c.s0, c.s1 = SILO_Y0, g["y1"]
+    c.r0, c.r1 = SILO_OUTLET_R, SILO_R
+    vel = np.zeros((n, 3), np.float32)
+    return _deck(sim, [cyl, hatch], pobj, pos, vel, ext=ext, radius=rad, mass=mass,
+                 name=f"silo{n}")
+
+
+def _fcc_fill(n, d, rmax, g):
+    """First n sites (bottom-up) of an FCC lattice of nearest-neighbour
+    distance d that lie inside the hopper and cylinder with clearance rmax."""
+    a = d * math.sqrt(2.0)  # cubic cell edge
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) * a
+    cx = cz = SILO_AXIS_XZ
+    half = SILO_R
+    nxz = int(math.ceil(2 * half / a)) + 1
+    layer_h = a
+    out = []
+    count = 0
+    y = SILO_Y0 + rmax + 0.01
+    k = 0
+    tan = math.tan(math.radians(SILO_HALF_ANGLE))
+    cos = math.cos(math.radians(SILO_HALF_ANGLE))
+    ii, jj = np.meshgrid(np.arange(nxz), np.arange(nxz), indexing="ij")
+    while count < n:
+        for b in basis:
+            yy = y + k * layer_h + b[1]
+            xs = cx - half + ii * a + b[0]
+            zs = cz - half + jj * a + b[2]
+            rho = np.sqrt((xs - cx) ** 2 + (zs - cz) ** 2)
+            if yy < g["y1"]:
+                rcone = SILO_OUTLET_R + (yy - SILO_Y0) * tan
+                ok = rho <= rcone - rmax / cos - 0.01
+            else:
+                ok = rho <= SILO_R - rmax - 0.01
+            sel = np.stack([xs[ok], np.full(ok.sum(), yy), zs[ok]], axis=1)
+            out.append(sel)
+            count += len(sel)
+        k += 1
+    pts = np.concatenate(out)
+    pts = pts[np.lexsort((pts[:, 2], pts[:, 0], pts[:, 1]))]
+    return pts[:n].astype(np.float32)

@@ -1,0 +1,739 @@
+/*
+ * bdem_abi.cpp - context, buffer management and the C ABI (include/bdem_abi.h).
+ *
+ * One context = one simulation on one HIP stream.  All device buffers are
+ * allocated once in bdem_set_simulation_data; a step launches
+ *   [hash] -> radix sort (3 x hist/scan/scatter) -> gather -> fill+bin -> step
+ * with no host synchronisation and no allocation.
+ */
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "bdem_abi.h"
+#include "bdem_kernels.hpp"
+
+using namespace bdem;
+
+struct bdem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool configured = false;
+
+  bdem_SimulationInfo sim{};
+  std::vector<bdem_WOBJ> wobj;
+  std::vector<bdem_POBJ> pobj;
+  std::vector<bdem_DOBJ> dobj;
+  bdem_ext_params ext{};
+  bool has_ext = false;
+  int num_wobj_active = 0;
+  bool cyl_rotation = false;
+  bool record_wf = false;
+  bool start_bin = false;
+  float tallys[6] = {0, 0, 0, 0, 0, 0};
+
+  DevParams hp{};
+  DevParams* dp = nullptr;
+  StepBuffers b{};
+  std::vector<void*> allocs;
+  bool orient = false;
+  bool keys_valid = false;
+  int hist_par = 0;
+  int stepc = 0;
+};
+
+namespace {
+
+inline float bits_f(int v) { float f; std::memcpy(&f, &v, 4); return f; }
+inline int f_bits(float f) { int v; std::memcpy(&v, &f, 4); return v; }
+
+int fail(bdem_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(ctx, BDEM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(bdem_ctx* c, T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) return fail(c, BDEM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back((void*)*p);
+  return BDEM_OK;
+}
+
+void free_all(bdem_ctx* c) {
+  for (void* p : c->allocs) hipFree(p);
+  c->allocs.clear();
+  c->b = StepBuffers{};
+  c->dp = nullptr;
+  c->configured = false;
+}
+
+/* Host derivation of the history-law constants; the oracle restates the
+ * same formulas (beta = ln e / sqrt(ln^2 e + pi^2), Hertz E*, G*). */
+void derive_laws(const bdem_ext_params& ext, DevParams& P) {
+  const double pi = 3.14159265358979323846;
+  auto beta_of = [&](double e) {
+    return (e > 0.0 && e < 1.0) ? std::log(e) / std::sqrt(std::log(e) * std::log(e) + pi * pi) : 0.0;
+  };
+  for (int a = 0; a < MAX_TYPES; ++a) {
+    const bdem_material& ma = ext.material[a];
+    for (int b = 0; b < MAX_TYPES; ++b) {
+      const bdem_material& mb = ext.material[b];
+      const double e = 0.5 * (double(ma.restitution) + double(mb.restitution));
+      const double kn = 0.5 * (double(ma.kn) + double(mb.kn));
+      const double ktr = 0.5 * (double(ma.kt_over_kn) + double(mb.kt_over_kn));
+      const double mu = 0.5 * (double(ma.mu) + double(mb.mu));
+      const double ea = ma.youngs, eb = mb.youngs, na = ma.poisson, nb = mb.poisson;
+      double estar = 0.0, gstar = 0.0;
+      if (ea > 0 && eb > 0) {
+        estar = 1.0 / ((1.0 - na * na) / ea + (1.0 - nb * nb) / eb);
+        gstar = 1.0 / (2.0 * (2.0 - na) * (1.0 + na) / ea + 2.0 * (2.0 - nb) * (1.0 + nb) / eb);
+      }
+      P.law_pp[a][b] = LawPair{float(kn), float(kn * ktr), float(beta_of(e)), float(mu),
+                               float(estar), float(gstar), 0.f, 0.f};
+    }
+    const double ew = ext.wall_youngs > 0 ? ext.wall_youngs : ma.youngs;
+    const double nw = ext.wall_youngs > 0 ? ext.wall_poisson : ma.poisson;
+    double estar = 0.0, gstar = 0.0;
+    if (ma.youngs > 0 && ew > 0) {
+      estar = 1.0 / ((1.0 - double(ma.poisson) * ma.poisson) / ma.youngs + (1.0 - nw * nw) / ew);
+      gstar = 1.0 / (2.0 * (2.0 - double(ma.poisson)) * (1.0 + double(ma.poisson)) / ma.youngs +
+                     2.0 * (2.0 - nw) * (1.0 + nw) / ew);
+    }
+    P.law_w[a] = LawPair{ma.kn, float(double(ma.kn) * ma.kt_over_kn), float(beta_of(ma.restitution)),
+                         ma.mu, float(estar), float(gstar), 0.f, 0.f};
+  }
+}
+
+void build_params(bdem_ctx* c) {
+  DevParams& P = c->hp;
+  std::memset(&P, 0, sizeof(P));
+  const bdem_SimulationInfo& s = c->sim;
+  P.origin = s.worldOrigin;
+  P.cell = s.cellSize;
+  P.nx = s.num_NNCells.x; P.ny = s.num_NNCells.y; P.nz = s.num_NNCells.z;
+  P.C = uint32_t(P.nx) * uint32_t(P.ny) * uint32_t(P.nz);
+  P.dt = s.InitalDelta_t;
+  P.gravity = s.Force_Field;
+  P.roll_res = s.Roll_Res;
+  P.rotation = s.Rotation ? 1 : 0;
+  P.mode2d = s.Mode2D ? 1 : 0;
+  P.sphere_orient = s.sphere_orient ? 1 : 0;
+  P.law = c->has_ext ? c->ext.contact_law : BDEM_LAW_REFERENCE;
+  P.per_particle = (c->has_ext && c->ext.per_particle_radius) ? 1 : 0;
+  P.num_wobj = c->num_wobj_active;
+  P.cyl_rotation = c->cyl_rotation ? 1 : 0;
+  P.rotating_wobj = s.Rotating_WObject ? 1 : 0;
+  P.mill_rot_vel = s.MillRotVel;
+  P.num_cones = c->has_ext ? c->ext.num_cones : 0;
+  P.kill_height = s.Silo_Kill_Height;
+  P.max_size = s.max_size;
+  P.kill_enabled = s.Kill_Particles ? 1 : 0;
+  for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
+    const bdem_POBJ& po = c->pobj[t];
+    P.t_radius[t] = po.radius;
+    P.t_mass[t] = po.mass;
+    P.t_inertia[t] = po.InertiaT[0];
+    for (size_t u = 0; u < c->pobj.size() && u < size_t(MAX_TYPES); ++u) {
+      /* POBJ::PP has 6 entries (KSimulationObjects.h:290) */
+      if (u < 6) {
+        P.pp_kn[t][u] = po.PP[u].Kn;
+        P.pp_cn[t][u] = po.PP[u].Cn;
+        P.pp_fk[t][u] = po.PP[u].Fric_kinetic;
+      }
+    }
+    P.s_kn[t] = po.surface_Kn;
+    P.s_cn[t] = po.surface_Cn;
+    P.s_fs[t] = po.surface_Fric_static;
+    P.s_fk[t] = po.surface_Fric_kinetic;
+  }
+  if (c->has_ext) {
+    derive_laws(c->ext, P);
+    for (int k = 0; k < c->ext.num_cones && k < BDEM_MAX_CONES; ++k) P.cone[k] = c->ext.cone[k];
+  }
+  for (size_t k = 0; k < c->wobj.size() && k < size_t(MAX_WOBJ); ++k) P.wobj[k] = c->wobj[k];
+}
+
+int upload_params(bdem_ctx* c) {
+  build_params(c);
+  HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BDEM_OK;
+}
+
+bool need_history(const bdem_ctx* c) {
+  return c->has_ext && c->ext.contact_law != BDEM_LAW_REFERENCE;
+}
+
+int require(bdem_ctx* c) {
+  if (!c) return BDEM_EINVAL;
+  if (!c->configured) return fail(c, BDEM_ESTATE, "bdem_set_simulation_data has not been called");
+  return BDEM_OK;
+}
+
+/* Host AoS -> device float4 streams.  Radius and mass come from the type
+ * table (the reference's only source) unless per-particle values exist. */
+int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                 const bdem_float3* vel, const bdem_float3* avel, const unsigned int* ptype,
+                 const int* pid, bool keep_vel) {
+  const int n = c->b.n;
+  std::vector<float4> P(n), V(n), O(n), Q;
+  std::vector<float4> oldP, oldV, oldO;
+  if (keep_vel || c->hp.per_particle) {
+    oldP.resize(n); oldV.resize(n); oldO.resize(n);
+    HIPCHK(c, hipMemcpy(oldP.data(), c->b.pos_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(oldV.data(), c->b.vel_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(oldO.data(), c->b.omg_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  }
+  for (int i = 0; i < n; ++i) {
+    const unsigned t = ptype[i];
+    if (t >= c->pobj.size() || t >= (unsigned)MAX_TYPES)
+      return fail(c, BDEM_EINVAL, "particle type out of range at index " + std::to_string(i));
+    const float r = c->hp.per_particle ? oldP[i].w : c->pobj[t].radius;
+    const float m = c->hp.per_particle ? oldV[i].w : c->pobj[t].mass;
+    P[i] = make_float4(pos[i].x, pos[i].y, pos[i].z, r);
+    if (vel) V[i] = make_float4(vel[i].x, vel[i].y, vel[i].z, m);
+    else V[i] = make_float4(oldV[i].x, oldV[i].y, oldV[i].z, m);
+    if (avel) O[i] = make_float4(avel[i].x, avel[i].y, avel[i].z, bits_f(pack_id_type(pid[i], t)));
+    else O[i] = make_float4(oldO[i].x, oldO[i].y, oldO[i].z, bits_f(pack_id_type(pid[i], t)));
+  }
+  HIPCHK(c, hipMemcpy(c->b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->b.omg_cur, O.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  if (c->orient) {
+    Q.resize(n);
+    for (int i = 0; i < n; ++i) {
+      if (ornt && !c->sim.sphere_orient) Q[i] = make_float4(ornt[i].w, ornt[i].x, ornt[i].y, ornt[i].z);
+      else Q[i] = make_float4(1.f, 0.f, 0.f, 0.f);  /* Set_Pos: identity for sphere_orient */
+    }
+    HIPCHK(c, hipMemcpy(c->b.quat_cur, Q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (need_history(c)) {
+    for (int i = 0; i < n; ++i)
+      if (pid[i] >= c->b.hist_cap)
+        return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
+    for (int h = 0; h < 2; ++h) {
+      launch_fill_u32((uint32_t*)c->b.hcount[h], c->b.hist_cap, 0u, c->stream);
+      launch_fill_u32((uint32_t*)c->b.wcount[h], c->b.hist_cap, 0u, c->stream);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->keys_valid = false;
+  return BDEM_OK;
+}
+
+int download(bdem_ctx* c, std::vector<float4>& P, std::vector<float4>& V, std::vector<float4>& O,
+             std::vector<float4>& Q) {
+  const int n = c->b.n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  P.resize(n); V.resize(n); O.resize(n);
+  HIPCHK(c, hipMemcpy(P.data(), c->b.pos_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(V.data(), c->b.vel_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(O.data(), c->b.omg_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (c->orient) {
+    Q.resize(n);
+    HIPCHK(c, hipMemcpy(Q.data(), c->b.quat_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  }
+  return BDEM_OK;
+}
+
+/* hash (if needed) + sort + gather + cells: the sorted scratch state. */
+int prepare_sorted(bdem_ctx* c, int* sorted_buf) {
+  if (!c->keys_valid) launch_hash(c->dp, c->b, c->stream);
+  const int bits = key_bits(c->hp.C + 1u);
+  const int cur = launch_sort(c->b, bits, c->stream);
+  launch_gather(c->b, c->b.vals[cur], c->orient, c->stream);
+  launch_cells(c->dp, c->b, c->b.keys[cur], c->hp.C, c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (sorted_buf) *sorted_buf = cur;
+  return BDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bdem_create(bdem_ctx** out, int device) {
+  if (!out) return BDEM_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BDEM_EHIP;
+  if (device < 0 || device >= ndev) return BDEM_EINVAL;
+  bdem_ctx* c = new bdem_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BDEM_EHIP;
+  }
+  *out = c;
+  return BDEM_OK;
+}
+
+int bdem_destroy(bdem_ctx* c) {
+  if (!c) return BDEM_EINVAL;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_all(c);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return BDEM_OK;
+}
+
+const char* bdem_last_error(const bdem_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ* pobj,
+                             const bdem_DOBJ* dobj, const bdem_SimulationInfo* sim, int num_wobj,
+                             int num_pobj, int num_dobj, const bdem_InitConfig* init) {
+  if (!c || !sim || !pobj || (num_wobj > 0 && !wobj)) return c ? fail(c, BDEM_EINVAL, "null argument") : BDEM_EINVAL;
+  if (sim->particle_type != 0) return fail(c, BDEM_ENOTSUP, "polyhedra (particle_type 1) are not implemented yet");
+  if (sim->Integration_Type != 0) return fail(c, BDEM_ENOTSUP, "only symplectic Euler (Integration_Type 0) is live in the reference");
+  if (sim->use_symmetry) return fail(c, BDEM_ENOTSUP, "the symmetric path is disabled in the reference (main.cpp:811)");
+  if (num_wobj < 0 || num_wobj > MAX_WOBJ) return fail(c, BDEM_EINVAL, "num_wobj must be in [0, 8]");
+  if (num_pobj <= 0 || num_pobj > MAX_TYPES) return fail(c, BDEM_EINVAL, "num_pobj must be in [1, 8]");
+  if (sim->Num_Particles <= 0) return fail(c, BDEM_EINVAL, "Num_Particles must be positive");
+  if (sim->num_NNCells.x <= 0 || sim->num_NNCells.y <= 0 || sim->num_NNCells.z <= 0)
+    return fail(c, BDEM_EINVAL, "num_NNCells must be positive");
+  const unsigned long long C = (unsigned long long)sim->num_NNCells.x * sim->num_NNCells.y * sim->num_NNCells.z;
+  if (C + 2 > 0xFFFFFFF0ull) return fail(c, BDEM_EINVAL, "grid too large for 32-bit keys");
+  if (num_dobj > 0) return fail(c, BDEM_ENOTSUP, "lifters (dynamic objects) are not implemented yet");
+  for (int k = 0; k < num_wobj; ++k) {
+    const bdem_WOBJ& w = wobj[k];
+    if (w.surface_type != BDEM_SURFACE_PLANE && w.surface_type != BDEM_SURFACE_CYLINDER)
+      return fail(c, BDEM_EINVAL, "unknown world-object surface type");
+    if (w.surface_type != BDEM_SURFACE_PLANE) continue;
+    if (w.num_surfaces < 0 || w.num_surfaces > 16 || w.num_vertex < 0 || w.num_vertex > 16)
+      return fail(c, BDEM_EINVAL, "world object surface/vertex count out of range");
+    for (int s = 0; s < w.num_surfaces; ++s) {
+      const bdem_Surface& S = w.surfaces[s];
+      if (S.area == -1.0f) continue;  /* infinite plane: no polygon */
+      if (S.num_vertex < 1 || S.num_vertex > 8)
+        return fail(c, BDEM_EINVAL, "surface vertex count out of range");
+      for (int q = 0; q < S.num_vertex; ++q)
+        if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= w.num_vertex)
+          return fail(c, BDEM_EINVAL, "surface vertex index out of range");
+    }
+  }
+
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  free_all(c);
+  c->sim = *sim;
+  c->wobj.assign(wobj, wobj + num_wobj);
+  c->pobj.assign(pobj, pobj + num_pobj);
+  c->dobj.assign(dobj ? dobj : nullptr, dobj ? dobj + num_dobj : nullptr);
+  c->num_wobj_active = num_wobj;
+  c->has_ext = false;
+  std::memset(&c->ext, 0, sizeof(c->ext));
+  c->orient = sim->sphere_orient;
+  c->stepc = 0;
+  c->hist_par = 0;
+
+  const int n = sim->Num_Particles;
+  StepBuffers& b = c->b;
+  b.n = n;
+  int rc;
+#define ALLOC(p, cnt) if ((rc = dalloc(c, &(p), (cnt))) != BDEM_OK) return rc
+  ALLOC(c->dp, 1);
+  ALLOC(b.pos_cur, n); ALLOC(b.vel_cur, n); ALLOC(b.omg_cur, n);
+  ALLOC(b.pos_srt, n); ALLOC(b.vel_srt, n); ALLOC(b.omg_srt, n);
+  if (c->orient) { ALLOC(b.quat_cur, n); ALLOC(b.quat_srt, n); }
+  ALLOC(b.keys[0], n); ALLOC(b.keys[1], n); ALLOC(b.vals[0], n); ALLOC(b.vals[1], n);
+  ALLOC(b.radix_hist, (size_t)RADIX * sort_blocks(n));
+  ALLOC(b.cells, (size_t)C);
+  ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
+  ALLOC(b.wall_force, (size_t)n * 3);
+#undef ALLOC
+  HIPCHK(c, hipMemset(b.status, 0, sizeof(uint32_t)));
+  HIPCHK(c, hipMemset(b.kill_count, 0, sizeof(int)));
+  HIPCHK(c, hipMemset(b.wall_force, 0, (size_t)n * 3 * sizeof(float)));
+  HIPCHK(c, hipMemset(b.pos_cur, 0, (size_t)n * sizeof(float4)));
+  c->configured = true;
+  if ((rc = upload_params(c)) != BDEM_OK) return rc;
+
+  /* Set_Init_Config_Device (GPU_Memory_Transfer.cuh:205-271): velocity =
+   * InitVel[0], angular velocity = 0.  InitVel[0] is init->velocity[0] for a
+   * grid start, launch_Vel for a mill fill, zero when reading a file
+   * (Device/DeviceInterface.cu:1708-1735). */
+  bdem_float3 v0{0.f, 0.f, 0.f};
+  if (init && !init->use_file) v0 = init->grid_type < 2 ? init->velocity[0] : init->launch_Vel;
+  std::vector<float4> P(n), V(n), O(n);
+  for (int i = 0; i < n; ++i) {
+    P[i] = make_float4(0.f, 0.f, 0.f, pobj[0].radius);
+    V[i] = make_float4(v0.x, v0.y, v0.z, pobj[0].mass);
+    O[i] = make_float4(0.f, 0.f, 0.f, bits_f(pack_id_type(-1, 0)));
+  }
+  HIPCHK(c, hipMemcpy(b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.omg_cur, O.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  if (c->orient) {
+    std::vector<float4> Q(n, make_float4(1.f, 0.f, 0.f, 0.f));
+    HIPCHK(c, hipMemcpy(b.quat_cur, Q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  c->keys_valid = false;
+  std::memset(c->tallys, 0, sizeof(c->tallys));
+  return BDEM_OK;
+}
+
+int bdem_set_ext_params(bdem_ctx* c, const bdem_ext_params* ext) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!ext) return fail(c, BDEM_EINVAL, "null ext");
+  if (ext->contact_law < 0 || ext->contact_law > 2) return fail(c, BDEM_EINVAL, "unknown contact law");
+  if (ext->num_cones < 0 || ext->num_cones > BDEM_MAX_CONES) return fail(c, BDEM_EINVAL, "num_cones out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ext = *ext;
+  c->has_ext = true;
+  const int n = c->b.n;
+  if (need_history(c) && !c->b.hcount[0]) {
+    c->b.hist_cap = n;
+    for (int h = 0; h < 2; ++h) {
+      if ((rc = dalloc(c, &c->b.hcount[h], n)) != BDEM_OK) return rc;
+      if ((rc = dalloc(c, &c->b.hrec[h], (size_t)n * HMAX)) != BDEM_OK) return rc;
+      if ((rc = dalloc(c, &c->b.wcount[h], n)) != BDEM_OK) return rc;
+      if ((rc = dalloc(c, &c->b.wrec[h], (size_t)n * WMAX)) != BDEM_OK) return rc;
+      launch_fill_u32((uint32_t*)c->b.hcount[h], n, 0u, c->stream);
+      launch_fill_u32((uint32_t*)c->b.wcount[h], n, 0u, c->stream);
+    }
+  }
+  return upload_params(c);
+}
+
+int bdem_set_pos(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                 const unsigned int* ptype, const int* pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!pos || !ptype || !pid) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return upload_state(c, pos, ornt, nullptr, nullptr, ptype, pid, true);
+}
+
+int bdem_set_pos_velocity(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                          const bdem_float3* vel, const bdem_float3* avel,
+                          const unsigned int* ptype, const int* pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!pos || !vel || !avel || !ptype || !pid) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return upload_state(c, pos, ornt, vel, avel, ptype, pid, false);
+}
+
+int bdem_set_system_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt,
+                          const bdem_float3* vel, const bdem_float3* avel, const bdem_float3* acc,
+                          const unsigned int* ptype, const int* pid) {
+  (void)acc;
+  return bdem_set_pos_velocity(c, pos, ornt, vel, avel, ptype, pid);
+}
+
+int bdem_set_radii(bdem_ctx* c, const float* radius, const float* mass) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!c->hp.per_particle) return fail(c, BDEM_ESTATE, "per_particle_radius is not enabled");
+  if (!radius || !mass) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int n = c->b.n;
+  std::vector<float4> P(n), V(n);
+  HIPCHK(c, hipMemcpy(P.data(), c->b.pos_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(V.data(), c->b.vel_cur, n * sizeof(float4), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) {
+    if (!(radius[i] > 0.f) || !(mass[i] > 0.f)) return fail(c, BDEM_EINVAL, "radius and mass must be positive");
+    P[i].w = radius[i];
+    V[i].w = mass[i];
+  }
+  HIPCHK(c, hipMemcpy(c->b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  return BDEM_OK;
+}
+
+int bdem_set_tallys(bdem_ctx* c, const float* t) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!t) return fail(c, BDEM_EINVAL, "null argument");
+  for (int i = 0; i < 3; ++i) c->tallys[i] = t[i];
+  return BDEM_OK;
+}
+
+int bdem_get_system_state(bdem_ctx* c, bdem_float3* pos, bdem_Quaterion* ornt, bdem_float3* vel,
+                          bdem_float3* avel, bdem_float3* acc, unsigned int* ptype, int* pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<float4> P, V, O, Q;
+  if ((rc = download(c, P, V, O, Q)) != BDEM_OK) return rc;
+  for (int i = 0; i < c->b.n; ++i) {
+    if (pos) pos[i] = bdem_float3{P[i].x, P[i].y, P[i].z};
+    if (vel) vel[i] = bdem_float3{V[i].x, V[i].y, V[i].z};
+    if (avel) avel[i] = bdem_float3{O[i].x, O[i].y, O[i].z};
+    if (acc) acc[i] = bdem_float3{0.f, 0.f, 0.f};
+    const int packed = f_bits(O[i].w);
+    if (ptype) ptype[i] = (unsigned)(packed & 7);
+    if (pid) pid[i] = packed >> 3;
+    if (ornt && c->orient) ornt[i] = bdem_Quaterion{Q[i].x, Q[i].y, Q[i].z, Q[i].w};
+  }
+  return BDEM_OK;
+}
+
+int bdem_get_positions(bdem_ctx* c, bdem_float3* pos, bdem_Quaterion* ornt, unsigned int* ptype, int* pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (c->b.n == 0) { if (pid) pid[0] = -2; return BDEM_OK; }  /* Device/DeviceInterface.cu:2599-2602 */
+  return bdem_get_system_state(c, pos, ornt, nullptr, nullptr, nullptr, ptype, pid);
+}
+
+int bdem_get_velocity(bdem_ctx* c, bdem_float3* vel, bdem_float3* avel, int flag, unsigned int* ptype, int* pid) {
+  return bdem_get_system_state(c, nullptr, nullptr, vel, flag == 1 ? avel : nullptr, nullptr, ptype, pid);
+}
+
+int bdem_record_wall_forces(bdem_ctx* c, int on) {
+  int rc = require(c);
+  if (rc) return rc;
+  c->record_wf = on != 0;
+  return BDEM_OK;
+}
+
+int bdem_get_wall_forces(bdem_ctx* c, bdem_float3* wf, bdem_float3* cp) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int n = c->b.n;
+  std::vector<float> F((size_t)n * 3);
+  HIPCHK(c, hipMemcpy(F.data(), c->b.wall_force, F.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) {
+    /* Get_Wall_Forces returns the negated wall force, GPU_Memory_Transfer.cuh:552-566 */
+    if (wf) wf[i] = bdem_float3{-1.0f * F[3 * i], -1.0f * F[3 * i + 1], -1.0f * F[3 * i + 2]};
+    if (cp) cp[i] = bdem_float3{0.f, 0.f, 0.f};
+  }
+  return BDEM_OK;
+}
+
+int bdem_get_energy(bdem_ctx* c, float* e) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!e) return fail(c, BDEM_EINVAL, "null argument");
+  for (int i = 0; i < 3; ++i) e[i] = float(double(c->tallys[i]) * 1E-4);
+  return BDEM_OK;
+}
+
+int bdem_get_status(bdem_ctx* c, unsigned int* w) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  uint32_t s = 0;
+  HIPCHK(c, hipMemcpy(&s, c->b.status, sizeof(s), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemset(c->b.status, 0, sizeof(uint32_t)));
+  if (w) *w = s;
+  return BDEM_OK;
+}
+
+int bdem_get_kill_count(bdem_ctx* c, int* k) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(k, c->b.kill_count, sizeof(int), hipMemcpyDeviceToHost));
+  return BDEM_OK;
+}
+
+int bdem_num_particles(const bdem_ctx* c) { return (c && c->configured) ? c->b.n : 0; }
+long long bdem_num_cells(const bdem_ctx* c) { return (c && c->configured) ? (long long)c->hp.C : 0; }
+
+int bdem_update_dobject_positions(bdem_ctx* c, const bdem_DOBJ* d) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (c->dobj.empty()) return BDEM_OK;
+  (void)d;
+  return fail(c, BDEM_ENOTSUP, "lifters are not implemented yet");
+}
+
+int bdem_update_world_object_positions(bdem_ctx* c, const bdem_WOBJ* w) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!w) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < c->wobj.size(); ++k) c->wobj[k] = w[k];
+  return upload_params(c);
+}
+
+int bdem_open_hatch(bdem_ctx* c, const bdem_WOBJ* w, int nw) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (w) {
+    if (nw < 0 || nw > MAX_WOBJ) return fail(c, BDEM_EINVAL, "num_wobj out of range");
+    c->wobj.assign(w, w + nw);
+  }
+  if (c->num_wobj_active > 0) c->num_wobj_active -= 1;
+  return upload_params(c);
+}
+
+int bdem_reset_energy(bdem_ctx* c, int num) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (num <= 3) { c->tallys[0] = c->tallys[1] = c->tallys[2] = 0.f; }
+  else { c->tallys[3] = c->tallys[4] = 0.f; }
+  return BDEM_OK;
+}
+
+int bdem_drum_rotation(bdem_ctx* c, int on) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->cyl_rotation = on != 0;
+  return upload_params(c);
+}
+
+int bdem_start_bin(bdem_ctx* c) {
+  int rc = require(c);
+  if (rc) return rc;
+  c->start_bin = true;
+  return BDEM_OK;
+}
+
+int bdem_add_force(bdem_ctx* c) { return require(c); }
+int bdem_sub_force(bdem_ctx* c) { return require(c); }
+int bdem_zero_force(bdem_ctx* c) { return require(c); }
+
+int bdem_update_sim(bdem_ctx* c) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  bool do_kill = false;
+  if (c->sim.Kill_Particles) {
+    const int interval = (c->has_ext && c->ext.kill_check_interval > 0) ? c->ext.kill_check_interval : 1000;
+    if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
+    else c->stepc += 1;
+  }
+  launch_step(c->dp, c->b, c->hp.law, c->orient, c->hist_par, do_kill, c->record_wf, c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (need_history(c)) c->hist_par ^= 1;
+  c->keys_valid = true;
+  return BDEM_OK;
+}
+
+int bdem_step(bdem_ctx* c, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = bdem_update_sim(c);
+    if (rc) return rc;
+  }
+  return BDEM_OK;
+}
+
+int bdem_sync(bdem_ctx* c) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BDEM_OK;
+}
+
+int bdem_debug_hash(bdem_ctx* c, unsigned int* keys, int* idx) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  launch_hash(c->dp, c->b, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int n = c->b.n;
+  HIPCHK(c, hipMemcpy(keys, c->b.keys[0], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  c->keys_valid = true;
+  return BDEM_OK;
+}
+
+int bdem_debug_sorted(bdem_ctx* c, unsigned int* keys, int* idx) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  int cur = 0;
+  if ((rc = prepare_sorted(c, &cur)) != BDEM_OK) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int n = c->b.n;
+  HIPCHK(c, hipMemcpy(keys, c->b.keys[cur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(idx, c->b.vals[cur], n * sizeof(int), hipMemcpyDeviceToHost));
+  c->keys_valid = false;
+  return BDEM_OK;
+}
+
+int bdem_debug_cells(bdem_ctx* c, unsigned int* cs, unsigned int* ce) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<uint2> cells(c->hp.C);
+  HIPCHK(c, hipMemcpy(cells.data(), c->b.cells, cells.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < cells.size(); ++k) { cs[k] = cells[k].x; ce[k] = cells[k].y; }
+  c->keys_valid = false;
+  return BDEM_OK;
+}
+
+int bdem_debug_contacts(bdem_ctx* c, unsigned int* list32, unsigned int* nn) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  const int n = c->b.n;
+  uint32_t *dl = nullptr, *dn = nullptr;
+  HIPCHK(c, hipMalloc((void**)&dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t)));
+  HIPCHK(c, hipMalloc((void**)&dn, (size_t)n * sizeof(uint32_t)));
+  launch_fill_u32(dl, (size_t)n * LIST_STRIDE, 0u, c->stream);
+  launch_contact_list(c->dp, c->b, dl, dn, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(list32, dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nn, dn, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  hipFree(dl);
+  hipFree(dn);
+  c->keys_valid = false;
+  if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("debug_contacts: ") + hipGetErrorString(e));
+  return BDEM_OK;
+}
+
+int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw, int* sorted_pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  const int n = c->b.n;
+  float* d = nullptr;
+  HIPCHK(c, hipMalloc((void**)&d, (size_t)n * 12 * sizeof(float)));
+  launch_forces(c->dp, c->b, c->hp.law, c->hist_par, d, d + 3 * (size_t)n, d + 6 * (size_t)n,
+                d + 9 * (size_t)n, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  std::vector<float> h((size_t)n * 12);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("debug_forces: ") + hipGetErrorString(e));
+  std::memcpy(fpp, h.data(), (size_t)n * 3 * sizeof(float));
+  std::memcpy(tpp, h.data() + 3 * (size_t)n, (size_t)n * 3 * sizeof(float));
+  std::memcpy(fw, h.data() + 6 * (size_t)n, (size_t)n * 3 * sizeof(float));
+  std::memcpy(tw, h.data() + 9 * (size_t)n, (size_t)n * 3 * sizeof(float));
+  if (sorted_pid) {
+    std::vector<float4> O(n);
+    HIPCHK(c, hipMemcpy(O.data(), c->b.omg_srt, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) sorted_pid[i] = f_bits(O[i].w) >> 3;
+  }
+  c->keys_valid = false;
+  return BDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,569 @@
+/*
+ * bdem_device.hpp - device-side data layout, parameters and contact physics
+ * for the gfx950 DEM step.
+ *
+ * Every floating-point expression here is written so that, compiled with
+ * -ffp-contract=off and IEEE division/sqrt, it performs exactly the
+ * operations of the reference arithmetic in the same order (the reference
+ * lines are cited per function).  No FMA contraction, no reciprocal tricks:
+ * the sphere path is memory-bound, so exactness costs nothing measurable.
+ *
+ * Particle state is SoA, three float4 streams in sorted (cell) order:
+ *   pos_r = (x, y, z, radius)     vel_m = (vx, vy, vz, mass)
+ *   omg_t = (wx, wy, wz, bits(P_ID * 8 + type))
+ * plus an optional float4 orientation (w, x, y, z).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "bdem_types.h"
+
+namespace bdem {
+
+constexpr int LIST_STRIDE = 32;   /* Broad_List stride, Collision_Detection_BroadPhase.cuh:481 */
+constexpr int HMAX = 16;          /* pair tangential-history records per particle */
+constexpr int WMAX = 4;           /* wall tangential-history records per particle */
+constexpr int MAX_TYPES = 8;      /* __constant__ POBJ ParticleObject[8], GPU_Memory_Transfer.cuh:81 */
+constexpr int MAX_WOBJ = 8;       /* __constant__ WOBJ WorldObject[8], GPU_Memory_Transfer.cuh:80 */
+
+/* Status bits raised by kernels into the context's status word. */
+enum : uint32_t {
+  ST_OUT_OF_GRID   = 1u,   /* a live particle left the hash grid */
+  ST_LIST_OVERFLOW = 2u,   /* more than 32 contacts (reference list capacity) */
+  ST_HIST_OVERFLOW = 4u,   /* more contacts than history slots */
+};
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+/* Utilities/Device/Functions.cuh:198-201 */
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* Utilities/Device/Functions.cuh:370-373 */
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+/* Utilities/Device/Functions.cuh:241-244 */
+__device__ __forceinline__ float length(f3 a) { return __builtin_sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 cvt(bdem_float3 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+
+__device__ __forceinline__ int packed_pid(float w) { return __float_as_int(w) >> 3; }
+__device__ __forceinline__ int packed_type(float w) { return __float_as_int(w) & 7; }
+__host__ __device__ __forceinline__ int pack_id_type(int pid, int type) { return pid * 8 + (type & 7); }
+
+/* Per type-pair constants of the history laws (derived on the host). */
+struct LawPair { float kn, kt, beta, mu, estar, gstar, pad0, pad1; };
+
+/* Tangential spring record: (xi.x, xi.y, xi.z, bits(partner id)). */
+typedef float4 HistRec;
+
+/* Everything a kernel reads besides particle streams; lives in device memory
+ * (one copy per context), read through uniform scalar loads. */
+struct DevParams {
+  /* grid (SimulationInfo worldOrigin/cellSize/num_NNCells) */
+  bdem_float3 origin;
+  bdem_float3 cell;
+  int nx, ny, nz;
+  uint32_t C;                 /* nx*ny*nz */
+  /* time stepping */
+  float dt;
+  bdem_float3 gravity;
+  float roll_res;
+  int rotation, mode2d, sphere_orient;
+  int law;                    /* BDEM_LAW_* */
+  int per_particle;           /* 1: inverse inertia from per-particle radius/mass */
+  /* walls */
+  int num_wobj;               /* Num_WorldObjects (Hatch decrements it) */
+  int cyl_rotation;           /* drum_rotation() flag */
+  int rotating_wobj;          /* SimulationInfo::Rotating_WObject */
+  float mill_rot_vel;
+  int num_cones;
+  /* discharge */
+  float kill_height;
+  bdem_float3 max_size;
+  int kill_enabled;
+  /* type tables (POBJ fields the sphere path reads) */
+  float t_radius[MAX_TYPES], t_mass[MAX_TYPES], t_inertia[MAX_TYPES];
+  float pp_kn[MAX_TYPES][MAX_TYPES], pp_cn[MAX_TYPES][MAX_TYPES], pp_fk[MAX_TYPES][MAX_TYPES];
+  float s_kn[MAX_TYPES], s_cn[MAX_TYPES], s_fs[MAX_TYPES], s_fk[MAX_TYPES];
+  LawPair law_pp[MAX_TYPES][MAX_TYPES];
+  LawPair law_w[MAX_TYPES];
+  bdem_cone cone[BDEM_MAX_CONES];
+  bdem_WOBJ wobj[MAX_WOBJ];
+};
+
+/* ------------------------------------------------------------------------ */
+/* a1: grid position + hash                                                 */
+/* SpatialDecomp_CalcGridPos, Collision_Detection_BroadPhase.cuh:20-29       */
+/* SpatialDecomp_CalcGridHash, Collision_Detection_BroadPhase.cuh:42-50      */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void grid_pos(const DevParams& P, f3 p, int g[3]) {
+  g[0] = (int)floorf((p.x - P.origin.x) / P.cell.x);
+  g[1] = (int)floorf((p.y - P.origin.y) / P.cell.y);
+  g[2] = (int)floorf((p.z - P.origin.z) / P.cell.z);
+}
+__device__ __forceinline__ uint32_t hash_of(const DevParams& P, int gx, int gy, int gz) {
+  return ((uint32_t)gx + (uint32_t)gz * (uint32_t)P.nz) + ((uint32_t)gy * (uint32_t)P.nx) * (uint32_t)P.nz;
+}
+/* Key of a particle: reference hash when in the grid, C for a live particle
+ * outside it, C+1 for a dead one.  Both sentinels sort behind every cell. */
+__device__ __forceinline__ uint32_t key_of(const DevParams& P, f3 p, int pid, uint32_t* status) {
+  if (pid < 0) return P.C + 1u;
+  int g[3];
+  grid_pos(P, p, g);
+  if (g[0] < 0 || g[1] < 0 || g[2] < 0 || g[0] >= P.nx || g[1] >= P.ny || g[2] >= P.nz) {
+    atomicOr(status, ST_OUT_OF_GRID);
+    return P.C;
+  }
+  uint32_t h = hash_of(P, g[0], g[1], g[2]);
+  return h < P.C ? h : P.C;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a5 traversal: 27-cell stencil in the reference order (y outer, z, x      */
+/* inner, ascending sorted index inside a cell), clamped to the grid.        */
+/* Collision_Detection_BroadPhase.cuh:428-489, 369-383.                      */
+/* The three x-neighbours of a (y,z) row have consecutive hashes, so their   */
+/* particles are one contiguous run of the sorted arrays: a row is scanned   */
+/* as a single index range.                                                  */
+/* ------------------------------------------------------------------------ */
+template <class Visit>
+__device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2* __restrict__ cells,
+                                                const float4* __restrict__ pos_r, int i, f3 pa,
+                                                float ra, Visit&& visit) {
+  int g[3];
+  grid_pos(P, pa, g);
+  const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
+  const int ys = max(g[1] - 1, 0), ye = min(g[1] + 1, P.ny - 1);
+  const int zs = max(g[2] - 1, 0), ze = min(g[2] + 1, P.nz - 1);
+  int count = 0;
+  for (int y = ys; y <= ye; ++y) {
+    for (int z = zs; z <= ze; ++z) {
+      if (xs > xe) continue;
+      const uint32_t h0 = hash_of(P, xs, y, z);
+      const uint32_t h1 = hash_of(P, xe, y, z);
+      if (h1 >= P.C) continue;
+      uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+      for (uint32_t h = h0; h <= h1; ++h) {
+        const uint2 c = cells[h];
+        if (c.x != 0xFFFFFFFFu) {
+          lo = min(lo, c.x);
+          hi = c.y;
+        }
+      }
+      if (lo == 0xFFFFFFFFu) continue;
+      for (uint32_t j = lo; j < hi; ++j) {
+        if ((int)j == i) continue;
+        const float4 pb = pos_r[j];
+        /* length(pA-pB) - (rA+rB) < 0 */
+        if (length(pa - xyz(pb)) - (ra + pb.w) < 0.0f) {
+          visit((int)j, pb, count);
+          ++count;
+        }
+      }
+    }
+  }
+  return count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a6: reference pair law on A from B.                                       */
+/* Kernel_ParticleInteraction_Spheres_NonSymmetry DEM_Kernels_Compute.cuh:656-712 */
+/* Collision_Response_Particle_Particle Collision_Response.cuh:374-494       */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3 va, float ma,
+                                         f3 wa, int ta, f3 pb, float rb, f3 vb, float mb, f3 wb,
+                                         int tb, f3& F, f3& T) {
+  const f3 relPos = pa - pb;
+  const f3 RVel = va - vb;
+  const f3 N = relPos / length(relPos);
+  const float dis = fabsf(length(relPos) - (ra + rb)) + 0.000010f;
+  const f3 RA = (-1.0f * N) * ra;
+  const f3 RB = N * rb;
+  const float Kn = P.pp_kn[ta][tb], Cn = P.pp_cn[ta][tb], Fk = P.pp_fk[ta][tb];
+  const f3 Vnormal = dot(RVel, N) * N;
+  const float Fnormal = Kn * dis;
+  const f3 Fdiss = Cn * Vnormal;
+  f3 Ft = Fnormal * N - Fdiss;
+  const f3 V_CPA = va + cross(wa, RA);
+  const f3 V_CPB = vb + cross(wb, RB);
+  const f3 V_CPNA = dot(V_CPA, N) * N;
+  const f3 V_CPNB = dot(V_CPB, N) * N;
+  const f3 Vt_A = V_CPA - V_CPNA;
+  const f3 Vt_B = V_CPB - V_CPNB;
+  const f3 Vt = Vt_A - Vt_B;
+  float mag = dot(Vt, Vt);
+  f3 Tang = mk3(0.f, 0.f, 0.f);
+  if (mag > 0.0f) {
+    mag = __builtin_sqrtf(mag);
+    float Velf = dot(RVel, RVel);
+    Velf = Velf > 0.0f ? __builtin_sqrtf(Velf) : 1.0f;
+    const float tan_f = (((0.50f * (ma + mb)) / P.dt) * 0.050f) * Fk * (mag / Velf);
+    Tang = (-1.0f * tan_f) * (Vt / mag);
+  }
+  Ft = Ft + Tang;
+  F = F + Ft;
+  if (P.rotation) T = T + cross(RA, Ft);
+}
+
+/* ------------------------------------------------------------------------ */
+/* History laws (extensions, textbook forms; same arithmetic as the oracle) */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ f3 rotate_spring(const DevParams& P, f3 xi_old, f3 n, f3 vt) {
+  f3 xi = xi_old - dot(xi_old, n) * n;
+  const float m_old = dot(xi_old, xi_old);
+  const float m_new = dot(xi, xi);
+  if (m_new > 0.0f) xi = xi * __builtin_sqrtf(m_old / m_new);
+  return xi + vt * P.dt;
+}
+
+__device__ __forceinline__ f3 history_law(const DevParams& P, f3 n, float delta, f3 vc,
+                                          float rstar, float mstar, const LawPair& lp, f3& xi) {
+  const float vn = dot(vc, n);
+  const f3 vt = vc - vn * n;
+  xi = rotate_spring(P, xi, n, vt);
+  float fn, kt, gt;
+  if (P.law == BDEM_LAW_LSD_COULOMB) {
+    const float cn = -2.0f * lp.beta * __builtin_sqrtf(lp.kn * mstar);
+    fn = lp.kn * delta - cn * vn;
+    kt = lp.kt;
+    gt = 0.0f;
+  } else {
+    const float srd = __builtin_sqrtf(rstar * delta);
+    const float sn = 2.0f * lp.estar * srd;
+    const float st = 8.0f * lp.gstar * srd;
+    const float gn = -1.8257418583505538f * lp.beta * __builtin_sqrtf(sn * mstar);
+    gt = -1.8257418583505538f * lp.beta * __builtin_sqrtf(st * mstar);
+    fn = (4.0f / 3.0f) * lp.estar * srd * delta - gn * vn;
+    kt = st;
+  }
+  if (fn < 0.0f) fn = 0.0f;
+  f3 ft = -1.0f * kt * xi - gt * vt;
+  const float ft2 = dot(ft, ft);
+  const float fc = lp.mu * fn;
+  if (ft2 > fc * fc) {
+    const float s = fc / __builtin_sqrtf(ft2);
+    ft = ft * s;
+    xi = (-1.0f / kt) * ft;
+  }
+  return fn * n + ft;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a8: walls                                                                 */
+/* ------------------------------------------------------------------------ */
+struct Coll { bool hit; float dis; f3 cp; f3 n; };
+
+/* is_PointOnSurface, Device/Utilities.cuh:20-53 */
+__device__ __forceinline__ bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, f3 point) {
+  float area = 0.0f;
+  const int nv = s.num_vertex;
+  for (int i = 0; i < nv - 1; ++i) {
+    const f3 V1 = point - cvt(w.vertex[s.vertex_Order[i]]);
+    const f3 V2 = point - cvt(w.vertex[s.vertex_Order[i + 1]]);
+    area = area + 0.500000f * length(cross(V1, V2));
+  }
+  const f3 V1 = point - cvt(w.vertex[s.vertex_Order[nv - 1]]);
+  const f3 V2 = point - cvt(w.vertex[s.vertex_Order[0]]);
+  area = area + 0.500000f * length(cross(V1, V2));
+  return (fabsf(area - s.area) / s.area) * 100.0000f <= s.Tol;
+}
+
+/* Collision_Detection_Sphere_Surface / _Plane, Collision_Detection_Spheres.cuh:6-100 */
+__device__ __forceinline__ Coll sphere_plane(f3 p, float r, f3 nrm, f3 cen, const bdem_Surface* s,
+                                             const bdem_WOBJ* w) {
+  Coll c{false, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  const f3 Pt = p - r * nrm;
+  float dis = dot(nrm, Pt - cen);
+  if (dis < 0.000000f && dis > -r) {
+    dis = fabsf(dis);
+    if (s) {
+      const f3 point = Pt + nrm * dis;
+      if (!point_on_surface(*s, *w, point)) return c;
+    }
+    c.hit = true; c.dis = dis; c.cp = Pt; c.n = nrm;
+  }
+  return c;
+}
+
+/* Collision_Detection_Sphere_Cylinder, Collision_Detection_Spheres.cuh:234-265 */
+__device__ __forceinline__ Coll sphere_cylinder(f3 p, float r, const bdem_Macro_Cylinder& cyl) {
+  Coll c{false, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  const f3 Pvec = p * cvt(cyl.AXIS) - cvt(cyl.center_bot_cap);
+  const float Pd = length(Pvec) + r;
+  if (Pd > cyl.radius) {
+    f3 n = -1.0f * Pvec;
+    n = n * (1.0f / length(n));
+    c.n = n;
+    c.cp = p - n * r;
+    c.dis = fabsf(Pd - cyl.radius);
+    c.hit = true;
+  }
+  return c;
+}
+
+/* Conical-frustum wall (extension): closest point of the meridian segment. */
+__device__ __forceinline__ Coll sphere_cone(f3 p, float r, const bdem_cone& cn) {
+  Coll c{false, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  const f3 ax = cvt(cn.axis);
+  const f3 d = p - cvt(cn.base);
+  const float s = dot(d, ax);
+  const f3 q = d - s * ax;
+  const float rho = length(q);
+  if (!(rho > 0.0f)) return c;
+  const float ds = cn.s1 - cn.s0, dr = cn.r1 - cn.r0;
+  float t = ((s - cn.s0) * ds + (rho - cn.r0) * dr) / (ds * ds + dr * dr);
+  t = fminf(fmaxf(t, 0.0f), 1.0f);
+  const float sc = cn.s0 + t * ds, rc = cn.r0 + t * dr;
+  const float es = s - sc, er = rho - rc;
+  const float dist = __builtin_sqrtf(es * es + er * er);
+  if (dist < r && dist > 0.0f) {
+    const f3 qh = q / rho;
+    const f3 n = (es / dist) * ax + (er / dist) * qh;
+    c.hit = true;
+    c.dis = r - dist;
+    c.n = n;
+    c.cp = p - n * r;
+  }
+  return c;
+}
+
+/* Collision_Response_Particle_StaticSurface, Collision_Response.cuh:4-109 */
+__device__ __forceinline__ void resp_static(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
+                                            const Coll& c, f3& F, f3& T) {
+  const f3 Rc = c.cp - p;
+  const f3 Vnormal = dot(v, c.n) * c.n;
+  const float Fnormal = P.s_kn[t] * c.dis;
+  const f3 Fdiss = P.s_cn[t] * Vnormal;
+  f3 Ftr = Fnormal * c.n - Fdiss;
+  const f3 Fdir = Ftr / length(Ftr);
+  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  f3 Vt = v - Vnormal;
+  if (P.rotation) Vt = Vt + cross(w, Rc);
+  float mag = dot(Vt, Vt);
+  f3 Tang = mk3(0.f, 0.f, 0.f);
+  if (mag > 0.0f) {
+    mag = __builtin_sqrtf(mag);
+    float Velf = dot(v, v);
+    Velf = Velf > 0.0f ? __builtin_sqrtf(Velf) : 1.0f;
+    const float tan_f = (0.050f * (m / P.dt)) * (P.s_fs[t] * (mag / Velf));
+    Tang = (-1.0f * tan_f) * (Vt / mag);
+  }
+  Ftr = Ftr + Tang;
+  F = F + Ftr;
+  if (P.rotation) T = T + cross(Rc, Ftr);
+}
+
+/* Collision_Response_Particle_MovingSurface, Collision_Response.cuh:116-233 */
+__device__ __forceinline__ f3 resp_moving(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
+                                          const Coll& c, f3 Vs, f3& Tout) {
+  const f3 Rc = c.cp - p;
+  const f3 RVel = v - Vs;
+  const f3 Vn = dot(RVel, c.n) * c.n;
+  const float Fnormal = P.s_kn[t] * c.dis;
+  const f3 Fdiss = P.s_cn[t] * Vn;
+  f3 Ftr = Fnormal * c.n - Fdiss;
+  const f3 Fdir = Ftr / length(Ftr);
+  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  const f3 V_CPA = v + cross(w, Rc);
+  const f3 V_CPNA = dot(V_CPA, c.n) * c.n;
+  const f3 Vt_A = V_CPA - V_CPNA;
+  const f3 Vt_B = Vs - dot(Vs, c.n) * c.n;
+  const f3 Vt = Vt_A - Vt_B;
+  float mag = dot(Vt, Vt);
+  f3 Tang = mk3(0.f, 0.f, 0.f);
+  if (mag > 0.0f) {
+    mag = __builtin_sqrtf(mag);
+    float Velf = dot(RVel, RVel);
+    Velf = Velf > 0.0f ? __builtin_sqrtf(Velf) : 1.0f;
+    const float tan_f = P.s_fk[t] * (mag / Velf);
+    Tang = ((-1.0f * tan_f) * (Vt / mag)) * (m / P.dt) * 0.050f;
+  }
+  Ftr = Ftr + Tang;
+  Tout = P.rotation ? cross(Rc, Ftr) : mk3(0.f, 0.f, 0.f);
+  return Ftr;
+}
+
+/* Collision_Response_EndPlate_Particle, Collision_Response.cuh:504-526 */
+__device__ __forceinline__ f3 resp_endplate(const DevParams& P, int t, f3 v, const Coll& c) {
+  const f3 Vn = dot(v, c.n) * c.n;
+  const f3 Fdiss = P.s_cn[t] * Vn;
+  const float Fn = P.s_kn[t] * c.dis;
+  return Fn * c.n - Fdiss;
+}
+
+/* Drum surface velocity, DEM_Kernels_Compute.cuh:375-378, 449-452 */
+__device__ __forceinline__ f3 drum_velocity(const DevParams& P, f3 cp) {
+  const float Rd = P.wobj[0].cyl_geo.radius;
+  return cross(mk3(0.0f, 0.0f, -P.mill_rot_vel), cp - mk3(Rd, Rd, 0.0f));
+}
+
+/* Look up a spring by id among `cnt` records, starting at `cursor` (the
+ * partner order is nearly unchanged from step to step, so the first probe
+ * usually hits). */
+__device__ __forceinline__ f3 find_spring(const HistRec* __restrict__ recs, int cnt, int id,
+                                          int& cursor) {
+  for (int k = 0; k < cnt; ++k) {
+    int s = cursor + k;
+    if (s >= cnt) s -= cnt;
+    const HistRec r = recs[s];
+    if (__float_as_int(r.w) == id) {
+      cursor = s + 1;
+      return mk3(r.x, r.y, r.z);
+    }
+  }
+  return mk3(0.f, 0.f, 0.f);
+}
+
+/* Wall force and torque on one particle.  The reference law keeps the
+ * accumulate/overwrite pattern of DEM_Kernels_Compute.cuh:283-604 (plane hits
+ * add; cylinder and cap hits assign the force; torques add).  History laws
+ * sum every wall contact and keep one spring per touched wall. */
+struct WallHist {
+  const HistRec* in; int in_cnt;
+  HistRec* out; int out_cnt;
+};
+
+__device__ __forceinline__ void wall_contact_hist(const DevParams& P, int t, f3 p, float r, f3 v,
+                                                  f3 w, float m, const Coll& c, f3 vw, int wall_id,
+                                                  f3& F, f3& T, WallHist& wh) {
+  const f3 la = c.cp - p;
+  const f3 vc = (v + cross(w, la)) - vw;
+  int cur = 0;
+  f3 xi = find_spring(wh.in, wh.in_cnt, wall_id, cur);
+  const f3 f = history_law(P, c.n, c.dis, vc, r, m, P.law_w[t], xi);
+  F = F + f;
+  if (P.rotation) T = T + cross(la, f);
+  if (wh.out_cnt < WMAX) {
+    wh.out[wh.out_cnt] = make_float4(xi.x, xi.y, xi.z, __int_as_float(wall_id));
+    ++wh.out_cnt;
+  }
+}
+
+__device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, float r, f3 v, f3 w,
+                                            float m, f3& F, f3& T, WallHist* wh) {
+  F = mk3(0.f, 0.f, 0.f);
+  T = mk3(0.f, 0.f, 0.f);
+  const bool hist = P.law != BDEM_LAW_REFERENCE;
+  const bool moving_planes = P.rotating_wobj && P.cyl_rotation;
+  for (int k = 0; k < P.num_wobj; ++k) {
+    const bdem_WOBJ& W = P.wobj[k];
+    if (W.surface_type == BDEM_SURFACE_PLANE) {
+      for (int s = 0; s < W.num_surfaces; ++s) {
+        const bdem_Surface& S = W.surfaces[s];
+        const Coll c = (S.area == -1.0f) ? sphere_plane(p, r, cvt(S.normal), cvt(S.centroid), nullptr, nullptr)
+                                         : sphere_plane(p, r, cvt(S.normal), cvt(S.centroid), &S, &W);
+        if (!c.hit) continue;
+        if (hist) {
+          const f3 vw = moving_planes ? drum_velocity(P, c.cp) : mk3(0.f, 0.f, 0.f);
+          wall_contact_hist(P, t, p, r, v, w, m, c, vw, k * 16 + s, F, T, *wh);
+        } else if (moving_planes) {
+          f3 tq;
+          F = F + resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq);
+          T = T + tq;
+        } else {
+          resp_static(P, t, p, v, w, m, c, F, T);
+        }
+      }
+    } else if (W.surface_type == BDEM_SURFACE_CYLINDER) {
+      const bdem_Macro_Cylinder& cyl = W.cyl_geo;
+      const Coll c = sphere_cylinder(p, r, cyl);
+      if (c.hit) {
+        if (hist) {
+          const f3 vw = P.cyl_rotation ? drum_velocity(P, c.cp) : mk3(0.f, 0.f, 0.f);
+          wall_contact_hist(P, t, p, r, v, w, m, c, vw, 128 + k, F, T, *wh);
+        } else if (P.cyl_rotation) {
+          f3 tq;
+          F = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq);
+          T = T + tq;
+        } else {
+          f3 f = mk3(0.f, 0.f, 0.f), tq = mk3(0.f, 0.f, 0.f);
+          resp_static(P, t, p, v, w, m, c, f, tq);
+          F = f;
+          T = T + tq;
+        }
+      }
+      if (cyl.has_top_cap) {
+        const Coll cc = sphere_plane(p, r, cvt(cyl.normal_top_cap), cvt(cyl.center_top_cap), nullptr, nullptr);
+        if (cc.hit) {
+          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 144 + k, F, T, *wh);
+          else F = resp_endplate(P, t, v, cc);
+        }
+      }
+      if (cyl.has_bot_cap) {
+        const Coll cc = sphere_plane(p, r, cvt(cyl.normal_bot_cap), cvt(cyl.center_bot_cap), nullptr, nullptr);
+        if (cc.hit) {
+          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 160 + k, F, T, *wh);
+          else F = resp_endplate(P, t, v, cc);
+        }
+      }
+    }
+  }
+  for (int k = 0; k < P.num_cones; ++k) {
+    const Coll c = sphere_cone(p, r, P.cone[k]);
+    if (!c.hit) continue;
+    if (hist) wall_contact_hist(P, t, p, r, v, w, m, c, mk3(0.f, 0.f, 0.f), 176 + k, F, T, *wh);
+    else resp_static(P, t, p, v, w, m, c, F, T);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a10: Integrate_Euler_NonSymmetry (Integration.cuh:6-184), spheres.         */
+/* ------------------------------------------------------------------------ */
+struct QuatF { float w, x, y, z; };
+
+/* Quaternion product, Utilities/Device/Opertators.cuh:866-874 */
+__device__ __forceinline__ QuatF qmul(QuatF a, QuatF b) {
+  QuatF q;
+  q.x = (a.x * b.w) + (a.y * b.z) - (a.z * b.y) + (a.w * b.x);
+  q.y = -(a.x * b.z) + (a.y * b.w) + (a.z * b.x) + (a.w * b.y);
+  q.z = (a.x * b.y) - (a.y * b.x) + (a.z * b.w) + (a.w * b.z);
+  q.w = -(a.x * b.x) - (a.y * b.y) - (a.z * b.z) + (a.w * b.w);
+  return q;
+}
+/* normaliseD, Utilities/Device/Functions.cuh:399-420 */
+__device__ __forceinline__ QuatF qnormalise(QuatF a) {
+  const float s = a.w * a.w + a.x * a.x + a.y * a.y + a.z * a.z;
+  const float n = s > 0.0f ? 1.0f / __builtin_sqrtf(s) : 0.0f;
+  return QuatF{a.w * n, a.x * n, a.y * n, a.z * n};
+}
+
+__device__ __forceinline__ void integrate(const DevParams& P, float inv_inertia, float m, f3 Fpp,
+                                          f3 Tpp, f3 Fw, f3 Tw, f3& p, f3& v, f3& w, QuatF* q) {
+  const float dt = P.dt;
+  f3 acc = Fw + Fpp;
+  acc = acc * (1.0f / m);
+  v = v + acc * dt;
+  p = p + v * dt;
+  if (P.mode2d) v.z = 0.0f;
+  const f3 fl = Tpp + Tw;
+  if (P.rotation) {
+    if (dot(fl, fl) > 0.0f) {
+      const f3 aa = inv_inertia * fl;
+      w = w + aa * dt;
+      w = w - w * P.roll_res;
+    }
+    if (P.sphere_orient && q) {
+      float mo = dot(w, w);
+      QuatF dQ{1.0f, 0.0f, 0.0f, 0.0f};
+      if (mo > 0.000000f) {
+        mo = __builtin_sqrtf(mo);
+        const f3 dir = w / mo;
+        dQ.w = cosf(0.5000f * mo * dt);
+        const float sv = sinf(0.5000f * mo * dt);
+        dQ.x = sv * dir.x; dQ.y = sv * dir.y; dQ.z = sv * dir.z;
+      }
+      const QuatF Q = qnormalise(qmul(dQ, *q));
+      *q = (Q.w <= -1.00000f) ? dQ : Q;
+      if (P.mode2d) { q->x = 0.0f; q->y = 0.0f; }
+    }
+    if (P.mode2d) { w.x = 0.0f; w.y = 0.0f; }
+  }
+}
+
+}  // namespace bdem
