@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Plain-step throughput of the DEM engine on one MI355X.
+
+Workload: silo discharge, 1,048,576 polydisperse spheres (r ~ U[2, 4] mm,
+rho 2600), Hertz-Mindlin (E 1e7 Pa, nu 0.3, e 0.5, mu 0.5) with tangential
+spring history, cylindrical silo R 25 cm over a 30-degree conical hopper with
+a 6 cm outlet, cell 2*rmax, dt 5e-6 s.  Setup (untimed): a seeded FCC fill,
+settled with the outlet closed at a larger time step, then the outlet opens
+(Device_OpenHatch) and the time step returns to 5e-6.  W warm-up steps, then
+exactly K timed steps of the full step (sort, gather, cell table, fused
+contact/force/wall/integrate kernel), bracketed by stream synchronisation.
+
+Metric: particle-steps/s = live particles x K / elapsed.  One JSON line on
+stdout; progress on stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "softx-d-15-00085_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_device(bdem, deck):
+    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"])
+    dev.set_ext(deck["ext"])
+    dev.set_radii(deck["radius"], deck["mass"])
+    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"])
+    return dev
+
+
+def step_kernel_bytes(n, k, ncells):
+    """Algorithmic bytes of one fused-step launch (history law), per the
+    stage I/O: reads state 48 + sorted key 4 + spring count 4 + springs 16k
+    + wall-spring count 4; writes state 48 + key 4 + changed flag 1 + spring
+    count 4 + springs 16k + wall-spring count 4; plus the cell table 8C once."""
+    per = 48 + 4 + 4 + 16 * k + 4 + 48 + 4 + 1 + 4 + 16 * k + 4
+    return n * per + 8 * ncells
+
+
+def step_total_bytes(n, k, ncells):
+    """Whole-step algorithmic bytes: 320 + 8k + 8C/N + 64k per particle
+    (history law), the per-particle-step figure of the measurement plan."""
+    return n * (320 + 8 * k + 64 * k) + 8 * ncells
+
+
+def cpu_baseline(deck, dev, settled_state, target_s=15.0):
+    """Time the FP32 CPU restatement (OpenMP) on the settled state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    from oracle_py import Oracle
+    d = dict(deck)
+    d["pos"] = settled_state["pos"]
+    d["vel"] = settled_state["vel"]
+    d["avel"] = settled_state["avel"]
+    d["ptype"] = settled_state["ptype"]
+    d["pid"] = settled_state["pid"]
+    # radius and mass travel with the particle: the deck's arrays are indexed
+    # by the original slot, which is the P_ID (dead particles keep slot 0's)
+    slot = np.clip(settled_state["pid"], 0, None)
+    d["radius"] = deck["radius"][slot]
+    d["mass"] = deck["mass"][slot]
+    ora = Oracle(d, bits=32)
+    ora.set_active_world_objects(len(deck["wobj"]) - 1)  # hatch open
+    t1 = ora.time_steps(1)
+    steps = max(1, min(20, int(target_s / max(t1, 1e-3))))
+    el = ora.time_steps(steps)
+    ora.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=deck["sim"].Num_Particles * steps / el, unit="particle-steps/s", cores=cores,
+                kind="port",
+                sample=f"{steps} steps (after 1 untimed) of the settled {deck['sim'].Num_Particles}-sphere "
+                       f"silo, hatch open, FP32 OpenMP restatement, tangential springs reset")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--settle", type=int, default=24000, help="untimed settle steps, outlet closed")
+    ap.add_argument("--settle-dt", type=float, default=2e-5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import bdem
+    from bdem import scenarios as S
+
+    t0 = time.time()
+    deck = S.silo_deck(args.n)
+    sim = deck["sim"]
+    dt = sim.InitalDelta_t
+    dev = make_device(bdem, deck)
+    ncells = dev.num_cells()
+    log(f"[bench] silo n={args.n} cells={ncells} setup {time.time() - t0:.1f}s")
+
+    # settle with the outlet closed at a larger time step
+    sim.InitalDelta_t = args.settle_dt
+    dev.update_sim_info(sim)
+    done = 0
+    chunk = 2000
+    while done < args.settle:
+        m = min(chunk, args.settle - done)
+        dev.step(m)
+        dev.sync()
+        done += m
+        log(f"[bench] settle {done}/{args.settle} steps ({time.time() - t0:.1f}s)")
+    st = dev.get_state()
+    live = st["pid"] >= 0
+    vmax = float(np.max(np.linalg.norm(st["vel"][live], axis=1)))
+    top = float(np.max(st["pos"][live, 1]))
+    log(f"[bench] settled: bed top y={top:.1f} cm, max |v| {vmax:.1f} cm/s")
+
+    # open the outlet, back to the configured time step
+    sim.InitalDelta_t = dt
+    dev.update_sim_info(sim)
+    dev.open_hatch()
+    _, nn = dev.debug_contacts()
+    k = float(nn[live].mean())
+    dev.step(args.warmup)
+    dev.sync()
+    n_live = int(np.count_nonzero(dev.get_state()["pid"] >= 0))
+    log(f"[bench] warmup {args.warmup} done; live {n_live}, mean contacts/particle {k:.2f}")
+
+    dev.sync()
+    dev.timing_begin(args.steps)
+    ta = time.perf_counter()
+    dev.step(args.steps)
+    dev.sync()
+    tb = time.perf_counter()
+    stages, nrec = dev.timing_end()
+    elapsed = tb - ta
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = n_live * args.steps / elapsed
+    status = dev.status()
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step; stage ms/step "
+        + ", ".join(f"{s}={v / nrec:.4f}" for s, v in stages.items()) + f"; status {status}")
+
+    step_ms = stages["step"] / nrec
+    kb = step_kernel_bytes(n_live, k, ncells)
+    achieved = kb / (step_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_step_silo1m_traffic.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = float(line.split(":")[1])
+    tot_b = step_total_bytes(n_live, k, ncells)
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel="k_step<HERTZ_MINDLIN> (fused contact+force+wall+integrate)",
+                algorithmic_bytes_per_launch=int(kb), avg_launch_ms=round(step_ms, 5))
+    step_roof = dict(bytes_per_step=int(tot_b),
+                     achieved=round(tot_b / (ms_per_step * 1e-3) / 1e9, 1),
+                     frac=round(tot_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (FP32 restatement)...")
+        cpu = cpu_baseline(deck, dev, dev.get_state())
+        log(f"[bench] CPU baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+    dev.close()
+
+    out = {
+        "metric": "particle-steps/s at 1M spheres (plain step, silo discharge)",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "silo discharge, 1,048,576 polydisperse spheres, Hertz-Mindlin + "
+                               "tangential history, dt 5e-6 s, outlet open",
+                   "particles": n_live, "cells": ncells, "mean_contacts": round(k, 3),
+                   "settle_steps": args.settle, "settle_dt": args.settle_dt,
+                   "stage_ms_per_step": {s: round(v / nrec, 5) for s, v in stages.items()}},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

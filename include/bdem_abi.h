@@ -67,6 +67,12 @@ int bdem_set_simulation_data(bdem_ctx* ctx, const bdem_WOBJ* wobj, const bdem_PO
  * bdem_set_simulation_data.  No reference counterpart. */
 int bdem_set_ext_params(bdem_ctx* ctx, const bdem_ext_params* ext);
 
+/* Re-upload SimulationInfo (time step, gravity, flags ...) between steps, as
+ * the reference re-uploads SimParms after fills and kills
+ * (Device/DeviceInterface.cu:1095, 2089).  Particle count and grid
+ * dimensions are fixed at setup. */
+int bdem_update_simulation_info(bdem_ctx* ctx, const bdem_SimulationInfo* sim);
+
 /* ---- state in ----------------------------------------------------------- */
 /* Device_Set_Pos, Device/DeviceInterface.h:22-23 (ornt may be NULL for spheres). */
 int bdem_set_pos(bdem_ctx* ctx, const bdem_float3* pos, const bdem_Quaterion* ornt,
@@ -142,6 +148,16 @@ int bdem_update_sim(bdem_ctx* ctx);
 int bdem_step(bdem_ctx* ctx, int nsteps);
 /* Wait for the context's stream. */
 int bdem_sync(bdem_ctx* ctx);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Record HIP events on the context's stream around each stage of the next
+ * max_steps bdem_update_sim calls.  No reference counterpart (the reference
+ * has no timers). */
+int bdem_timing_begin(bdem_ctx* ctx, int max_steps);
+/* Wait, then return the summed milliseconds of the 4 stages
+ * {hash+sort, gather, cell table, fused step kernel} and the number of
+ * steps recorded; releases the events. */
+int bdem_timing_end(bdem_ctx* ctx, double* stage_ms, int* nsteps);
 
 /* ---- stage outputs for parity checks (synchronous; state unchanged) -------- */
 /* Keys and identity index of the current state (Kernel_SpatialDecomp_CalcPHash). */

@@ -38,6 +38,7 @@ _PROTOS = {
                                            C.POINTER(T.SimulationInfo), C.c_int, C.c_int, C.c_int,
                                            C.POINTER(T.InitConfig)]),
     "bdem_set_ext_params": (C.c_int, [_P, C.POINTER(T.ExtParams)]),
+    "bdem_update_simulation_info": (C.c_int, [_P, C.POINTER(T.SimulationInfo)]),
     "bdem_set_pos": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
     "bdem_set_pos_velocity": (C.c_int, [_P, _f3p, _qp, _f3p, _f3p, _up, _ip]),
     "bdem_set_system_state": (C.c_int, [_P, _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip]),
@@ -65,6 +66,8 @@ _PROTOS = {
     "bdem_update_sim": (C.c_int, [_P]),
     "bdem_step": (C.c_int, [_P, C.c_int]),
     "bdem_sync": (C.c_int, [_P]),
+    "bdem_timing_begin": (C.c_int, [_P, C.c_int]),
+    "bdem_timing_end": (C.c_int, [_P, C.POINTER(C.c_double), _ip]),
     "bdem_debug_hash": (C.c_int, [_P, _up, _ip]),
     "bdem_debug_sorted": (C.c_int, [_P, _up, _ip]),
     "bdem_debug_cells": (C.c_int, [_P, _up, _up]),
@@ -146,6 +149,10 @@ class Device:
         self._ext = ext
         self._check(self.lib.bdem_set_ext_params(self.h, C.byref(ext)))
 
+    def update_sim_info(self, sim):
+        self.sim = sim
+        self._check(self.lib.bdem_update_simulation_info(self.h, C.byref(sim)))
+
     def set_state(self, pos, ptype, pid, vel=None, avel=None, quat=None):
         pos = _f3(pos)
         ptype = np.ascontiguousarray(ptype, dtype=np.uint32)
@@ -208,6 +215,18 @@ class Device:
 
     def drum_rotation(self, on):
         self._check(self.lib.bdem_drum_rotation(self.h, 1 if on else 0))
+
+    # -- measurement ------------------------------------------------------
+    STAGES = ("sort", "gather", "cells", "step")
+
+    def timing_begin(self, max_steps):
+        self._check(self.lib.bdem_timing_begin(self.h, int(max_steps)))
+
+    def timing_end(self):
+        ms = (C.c_double * 4)()
+        n = C.c_int(0)
+        self._check(self.lib.bdem_timing_end(self.h, ms, C.byref(n)))
+        return dict(zip(self.STAGES, list(ms))), n.value
 
     # -- stage outputs ----------------------------------------------------
     def debug_hash(self):

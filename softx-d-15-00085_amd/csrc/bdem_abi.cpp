@@ -40,10 +40,18 @@ struct bdem_ctx {
   StepBuffers b{};
   std::vector<void*> allocs;
   bool orient = false;
-  bool keys_valid = false;
+  bool keys_valid = false;   /* keys[0] holds the keys of the current state */
+  bool flags_valid = false;  /* ... and the step kernel's changed flags for them */
   int hist_par = 0;
   int stepc = 0;
+  /* stage timing: one event per stage boundary per recorded step */
+  std::vector<hipEvent_t> tev;
+  int t_cap = 0, t_used = 0;
 };
+
+/* Stage boundaries recorded per step when timing is on:
+ * 0 start | hash+sort | gather | cells | step (fused kernel) */
+static const int BDEM_NSTAGE = 4;
 
 namespace {
 
@@ -73,7 +81,7 @@ int dalloc(bdem_ctx* c, T** p, size_t count) {
 }
 
 void free_all(bdem_ctx* c) {
-  for (void* p : c->allocs) hipFree(p);
+  for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->b = StepBuffers{};
   c->dp = nullptr;
@@ -249,13 +257,25 @@ int download(bdem_ctx* c, std::vector<float4>& P, std::vector<float4>& V, std::v
   return BDEM_OK;
 }
 
+hipEvent_t* timing_slot(bdem_ctx* c) {
+  if (c->t_used >= c->t_cap) return nullptr;
+  return &c->tev[(size_t)c->t_used * (BDEM_NSTAGE + 1)];
+}
+
 /* hash (if needed) + sort + gather + cells: the sorted scratch state. */
-int prepare_sorted(bdem_ctx* c, int* sorted_buf) {
+int prepare_sorted(bdem_ctx* c, int* sorted_buf, hipEvent_t* ev = nullptr) {
+  if (ev) (void)hipEventRecord(ev[0], c->stream);
+  const bool incremental = c->keys_valid && c->flags_valid;
   if (!c->keys_valid) launch_hash(c->dp, c->b, c->stream);
-  const int bits = key_bits(c->hp.C + 1u);
-  const int cur = launch_sort(c->b, bits, c->stream);
+  const int cur = launch_sort(c->b, c->hp.C + 1u, incremental, c->stream);
+  /* the sort may reuse keys[0]; the next step must produce new keys */
+  c->keys_valid = false;
+  c->flags_valid = false;
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
   launch_gather(c->b, c->b.vals[cur], c->orient, c->stream);
+  if (ev) (void)hipEventRecord(ev[2], c->stream);
   launch_cells(c->dp, c->b, c->b.keys[cur], c->hp.C, c->stream);
+  if (ev) (void)hipEventRecord(ev[3], c->stream);
   HIPCHK(c, hipGetLastError());
   if (sorted_buf) *sorted_buf = cur;
   return BDEM_OK;
@@ -283,10 +303,11 @@ int bdem_create(bdem_ctx** out, int device) {
 
 int bdem_destroy(bdem_ctx* c) {
   if (!c) return BDEM_EINVAL;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   free_all(c);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BDEM_OK;
 }
@@ -351,6 +372,12 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   if (c->orient) { ALLOC(b.quat_cur, n); ALLOC(b.quat_srt, n); }
   ALLOC(b.keys[0], n); ALLOC(b.keys[1], n); ALLOC(b.vals[0], n); ALLOC(b.vals[1], n);
   ALLOC(b.radix_hist, (size_t)RADIX * sort_blocks(n));
+  ALLOC(b.changed, n);
+  ALLOC(b.blk_changed, (size_t)(n + 255) / 256);
+  ALLOC(b.rkeys, n);
+  ALLOC(b.rvals, n);
+  ALLOC(b.sbuf, n);
+  ALLOC(b.sort_meta, 2);
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(b.wall_force, (size_t)n * 3);
@@ -408,6 +435,24 @@ int bdem_set_ext_params(bdem_ctx* c, const bdem_ext_params* ext) {
       launch_fill_u32((uint32_t*)c->b.wcount[h], n, 0u, c->stream);
     }
   }
+  return upload_params(c);
+}
+
+int bdem_update_simulation_info(bdem_ctx* c, const bdem_SimulationInfo* sim) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!sim) return fail(c, BDEM_EINVAL, "null argument");
+  const bdem_SimulationInfo& o = c->sim;
+  if (sim->Num_Particles != o.Num_Particles || sim->num_NNCells.x != o.num_NNCells.x ||
+      sim->num_NNCells.y != o.num_NNCells.y || sim->num_NNCells.z != o.num_NNCells.z ||
+      sim->particle_type != o.particle_type || sim->sphere_orient != o.sphere_orient)
+    return fail(c, BDEM_EINVAL, "particle count, grid size, particle type and orientation are fixed at setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const bool same_grid = std::memcmp(&sim->worldOrigin, &o.worldOrigin, sizeof(bdem_float3)) == 0 &&
+                         std::memcmp(&sim->cellSize, &o.cellSize, sizeof(bdem_float3)) == 0;
+  c->sim = *sim;
+  if (!same_grid) c->keys_valid = false;
   return upload_params(c);
 }
 
@@ -617,17 +662,25 @@ int bdem_update_sim(bdem_ctx* c) {
   int rc = require(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  hipEvent_t* ev = timing_slot(c);
+  int cur = 0;
+  if ((rc = prepare_sorted(c, &cur, ev)) != BDEM_OK) return rc;
   bool do_kill = false;
   if (c->sim.Kill_Particles) {
     const int interval = (c->has_ext && c->ext.kill_check_interval > 0) ? c->ext.kill_check_interval : 1000;
     if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
     else c->stepc += 1;
   }
-  launch_step(c->dp, c->b, c->hp.law, c->orient, c->hist_par, do_kill, c->record_wf, c->stream);
+  launch_step(c->dp, c->b, c->b.keys[cur], c->hp.law, c->orient, c->hist_par, do_kill, c->record_wf,
+              c->stream);
+  if (ev) {
+    (void)hipEventRecord(ev[4], c->stream);
+    c->t_used += 1;
+  }
   HIPCHK(c, hipGetLastError());
   if (need_history(c)) c->hist_par ^= 1;
   c->keys_valid = true;
+  c->flags_valid = true;
   return BDEM_OK;
 }
 
@@ -647,6 +700,42 @@ int bdem_sync(bdem_ctx* c) {
   return BDEM_OK;
 }
 
+int bdem_timing_begin(bdem_ctx* c, int max_steps) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (max_steps < 0 || max_steps > 1000000) return fail(c, BDEM_EINVAL, "max_steps out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+  c->tev.assign((size_t)max_steps * (BDEM_NSTAGE + 1), nullptr);
+  for (auto& e : c->tev) HIPCHK(c, hipEventCreate(&e));
+  c->t_cap = max_steps;
+  c->t_used = 0;
+  return BDEM_OK;
+}
+
+int bdem_timing_end(bdem_ctx* c, double* stage_ms, int* nsteps) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double acc[BDEM_NSTAGE] = {0, 0, 0, 0};
+  for (int s = 0; s < c->t_used; ++s) {
+    hipEvent_t* ev = &c->tev[(size_t)s * (BDEM_NSTAGE + 1)];
+    for (int k = 0; k < BDEM_NSTAGE; ++k) {
+      float ms = 0.f;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      acc[k] += ms;
+    }
+  }
+  if (stage_ms) for (int k = 0; k < BDEM_NSTAGE; ++k) stage_ms[k] = acc[k];
+  if (nsteps) *nsteps = c->t_used;
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
+  c->tev.clear();
+  c->t_cap = c->t_used = 0;
+  return BDEM_OK;
+}
+
 int bdem_debug_hash(bdem_ctx* c, unsigned int* keys, int* idx) {
   int rc = require(c);
   if (rc) return rc;
@@ -657,6 +746,7 @@ int bdem_debug_hash(bdem_ctx* c, unsigned int* keys, int* idx) {
   HIPCHK(c, hipMemcpy(keys, c->b.keys[0], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i) idx[i] = i;
   c->keys_valid = true;
+  c->flags_valid = false;
   return BDEM_OK;
 }
 
@@ -701,8 +791,8 @@ int bdem_debug_contacts(bdem_ctx* c, unsigned int* list32, unsigned int* nn) {
   hipError_t e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) e = hipMemcpy(list32, dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(nn, dn, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost);
-  hipFree(dl);
-  hipFree(dn);
+  (void)hipFree(dl);
+  (void)hipFree(dn);
   c->keys_valid = false;
   if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("debug_contacts: ") + hipGetErrorString(e));
   return BDEM_OK;
@@ -721,7 +811,7 @@ int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw,
   hipError_t e = hipStreamSynchronize(c->stream);
   std::vector<float> h((size_t)n * 12);
   if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("debug_forces: ") + hipGetErrorString(e));
   std::memcpy(fpp, h.data(), (size_t)n * 3 * sizeof(float));
   std::memcpy(tpp, h.data() + 3 * (size_t)n, (size_t)n * 3 * sizeof(float));

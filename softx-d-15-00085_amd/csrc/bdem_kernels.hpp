@@ -18,6 +18,15 @@ struct StepBuffers {
   uint32_t *keys[2];                   /* radix ping-pong keys */
   int *vals[2];                        /* radix ping-pong payload (original slot) */
   uint32_t *radix_hist;                /* 256 * ceil(n / tile) counters */
+  /* incremental sort: changed flags from the step kernel, per-256 counts,
+   * the unchanged run (rkeys/rvals), the changed set (sbuf) and
+   * sort_meta = {|S|, fallback flag} */
+  uint8_t *changed;
+  uint32_t *blk_changed;
+  uint32_t *rkeys;
+  int *rvals;
+  unsigned long long *sbuf;
+  uint32_t *sort_meta;
   uint2 *cells;                        /* (start, end) per cell, reference format */
   /* tangential history, indexed by P_ID, double-buffered across steps */
   int *hcount[2]; HistRec *hrec[2];
@@ -35,19 +44,23 @@ constexpr int RADIX = 256;
 
 inline int sort_blocks(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 int key_bits(uint32_t max_key);
+int sort_passes(uint32_t max_key);
 
 /* Individual stages (all asynchronous on `s`). */
 void launch_hash(const DevParams* dP, const StepBuffers& b, hipStream_t s);
-/* Stable LSD radix sort of keys[0] (payload: identity) over `bits` bits.
- * Returns the index (0 or 1) of the buffer pair holding the result. */
-int launch_sort(const StepBuffers& b, int bits, hipStream_t s);
+/* Stable sort of keys[0] (payload: slot index) for keys <= max_key.
+ * incremental: keys[0] came from the step kernel together with its changed
+ * flags; otherwise a full radix sort runs.  Returns the index (0 or 1) of the
+ * buffer pair holding the result. */
+int launch_sort(const StepBuffers& b, uint32_t max_key, bool incremental, hipStream_t s);
 void launch_gather(const StepBuffers& b, const int* idx, bool orient, hipStream_t s);
 void launch_cells(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys,
                   uint32_t ncells, hipStream_t s);
 /* Fused contact search + pair force + walls + integration, srt -> cur.
- * Writes next-step keys into keys[0]. */
-void launch_step(const DevParams* dP, const StepBuffers& b, int law, bool orient, int hist_par,
-                 bool do_kill, bool want_wall_force, hipStream_t s);
+ * Writes next-step keys into keys[0] and the changed flags/counts against
+ * sorted_keys (this step's sorted keys). */
+void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
+                 bool orient, int hist_par, bool do_kill, bool want_wall_force, hipStream_t s);
 
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,

@@ -102,3 +102,37 @@ def test_step_states_match(pair, steps):
         assert err <= TOL, f"{name} {key}: {err}"
     # the dense random packs eject particles: both sides must agree on it
     assert bool(dev.status() & bdem.lib.WARN_OUT_OF_GRID) == bool(ora.warnings() & 1)
+
+
+def test_incremental_sort_matches_full_sort():
+    """After steps, the next sort runs the incremental path (partition + LDS
+    sort + merge); its order must equal the oracle's stable sort."""
+    deck = S.random_deck(6000, seed=21, L=14.0, law="lsd", polydisperse=True, vmax=400.0)
+    dev, ora = make_pair(deck)
+    for rounds in range(3):
+        dev.step(7)
+        ora.step(7)
+        # debug_sorted on the GPU consumes the step kernel's changed flags
+        kg, ig = dev.debug_sorted()
+        ko, io = ora.sorted()
+        assert np.array_equal(kg, ko), f"round {rounds}: keys differ"
+        assert np.array_equal(ig, io), f"round {rounds}: order differs"
+    dev.close()
+    ora.close()
+
+
+def test_sort_fallback_when_many_particles_change_cell():
+    """Fast particles: more than 16384 change cell per step, so the
+    incremental path hands over to the full radix sort inside the step."""
+    deck = S.random_deck(40000, seed=22, L=30.0, radius=0.3, vmax=2.0e5, dt=1e-5)
+    dev, ora = make_pair(deck)
+    dev.step(3)
+    ora.step(3)
+    sg, so = by_pid(dev.get_state()), by_pid(ora.state())
+    assert np.array_equal(sg["pid"], so["pid"])
+    assert rel_err(sg["pos"], so["pos"], 1e-3) <= TOL
+    kg, ig = dev.debug_sorted()
+    ko, io = ora.sorted()
+    assert np.array_equal(kg, ko) and np.array_equal(ig, io)
+    dev.close()
+    ora.close()
