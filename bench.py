@@ -91,6 +91,10 @@ def main():
     ap.add_argument("--settle", type=int, default=24000, help="untimed settle steps, outlet closed")
     ap.add_argument("--settle-dt", type=float, default=2e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
+    ap.add_argument("--load-state", default=None,
+                    help="start from a state written by --save-state instead of settling "
+                         "(tangential springs restart at zero)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -111,18 +115,31 @@ def main():
     ncells = dev.num_cells()
     log(f"[bench] silo n={args.n} cells={ncells} setup {time.time() - t0:.1f}s")
 
-    # settle with the outlet closed at a larger time step
-    sim.InitalDelta_t = args.settle_dt
-    dev.update_sim_info(sim)
-    done = 0
-    chunk = 2000
-    while done < args.settle:
-        m = min(chunk, args.settle - done)
-        dev.step(m)
-        dev.sync()
-        done += m
-        log(f"[bench] settle {done}/{args.settle} steps ({time.time() - t0:.1f}s)")
+    if args.load_state:
+        z = np.load(args.load_state)
+        dev.set_radii(z["radius"], z["mass"])
+        dev.set_state(z["pos"], z["ptype"], z["pid"], vel=z["vel"], avel=z["avel"])
+        args.settle = int(z["settle"])
+        log(f"[bench] loaded settled state {args.load_state}")
+    else:
+        # settle with the outlet closed at a larger time step
+        sim.InitalDelta_t = args.settle_dt
+        dev.update_sim_info(sim)
+        done = 0
+        chunk = 2000
+        while done < args.settle:
+            m = min(chunk, args.settle - done)
+            dev.step(m)
+            dev.sync()
+            done += m
+            log(f"[bench] settle {done}/{args.settle} steps ({time.time() - t0:.1f}s)")
     st = dev.get_state()
+    if args.save_state:
+        slot = np.clip(st["pid"], 0, None)
+        np.savez(args.save_state, pos=st["pos"], vel=st["vel"], avel=st["avel"], ptype=st["ptype"],
+                 pid=st["pid"], radius=deck["radius"][slot], mass=deck["mass"][slot],
+                 settle=args.settle)
+        log(f"[bench] saved settled state to {args.save_state}")
     live = st["pid"] >= 0
     vmax = float(np.max(np.linalg.norm(st["vel"][live], axis=1)))
     top = float(np.max(st["pos"][live, 1]))

@@ -132,6 +132,26 @@ __device__ __forceinline__ uint32_t key_of(const DevParams& P, f3 p, int pid, ui
 /* particles are one contiguous run of the sorted arrays: a row is scanned   */
 /* as a single index range.                                                  */
 /* ------------------------------------------------------------------------ */
+/* The reference's overlap test, length(pA-pB) - (rA+rB) < 0
+ * (Collision_Detection_BroadPhase.cuh:376), decided without a square root
+ * whenever the answer is certain.  With d2 = fl(dot(d,d)) (the same value
+ * the reference takes the root of) and s = fl(rA+rB):
+ *   d2 < fl(fl(s*s)*(1-2^-20))  =>  sqrt(d2) < s(1-2^-22), at least two ulps
+ *                                   below s, so fl(sqrt(d2)) < s: overlap;
+ *   d2 > fl(fl(s*s)*(1+2^-20))  =>  sqrt(d2) > s(1+2^-22), so
+ *                                   fl(sqrt(d2)) >= s: no overlap.
+ * Only the band in between evaluates the reference expression itself, so
+ * the decision is bit-identical to it. */
+__device__ __forceinline__ bool overlaps(f3 pa, float ra, float4 pb) {
+  const f3 d = pa - xyz(pb);
+  const float d2 = dot(d, d);
+  const float s = ra + pb.w;
+  const float s2 = s * s;
+  if (d2 < s2 * 0.99999904632568359375f) return true;   /* 1 - 2^-20 */
+  if (d2 > s2 * 1.00000095367431640625f) return false;  /* 1 + 2^-20 */
+  return __builtin_sqrtf(d2) - s < 0.0f;
+}
+
 template <class Visit>
 __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2* __restrict__ cells,
                                                 const float4* __restrict__ pos_r, int i, f3 pa,
@@ -160,8 +180,7 @@ __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2*
       for (uint32_t j = lo; j < hi; ++j) {
         if ((int)j == i) continue;
         const float4 pb = pos_r[j];
-        /* length(pA-pB) - (rA+rB) < 0 */
-        if (length(pa - xyz(pb)) - (ra + pb.w) < 0.0f) {
+        if (overlaps(pa, ra, pb)) {
           visit((int)j, pb, count);
           ++count;
         }
@@ -178,7 +197,7 @@ __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2*
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3 va, float ma,
                                          f3 wa, int ta, f3 pb, float rb, f3 vb, float mb, f3 wb,
-                                         int tb, f3& F, f3& T) {
+                                         int tb, f3& Fc, f3& Tc) {
   const f3 relPos = pa - pb;
   const f3 RVel = va - vb;
   const f3 N = relPos / length(relPos);
@@ -207,8 +226,8 @@ __device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3
     Tang = (-1.0f * tan_f) * (Vt / mag);
   }
   Ft = Ft + Tang;
-  F = F + Ft;
-  if (P.rotation) T = T + cross(RA, Ft);
+  Fc = Ft;                 /* the kernel adds it: force += Force_PairA.trans (:709) */
+  Tc = cross(RA, Ft);      /* torque = cross(R_Body_COMA, Force.trans) (:485) */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -408,8 +427,9 @@ __device__ __forceinline__ f3 drum_velocity(const DevParams& P, f3 cp) {
  * usually hits). */
 __device__ __forceinline__ f3 find_spring(const HistRec* __restrict__ recs, int cnt, int id,
                                           int& cursor) {
+  const int start = (cursor >= 0 && cursor < cnt) ? cursor : 0;
   for (int k = 0; k < cnt; ++k) {
-    int s = cursor + k;
+    int s = start + k;
     if (s >= cnt) s -= cnt;
     const HistRec r = recs[s];
     if (__float_as_int(r.w) == id) {

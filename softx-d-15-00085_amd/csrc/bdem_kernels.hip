@@ -105,41 +105,155 @@ struct PairHistCtx {
   HistRec* out; int out_cnt;
 };
 
+/* Force and torque on particle A from contact partner j (all laws), and the
+ * spring record of that contact.  The caller accumulates Fc and Tc. */
+template <int LAW>
+__device__ __forceinline__ void contact_force(const DevParams& P, int j, const float4* __restrict__ pos_r,
+                                              const float4* __restrict__ vel_m,
+                                              const float4* __restrict__ omg_t, f3 pa, float ra, f3 va,
+                                              float ma, f3 wa, int ta, const HistRec* hin, int hin_cnt,
+                                              int cursor, HistRec* hrec, f3& Fc, f3& Tc) {
+  const float4 pb = pos_r[j];
+  const float4 vb = vel_m[j];
+  const float4 ob = omg_t[j];
+  if (LAW == BDEM_LAW_REFERENCE) {
+    pair_ref(P, pa, ra, va, ma, wa, ta, xyz(pb), pb.w, xyz(vb), vb.w, xyz(ob), packed_type(ob.w), Fc, Tc);
+  } else {
+    const f3 rel = pa - xyz(pb);
+    const float d = length(rel);
+    const f3 n = rel / d;
+    const float delta = (ra + pb.w) - d;
+    const f3 la = (-1.0f * n) * ra;
+    const f3 lb = n * pb.w;
+    const f3 vc = (va + cross(wa, la)) - (xyz(vb) + cross(xyz(ob), lb));
+    const float rstar = (ra * pb.w) / (ra + pb.w);
+    const float mstar = (ma * vb.w) / (ma + vb.w);
+    const int idb = packed_pid(ob.w);
+    f3 xi = find_spring(hin, hin_cnt, idb, cursor);
+    Fc = history_law(P, n, delta, vc, rstar, mstar, P.law_pp[ta][packed_type(ob.w)], xi);
+    Tc = cross(la, Fc);
+    if (hrec) *hrec = make_float4(xi.x, xi.y, xi.z, __int_as_float(idb));
+  }
+}
+
+/* Per-particle (uncompacted) loop over the contacts found by the scan. */
+template <int LAW>
+__device__ __forceinline__ void particle_contact(const DevParams& P, int j, const float4* __restrict__ pos_r,
+                                                 const float4* __restrict__ vel_m,
+                                                 const float4* __restrict__ omg_t, f3 pa, float ra, f3 va,
+                                                 float ma, f3 wa, int ta, f3& F, f3& T, PairHistCtx* hc) {
+  f3 Fc, Tc;
+  HistRec* rec = (LAW != BDEM_LAW_REFERENCE && hc->out_cnt < HMAX) ? hc->out + hc->out_cnt : nullptr;
+  contact_force<LAW>(P, j, pos_r, vel_m, omg_t, pa, ra, va, ma, wa, ta, hc->in, hc->in_cnt,
+                     hc->out_cnt, rec, Fc, Tc);
+  if (LAW != BDEM_LAW_REFERENCE && hc->out_cnt < HMAX) ++hc->out_cnt;
+  F = F + Fc;
+  if (P.rotation) T = T + Tc;
+}
+
+/* Pair forces of one particle in two phases: the candidate scan only
+ * records contact partners (in the reference order) into this thread's
+ * column of an LDS list, then the force loop walks the list.  A wave's force
+ * body then runs max(contacts) times instead of max(candidates) times.
+ * Particles with more than CLIST contacts (beyond the reference's 32-entry
+ * list) finish the tail with a second scan, in the same order. */
+constexpr int CLIST = 24;   /* LDS list entries per thread (VGPR and LDS occupancy both allow 4 blocks per CU) */
+constexpr int STEP_THREADS = 256;
+
 template <int LAW>
 __device__ __forceinline__ int particle_pair_forces(const DevParams& P, const uint2* __restrict__ cells,
                                                     const float4* __restrict__ pos_r,
                                                     const float4* __restrict__ vel_m,
                                                     const float4* __restrict__ omg_t, int i, f3 pa,
                                                     float ra, f3 va, float ma, f3 wa, int ta, f3& F,
-                                                    f3& T, PairHistCtx* hc) {
-  return for_each_contact(P, cells, pos_r, i, pa, ra, [&](int j, float4 pb, int k) {
-    const float4 vb = vel_m[j];
-    const float4 ob = omg_t[j];
-    if (LAW == BDEM_LAW_REFERENCE) {
-      pair_ref(P, pa, ra, va, ma, wa, ta, xyz(pb), pb.w, xyz(vb), vb.w, xyz(ob),
-               packed_type(ob.w), F, T);
-    } else {
-      const f3 rel = pa - xyz(pb);
-      const float d = length(rel);
-      const f3 n = rel / d;
-      const float delta = (ra + pb.w) - d;
-      const f3 la = (-1.0f * n) * ra;
-      const f3 lb = n * pb.w;
-      const f3 vc = (va + cross(wa, la)) - (xyz(vb) + cross(xyz(ob), lb));
-      const float rstar = (ra * pb.w) / (ra + pb.w);
-      const float mstar = (ma * vb.w) / (ma + vb.w);
-      const int idb = packed_pid(ob.w);
-      f3 xi = find_spring(hc->in, hc->in_cnt, idb, hc->cursor);
-      const f3 f = history_law(P, n, delta, vc, rstar, mstar, P.law_pp[ta][packed_type(ob.w)], xi);
-      F = F + f;
-      if (P.rotation) T = T + cross(la, f);
-      if (hc->out_cnt < HMAX) {
-        hc->out[hc->out_cnt] = make_float4(xi.x, xi.y, xi.z, __int_as_float(idb));
-        ++hc->out_cnt;
-      }
-    }
-    (void)k;
+                                                    f3& T, PairHistCtx* hc, uint32_t* clist) {
+  const int tid = threadIdx.x;
+  const int count = for_each_contact(P, cells, pos_r, i, pa, ra, [&](int j, float4, int k) {
+    if (k < CLIST) clist[k * STEP_THREADS + tid] = (uint32_t)j;
   });
+  const int nl = count < CLIST ? count : CLIST;
+  for (int k = 0; k < nl; ++k)
+    particle_contact<LAW>(P, (int)clist[k * STEP_THREADS + tid], pos_r, vel_m, omg_t, pa, ra, va, ma,
+                          wa, ta, F, T, hc);
+  if (count > CLIST) {
+    for_each_contact(P, cells, pos_r, i, pa, ra, [&](int j, float4, int k) {
+      if (k >= CLIST)
+        particle_contact<LAW>(P, j, pos_r, vel_m, omg_t, pa, ra, va, ma, wa, ta, F, T, hc);
+    });
+  }
+  return count;
+}
+
+/* Wave-compacted force loop.  The 64 lanes' listed contacts are numbered
+ * 0..T-1 in (lane, contact) order; each pass computes 64 of them, one per
+ * lane, into LDS, and every owner then adds its own results in contact
+ * order -- the same additions in the same order as the per-particle loop,
+ * with the wave's force body running ceil(T/64) times instead of
+ * max(contacts) times.  All lanes of the wave must call it (count 0 for
+ * idle lanes). */
+struct WaveScratch {
+  float f[6][64];
+  int off[65];
+};
+
+template <int LAW>
+__device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float4* __restrict__ pos_r,
+                                                 const float4* __restrict__ vel_m,
+                                                 const float4* __restrict__ omg_t,
+                                                 const uint32_t* clist, int nl,
+                                                 const int* __restrict__ hc_in,
+                                                 const HistRec* __restrict__ hr_in, HistRec* hr_out,
+                                                 int hist_cap, WaveScratch& ws, f3& F, f3& T) {
+  const int lane = threadIdx.x & 63;
+  const int wbase_tid = threadIdx.x & ~63;
+  int incl = nl;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const int excl = incl - nl;
+  const int total = __shfl(incl, 63, 64);
+  ws.off[lane] = excl;
+  if (lane == 63) ws.off[64] = total;
+  __builtin_amdgcn_wave_barrier();
+  for (int base = 0; base < total; base += 64) {
+    const int p = base + lane;
+    f3 Fc = mk3(0.f, 0.f, 0.f), Tc = Fc;
+    if (p < total) {
+      int lo = 0, hi = 63;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ws.off[mid] <= p) lo = mid; else hi = mid - 1;
+      }
+      const int k = p - ws.off[lo];
+      const int otid = wbase_tid + lo;
+      const int oi = blockIdx.x * STEP_THREADS + otid;
+      const int j = (int)clist[k * STEP_THREADS + otid];
+      const float4 po = pos_r[oi], vo = vel_m[oi], oo = omg_t[oi];
+      const int opid = packed_pid(oo.w);
+      const HistRec* hin = nullptr;
+      int hin_cnt = 0;
+      HistRec* rec = nullptr;
+      if (LAW != BDEM_LAW_REFERENCE && opid < hist_cap) {
+        hin = hr_in + (size_t)opid * HMAX;
+        hin_cnt = hc_in[opid];
+        if (k < HMAX) rec = hr_out + (size_t)opid * HMAX + k;
+      }
+      contact_force<LAW>(P, j, pos_r, vel_m, omg_t, xyz(po), po.w, xyz(vo), vo.w, xyz(oo),
+                         packed_type(oo.w), hin, hin_cnt, k, rec, Fc, Tc);
+    }
+    ws.f[0][lane] = Fc.x; ws.f[1][lane] = Fc.y; ws.f[2][lane] = Fc.z;
+    ws.f[3][lane] = Tc.x; ws.f[4][lane] = Tc.y; ws.f[5][lane] = Tc.z;
+    __builtin_amdgcn_wave_barrier();
+    const int a = max(excl, base), b = min(excl + nl, base + 64);
+    for (int q = a; q < b; ++q) {
+      const int s = q - base;
+      F = F + mk3(ws.f[0][s], ws.f[1][s], ws.f[2][s]);
+      if (P.rotation) T = T + mk3(ws.f[3][s], ws.f[4][s], ws.f[5][s]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -160,81 +274,96 @@ __global__ __launch_bounds__(256) void k_step(
     int* __restrict__ wc_out, HistRec* __restrict__ wr_out, int hist_cap,
     float* __restrict__ wall_force, int do_kill, uint32_t* status, int* kill_count) {
   const DevParams& P = *Pp;
+  __shared__ uint32_t clist[CLIST * STEP_THREADS];
+  __shared__ WaveScratch wscr[STEP_THREADS / 64];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  float4 pr = make_float4(0.f, 0.f, 0.f, 0.f), vm = pr, ot = pr;
+  if (in) { pr = pos_in[i]; vm = vel_in[i]; ot = omg_in[i]; }
+  int pid = in ? packed_pid(ot.w) : -1;
+  const int ta = packed_type(ot.w);
+  const bool alive = in && pid >= 0;
+  f3 p = xyz(pr), v = xyz(vm), w = xyz(ot);
+  const float r = pr.w, m = vm.w;
+
+  /* phase 1: contact partners in the reference order, into this thread's list */
+  int nc = 0;
+  if (alive) {
+    nc = for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
+      if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
+    });
+  }
+  /* phase 2: gravity first (DEM_Kernels_Compute.cuh:654), then the contacts */
+  f3 Fpp = alive ? cvt(P.gravity) * m : mk3(0.f, 0.f, 0.f);
+  f3 Tpp = mk3(0.f, 0.f, 0.f);
+  const int nl = nc < CLIST ? nc : CLIST;
+  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, hc_in, hr_in, hr_out, hist_cap,
+                        wscr[threadIdx.x >> 6], Fpp, Tpp);
+
   bool chg = false;
-  if (i < n) {
-    const float4 pr = pos_in[i];
-    const float4 vm = vel_in[i];
-    const float4 ot = omg_in[i];
-    int pid = packed_pid(ot.w);
-    const int ta = packed_type(ot.w);
-    uint32_t new_key;
-    if (pid < 0) {
-      /* dead: inert, copied through */
-      pos_out[i] = pr;
-      vel_out[i] = vm;
-      omg_out[i] = ot;
-      if (ORIENT) quat_out[i] = quat_in[i];
-      if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
-      new_key = P.C + 1u;
-    } else {
-      f3 p = xyz(pr), v = xyz(vm), w = xyz(ot);
-      const float r = pr.w, m = vm.w;
-
-      /* gravity is part of the pair force, DEM_Kernels_Compute.cuh:654 */
-      f3 Fpp = cvt(P.gravity) * m;
-      f3 Tpp = mk3(0.f, 0.f, 0.f);
-      PairHistCtx hc{nullptr, 0, 0, nullptr, 0};
-      WallHist wh{nullptr, 0, nullptr, 0};
-      const bool hist_slot = LAW != BDEM_LAW_REFERENCE && pid < hist_cap;
-      HistRec scratch_pair[1];
-      HistRec scratch_wall[1];
-      if (LAW != BDEM_LAW_REFERENCE) {
-        if (hist_slot) {
-          hc.in = hr_in + (size_t)pid * HMAX;
-          hc.in_cnt = hc_in[pid];
-          hc.out = hr_out + (size_t)pid * HMAX;
-          wh.in = wr_in + (size_t)pid * WMAX;
-          wh.in_cnt = wc_in[pid];
-          wh.out = wr_out + (size_t)pid * WMAX;
-        } else {
-          hc.out = scratch_pair; hc.out_cnt = HMAX;   /* no slot: springs are not kept */
-          wh.out = scratch_wall; wh.out_cnt = WMAX;
-        }
-      }
-      const int nc = particle_pair_forces<LAW>(P, cells, pos_in, vel_in, omg_in, i, p, r, v, m, w,
-                                               ta, Fpp, Tpp, &hc);
-      if (nc > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
-      if (LAW != BDEM_LAW_REFERENCE && nc > HMAX) atomicOr(status, ST_HIST_OVERFLOW);
-      if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
-
-      f3 Fw, Tw;
-      wall_forces(P, ta, p, r, v, w, m, Fw, Tw, &wh);
-      if (wall_force) {
-        wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
-      }
+  if (alive) {
+    const bool hist_slot = LAW != BDEM_LAW_REFERENCE && pid < hist_cap;
+    if (nc > CLIST) {
+      /* beyond the list: rescan, same order (rare) */
+      PairHistCtx hc{nullptr, 0, 0, nullptr, HMAX};   /* springs looked up, not stored */
       if (hist_slot) {
-        hc_out[pid] = hc.out_cnt;
-        wc_out[pid] = wh.out_cnt;
+        hc.in = hr_in + (size_t)pid * HMAX;
+        hc.in_cnt = hc_in[pid];
       }
-
-      const float inv_inertia = P.per_particle ? 1.0f / (0.4f * r * r * m) : P.t_inertia[ta];
-      QuatF q{1.f, 0.f, 0.f, 0.f};
-      if (ORIENT) { const float4 q4 = quat_in[i]; q = QuatF{q4.x, q4.y, q4.z, q4.w}; }
-      integrate(P, inv_inertia, m, Fpp, Tpp, Fw, Tw, p, v, w, ORIENT ? &q : nullptr);
-
-      /* a11: Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
-      if (do_kill && p.y < P.kill_height) {
-        p = cvt(P.max_size);
-        pid = -1;
-        atomicAdd(kill_count, 1);
-      }
-      pos_out[i] = make_float4(p.x, p.y, p.z, r);
-      vel_out[i] = make_float4(v.x, v.y, v.z, m);
-      omg_out[i] = make_float4(w.x, w.y, w.z, __int_as_float(pack_id_type(pid, ta)));
-      if (ORIENT) quat_out[i] = make_float4(q.w, q.x, q.y, q.z);
-      new_key = key_of(P, p, pid, status);
+      for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
+        if (k >= CLIST) particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc);
+      });
+      atomicOr(status, ST_LIST_OVERFLOW);
     }
+    if (LAW != BDEM_LAW_REFERENCE && nc > HMAX) atomicOr(status, ST_HIST_OVERFLOW);
+    if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
+
+    WallHist wh{nullptr, 0, nullptr, 0};
+    HistRec scratch_wall[1];
+    if (LAW != BDEM_LAW_REFERENCE) {
+      if (hist_slot) {
+        wh.in = wr_in + (size_t)pid * WMAX;
+        wh.in_cnt = wc_in[pid];
+        wh.out = wr_out + (size_t)pid * WMAX;
+      } else {
+        wh.out = scratch_wall; wh.out_cnt = WMAX;   /* no slot: springs are not kept */
+      }
+    }
+    f3 Fw, Tw;
+    wall_forces(P, ta, p, r, v, w, m, Fw, Tw, &wh);
+    if (wall_force) {
+      wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
+    }
+    if (hist_slot) {
+      hc_out[pid] = nc < HMAX ? nc : HMAX;
+      wc_out[pid] = wh.out_cnt;
+    }
+
+    const float inv_inertia = P.per_particle ? 1.0f / (0.4f * r * r * m) : P.t_inertia[ta];
+    QuatF q{1.f, 0.f, 0.f, 0.f};
+    if (ORIENT) { const float4 q4 = quat_in[i]; q = QuatF{q4.x, q4.y, q4.z, q4.w}; }
+    integrate(P, inv_inertia, m, Fpp, Tpp, Fw, Tw, p, v, w, ORIENT ? &q : nullptr);
+
+    /* a11: Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
+    if (do_kill && p.y < P.kill_height) {
+      p = cvt(P.max_size);
+      pid = -1;
+      atomicAdd(kill_count, 1);
+    }
+    pos_out[i] = make_float4(p.x, p.y, p.z, r);
+    vel_out[i] = make_float4(v.x, v.y, v.z, m);
+    omg_out[i] = make_float4(w.x, w.y, w.z, __int_as_float(pack_id_type(pid, ta)));
+    if (ORIENT) quat_out[i] = make_float4(q.w, q.x, q.y, q.z);
+  } else if (in) {
+    /* dead: inert, copied through */
+    pos_out[i] = pr;
+    vel_out[i] = vm;
+    omg_out[i] = ot;
+    if (ORIENT) quat_out[i] = quat_in[i];
+    if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
+  }
+  if (in) {
+    const uint32_t new_key = key_of(P, p, pid, status);
     keys_out[i] = new_key;
     chg = new_key != sorted_keys[i];
     changed[i] = chg ? 1 : 0;
@@ -308,6 +437,7 @@ __global__ __launch_bounds__(256) void k_forces(
     int hist_cap, float* __restrict__ fpp, float* __restrict__ tpp, float* __restrict__ fw,
     float* __restrict__ tw) {
   const DevParams& P = *Pp;
+  __shared__ uint32_t clist[CLIST * STEP_THREADS];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 pr = pos_in[i];
@@ -333,7 +463,7 @@ __global__ __launch_bounds__(256) void k_forces(
         wh.out = scratch_wall; wh.out_cnt = WMAX;
       }
     }
-    particle_pair_forces<LAW>(P, cells, pos_in, vel_in, omg_in, i, p, r, v, m, w, ta, F, T, &hc);
+    particle_pair_forces<LAW>(P, cells, pos_in, vel_in, omg_in, i, p, r, v, m, w, ta, F, T, &hc, clist);
     if (!P.rotation) T = mk3(0.f, 0.f, 0.f);
     wall_forces(P, ta, p, r, v, w, m, Fw, Tw, &wh);
   }
