@@ -40,6 +40,7 @@ extern "C" {
 #define BDEM_WARN_OUT_OF_GRID    1u  /* a live particle left the hash grid */
 #define BDEM_WARN_LIST_OVERFLOW  2u  /* > 32 contacts on a particle (reference list size) */
 #define BDEM_WARN_HIST_OVERFLOW  4u  /* more contacts than tangential-history slots */
+#define BDEM_WARN_SORT           8u  /* internal: incremental sort consistency check failed */
 
 typedef struct bdem_ctx bdem_ctx;
 

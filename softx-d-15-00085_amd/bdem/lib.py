@@ -20,6 +20,7 @@ BDEM_OK = 0
 WARN_OUT_OF_GRID = 1
 WARN_LIST_OVERFLOW = 2
 WARN_HIST_OVERFLOW = 4
+WARN_SORT = 8
 
 _lib = None
 
@@ -217,7 +218,7 @@ class Device:
         self._check(self.lib.bdem_drum_rotation(self.h, 1 if on else 0))
 
     # -- measurement ------------------------------------------------------
-    STAGES = ("sort", "gather", "cells", "step")
+    STAGES = ("sort", "place", "cells", "step")
 
     def timing_begin(self, max_steps):
         self._check(self.lib.bdem_timing_begin(self.h, int(max_steps)))

@@ -40,9 +40,10 @@ struct bdem_ctx {
   StepBuffers b{};
   std::vector<void*> allocs;
   bool orient = false;
-  bool keys_valid = false;   /* keys[0] holds the keys of the current state */
+  bool keys_valid = false;   /* keys_new holds the keys of the current state */
   bool flags_valid = false;  /* ... and the step kernel's changed flags for them */
   int hist_par = 0;
+  int sorted_par = 0;        /* keys_sorted[sorted_par] = latest sorted keys */
   int stepc = 0;
   /* stage timing: one event per stage boundary per recorded step */
   std::vector<hipEvent_t> tev;
@@ -50,7 +51,8 @@ struct bdem_ctx {
 };
 
 /* Stage boundaries recorded per step when timing is on:
- * 0 start | hash+sort | gather | cells | step (fused kernel) */
+ * 0 start | hash + changed-set sort | place + gather | cell table (+ fallback)
+ * | step (fused kernel) */
 static const int BDEM_NSTAGE = 4;
 
 namespace {
@@ -262,22 +264,45 @@ hipEvent_t* timing_slot(bdem_ctx* c) {
   return &c->tev[(size_t)c->t_used * (BDEM_NSTAGE + 1)];
 }
 
-/* hash (if needed) + sort + gather + cells: the sorted scratch state. */
-int prepare_sorted(bdem_ctx* c, int* sorted_buf, hipEvent_t* ev = nullptr) {
+/* hash (if needed) + sort + gather + cells: the sorted scratch state.
+ * On return b.keys_sorted[c->sorted_par] and b.idx_sorted hold the order. */
+int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   if (ev) (void)hipEventRecord(ev[0], c->stream);
   const bool incremental = c->keys_valid && c->flags_valid;
   if (!c->keys_valid) launch_hash(c->dp, c->b, c->stream);
-  const int cur = launch_sort(c->b, c->hp.C + 1u, incremental, c->stream);
-  /* the sort may reuse keys[0]; the next step must produce new keys */
+  const int prev = c->sorted_par;
+  const int next = incremental ? prev ^ 1 : prev;
+  SortArgs a{};
+  const StepBuffers& b = c->b;
+  a.n = b.n;
+  a.C = c->hp.C;
+  a.passes = sort_passes(c->hp.C + 1u);
+  a.keys_new = b.keys_new;
+  a.changed = b.changed;
+  a.sbuf = b.sbuf;
+  a.skey = b.skey;
+  a.sslot = b.sslot;
+  a.okeys = b.keys_sorted[prev];
+  a.keys_sorted = b.keys_sorted[next];
+  a.idx_sorted = b.idx_sorted;
+  a.rk[0] = b.rk[0]; a.rk[1] = b.rk[1];
+  a.rv[0] = b.rv[0]; a.rv[1] = b.rv[1];
+  a.hist = b.hist;
+  a.part = b.part;
+  a.meta = b.meta;
+  a.cells = b.cells;
+  a.status = b.status;
+  a.pos_cur = b.pos_cur; a.vel_cur = b.vel_cur; a.omg_cur = b.omg_cur;
+  a.quat_cur = c->orient ? b.quat_cur : nullptr;
+  a.pos_srt = b.pos_srt; a.vel_srt = b.vel_srt; a.omg_srt = b.omg_srt;
+  a.quat_srt = c->orient ? b.quat_srt : nullptr;
+  hipError_t e = launch_sort(a, incremental, c->stream, ev ? ev + 1 : nullptr);
+  if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("sort launch: ") + hipGetErrorString(e));
+  c->sorted_par = next;
+  /* the step kernel must produce new keys and flags before the next sort */
   c->keys_valid = false;
   c->flags_valid = false;
-  if (ev) (void)hipEventRecord(ev[1], c->stream);
-  launch_gather(c->b, c->b.vals[cur], c->orient, c->stream);
-  if (ev) (void)hipEventRecord(ev[2], c->stream);
-  launch_cells(c->dp, c->b, c->b.keys[cur], c->hp.C, c->stream);
-  if (ev) (void)hipEventRecord(ev[3], c->stream);
   HIPCHK(c, hipGetLastError());
-  if (sorted_buf) *sorted_buf = cur;
   return BDEM_OK;
 }
 
@@ -370,20 +395,23 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.pos_cur, n); ALLOC(b.vel_cur, n); ALLOC(b.omg_cur, n);
   ALLOC(b.pos_srt, n); ALLOC(b.vel_srt, n); ALLOC(b.omg_srt, n);
   if (c->orient) { ALLOC(b.quat_cur, n); ALLOC(b.quat_srt, n); }
-  ALLOC(b.keys[0], n); ALLOC(b.keys[1], n); ALLOC(b.vals[0], n); ALLOC(b.vals[1], n);
-  ALLOC(b.radix_hist, (size_t)RADIX * sort_blocks(n));
+  ALLOC(b.keys_new, n); ALLOC(b.keys_sorted[0], n); ALLOC(b.keys_sorted[1], n);
+  ALLOC(b.idx_sorted, n);
   ALLOC(b.changed, n);
-  ALLOC(b.blk_changed, (size_t)(n + 255) / 256);
-  ALLOC(b.rkeys, n);
-  ALLOC(b.rvals, n);
   ALLOC(b.sbuf, n);
-  ALLOC(b.sort_meta, 2);
+  ALLOC(b.skey, SMALL_MAX);
+  ALLOC(b.sslot, SMALL_MAX);
+  ALLOC(b.meta, META_WORDS);
+  ALLOC(b.rk[0], n); ALLOC(b.rk[1], n); ALLOC(b.rv[0], n); ALLOC(b.rv[1], n);
+  ALLOC(b.hist, (size_t)RADIX * sort_blocks(n));
+  ALLOC(b.part, 1024);
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(b.wall_force, (size_t)n * 3);
 #undef ALLOC
   HIPCHK(c, hipMemset(b.status, 0, sizeof(uint32_t)));
   HIPCHK(c, hipMemset(b.kill_count, 0, sizeof(int)));
+  HIPCHK(c, hipMemset(b.meta, 0, META_WORDS * sizeof(uint32_t)));
   HIPCHK(c, hipMemset(b.wall_force, 0, (size_t)n * 3 * sizeof(float)));
   HIPCHK(c, hipMemset(b.pos_cur, 0, (size_t)n * sizeof(float4)));
   c->configured = true;
@@ -663,16 +691,15 @@ int bdem_update_sim(bdem_ctx* c) {
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   hipEvent_t* ev = timing_slot(c);
-  int cur = 0;
-  if ((rc = prepare_sorted(c, &cur, ev)) != BDEM_OK) return rc;
+  if ((rc = prepare_sorted(c, ev)) != BDEM_OK) return rc;
   bool do_kill = false;
   if (c->sim.Kill_Particles) {
     const int interval = (c->has_ext && c->ext.kill_check_interval > 0) ? c->ext.kill_check_interval : 1000;
     if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
     else c->stepc += 1;
   }
-  launch_step(c->dp, c->b, c->b.keys[cur], c->hp.law, c->orient, c->hist_par, do_kill, c->record_wf,
-              c->stream);
+  launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par, do_kill,
+              c->record_wf, c->stream);
   if (ev) {
     (void)hipEventRecord(ev[4], c->stream);
     c->t_used += 1;
@@ -743,7 +770,7 @@ int bdem_debug_hash(bdem_ctx* c, unsigned int* keys, int* idx) {
   launch_hash(c->dp, c->b, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int n = c->b.n;
-  HIPCHK(c, hipMemcpy(keys, c->b.keys[0], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(keys, c->b.keys_new, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   for (int i = 0; i < n; ++i) idx[i] = i;
   c->keys_valid = true;
   c->flags_valid = false;
@@ -754,12 +781,11 @@ int bdem_debug_sorted(bdem_ctx* c, unsigned int* keys, int* idx) {
   int rc = require(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  int cur = 0;
-  if ((rc = prepare_sorted(c, &cur)) != BDEM_OK) return rc;
+  if ((rc = prepare_sorted(c)) != BDEM_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int n = c->b.n;
-  HIPCHK(c, hipMemcpy(keys, c->b.keys[cur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(idx, c->b.vals[cur], n * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(keys, c->b.keys_sorted[c->sorted_par], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(idx, c->b.idx_sorted, n * sizeof(int), hipMemcpyDeviceToHost));
   c->keys_valid = false;
   return BDEM_OK;
 }
@@ -768,7 +794,7 @@ int bdem_debug_cells(bdem_ctx* c, unsigned int* cs, unsigned int* ce) {
   int rc = require(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  if ((rc = prepare_sorted(c)) != BDEM_OK) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   std::vector<uint2> cells(c->hp.C);
   HIPCHK(c, hipMemcpy(cells.data(), c->b.cells, cells.size() * sizeof(uint2), hipMemcpyDeviceToHost));
@@ -781,7 +807,7 @@ int bdem_debug_contacts(bdem_ctx* c, unsigned int* list32, unsigned int* nn) {
   int rc = require(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  if ((rc = prepare_sorted(c)) != BDEM_OK) return rc;
   const int n = c->b.n;
   uint32_t *dl = nullptr, *dn = nullptr;
   HIPCHK(c, hipMalloc((void**)&dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t)));
@@ -802,7 +828,7 @@ int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw,
   int rc = require(c);
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = prepare_sorted(c, nullptr)) != BDEM_OK) return rc;
+  if ((rc = prepare_sorted(c)) != BDEM_OK) return rc;
   const int n = c->b.n;
   float* d = nullptr;
   HIPCHK(c, hipMalloc((void**)&d, (size_t)n * 12 * sizeof(float)));

@@ -31,6 +31,7 @@ enum : uint32_t {
   ST_OUT_OF_GRID   = 1u,   /* a live particle left the hash grid */
   ST_LIST_OVERFLOW = 2u,   /* more than 32 contacts (reference list capacity) */
   ST_HIST_OVERFLOW = 4u,   /* more contacts than history slots */
+  ST_SORT_ERROR    = 8u,   /* incremental sort produced an out-of-range position */
 };
 
 struct f3 { float x, y, z; };
@@ -150,6 +151,95 @@ __device__ __forceinline__ bool overlaps(f3 pa, float ra, float4 pb) {
   if (d2 < s2 * 0.99999904632568359375f) return true;   /* 1 - 2^-20 */
   if (d2 > s2 * 1.00000095367431640625f) return false;  /* 1 + 2^-20 */
   return __builtin_sqrtf(d2) - s < 0.0f;
+}
+
+/* Wave-cooperative form of the same traversal.  The 64 lanes are 64
+ * consecutive sorted particles, so for one stencil row offset (dy, dz) their
+ * cell runs and candidate runs overlap almost entirely: the wave loads the
+ * union of its cell entries (<= 64) and of its candidate positions
+ * (<= STAGE_POS) into LDS with coalesced loads, then each lane scans its own
+ * range from LDS in the reference order.  Rows whose union is wider fall back
+ * to global loads.  Every lane of the wave must call it; `active` false for
+ * lanes without a live particle.  Same visits, same order as
+ * for_each_contact. */
+constexpr int STAGE_POS = 128;
+struct WaveStage {
+  float4 pos[STAGE_POS];
+  uint2 cell[64];
+};
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+template <class Visit>
+__device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const uint2* __restrict__ cells,
+                                                     const float4* __restrict__ pos_r, int i, f3 pa,
+                                                     float ra, bool active, WaveStage& st,
+                                                     Visit&& visit) {
+  const int lane = threadIdx.x & 63;
+  int g[3];
+  grid_pos(P, pa, g);
+  const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
+  int count = 0;
+  for (int dy = -1; dy <= 1; ++dy) {
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int y = g[1] + dy, z = g[2] + dz;
+      bool rv = active && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
+      const uint32_t h0 = rv ? hash_of(P, xs, y, z) : 0u;
+      const uint32_t h1 = rv ? hash_of(P, xe, y, z) : 0u;
+      if (rv && h1 >= P.C) rv = false;
+      /* stage the wave's cell entries */
+      const uint32_t cmin = wave_min_u32(rv ? h0 : 0xFFFFFFFFu);
+      const uint32_t cmax = wave_max_u32(rv ? h1 : 0u);
+      if (cmin > cmax) continue;                    /* no lane has this row */
+      const bool cst = cmax - cmin < 64u;
+      if (cst) {
+        if (cmin + (uint32_t)lane <= cmax) st.cell[lane] = cells[cmin + lane];
+        __builtin_amdgcn_wave_barrier();
+      }
+      uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+      if (rv) {
+        for (uint32_t h = h0; h <= h1; ++h) {
+          const uint2 c = cst ? st.cell[h - cmin] : cells[h];
+          if (c.x != 0xFFFFFFFFu) {
+            lo = min(lo, c.x);
+            hi = c.y;
+          }
+        }
+      }
+      const bool ne = lo != 0xFFFFFFFFu;
+      /* stage the wave's candidate positions */
+      const uint32_t pmin = wave_min_u32(ne ? lo : 0xFFFFFFFFu);
+      const uint32_t pmax = wave_max_u32(ne ? hi : 0u);
+      __builtin_amdgcn_wave_barrier();
+      if (pmin >= pmax) continue;
+      const bool pst = pmax - pmin <= (uint32_t)STAGE_POS;
+      if (pst) {
+        for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (ne) {
+        for (uint32_t j = lo; j < hi; ++j) {
+          if ((int)j == i) continue;
+          const float4 pb = pst ? st.pos[j - pmin] : pos_r[j];
+          if (overlaps(pa, ra, pb)) {
+            visit((int)j, pb, count);
+            ++count;
+          }
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  return count;
 }
 
 template <class Visit>

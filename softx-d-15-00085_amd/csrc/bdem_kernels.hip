@@ -47,54 +47,7 @@ __global__ __launch_bounds__(256) void k_hash(const DevParams* __restrict__ Pp, 
 
 void launch_hash(const DevParams* dP, const StepBuffers& b, hipStream_t s) {
   hipLaunchKernelGGL(k_hash, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.pos_cur,
-                     b.omg_cur, b.keys[0], b.status);
-}
-
-/* ------------------------------------------------------------------------ */
-/* a3: one gather into sorted order.                                          */
-/* ------------------------------------------------------------------------ */
-__global__ __launch_bounds__(256) void k_gather(int n, const int* __restrict__ idx,
-                                                const float4* __restrict__ p_in,
-                                                const float4* __restrict__ v_in,
-                                                const float4* __restrict__ o_in,
-                                                const float4* __restrict__ q_in,
-                                                float4* __restrict__ p_out, float4* __restrict__ v_out,
-                                                float4* __restrict__ o_out, float4* __restrict__ q_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int j = idx[i];
-  p_out[i] = p_in[j];
-  v_out[i] = v_in[j];
-  o_out[i] = o_in[j];
-  if (q_in) q_out[i] = q_in[j];
-}
-
-void launch_gather(const StepBuffers& b, const int* idx, bool orient, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather, dim3((b.n + 255) / 256), dim3(256), 0, s, b.n, idx, b.pos_cur,
-                     b.vel_cur, b.omg_cur, orient ? b.quat_cur : nullptr, b.pos_srt, b.vel_srt,
-                     b.omg_srt, orient ? b.quat_srt : nullptr);
-}
-
-/* ------------------------------------------------------------------------ */
-/* a4: Kernel_SpatialDecomp_BinData (Collision_Detection_BroadPhase.cuh:279-325)
- * after the 0xFFFFFFFF fill (Device/DeviceInterface.cu:2097-2098).  Keys at or
- * beyond C (outside the grid, dead) own no cell. */
-__global__ __launch_bounds__(256) void k_bin(int n, const uint32_t* __restrict__ keys,
-                                             uint2* __restrict__ cells, uint32_t C) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t k = keys[i];
-  if (k >= C) return;
-  if (i == 0 || keys[i - 1] != k) cells[k].x = (uint32_t)i;
-  if (i == n - 1 || keys[i + 1] != k) cells[k].y = (uint32_t)(i + 1);
-}
-
-void launch_cells(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys,
-                  uint32_t ncells, hipStream_t s) {
-  (void)dP;
-  launch_fill_u32(reinterpret_cast<uint32_t*>(b.cells), (size_t)ncells * 2, 0xFFFFFFFFu, s);
-  hipLaunchKernelGGL(k_bin, dim3((b.n + 255) / 256), dim3(256), 0, s, b.n, sorted_keys, b.cells,
-                     ncells);
+                     b.omg_cur, b.keys_new, b.status);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -268,7 +221,7 @@ __global__ __launch_bounds__(256) void k_step(
     float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ omg_out,
     float4* __restrict__ quat_out, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ sorted_keys, uint8_t* __restrict__ changed,
-    uint32_t* __restrict__ blk_changed,
+    unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ meta,
     const int* __restrict__ hc_in, const HistRec* __restrict__ hr_in, int* __restrict__ hc_out,
     HistRec* __restrict__ hr_out, const int* __restrict__ wc_in, const HistRec* __restrict__ wr_in,
     int* __restrict__ wc_out, HistRec* __restrict__ wr_out, int hist_cap,
@@ -362,15 +315,24 @@ __global__ __launch_bounds__(256) void k_step(
     if (ORIENT) quat_out[i] = quat_in[i];
     if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
   }
+  uint32_t new_key = 0u;
   if (in) {
-    const uint32_t new_key = key_of(P, p, pid, status);
+    new_key = key_of(P, p, pid, status);
     keys_out[i] = new_key;
     chg = new_key != sorted_keys[i];
     changed[i] = chg ? 1 : 0;
   }
-  /* per-block changed count for the incremental sort's stable partition */
-  const int cnt = __syncthreads_count(chg);
-  if (threadIdx.x == 0) blk_changed[blockIdx.x] = (uint32_t)cnt;
+  /* append changed slots for the next incremental sort (one atomic per wave) */
+  const uint64_t mask = __ballot(chg);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(&meta[META_APPEND], (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    if (chg) sbuf[base + (uint32_t)__popcll(mask & lt)] = ((unsigned long long)new_key << 32) | (uint32_t)i;
+  }
 }
 
 template <int LAW, bool ORIENT>
@@ -378,7 +340,7 @@ static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint3
                           bool do_kill, bool want_wf, hipStream_t s) {
   hipLaunchKernelGGL((k_step<LAW, ORIENT>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n,
                      b.cells, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
-                     b.omg_cur, b.quat_cur, b.keys[0], sk, b.changed, b.blk_changed, b.hcount[hp],
+                     b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta, b.hcount[hp],
                      b.hrec[hp], b.hcount[hp ^ 1], b.hrec[hp ^ 1], b.wcount[hp], b.wrec[hp],
                      b.wcount[hp ^ 1], b.wrec[hp ^ 1], b.hist_cap, want_wf ? b.wall_force : nullptr,
                      do_kill ? 1 : 0, b.status, b.kill_count);

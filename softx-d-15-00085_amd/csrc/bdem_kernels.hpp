@@ -15,18 +15,18 @@ struct StepBuffers {
   int n;                               /* particle capacity (all slots processed) */
   float4 *pos_cur, *vel_cur, *omg_cur, *quat_cur;
   float4 *pos_srt, *vel_srt, *omg_srt, *quat_srt;
-  uint32_t *keys[2];                   /* radix ping-pong keys */
-  int *vals[2];                        /* radix ping-pong payload (original slot) */
-  uint32_t *radix_hist;                /* 256 * ceil(n / tile) counters */
-  /* incremental sort: changed flags from the step kernel, per-256 counts,
-   * the unchanged run (rkeys/rvals), the changed set (sbuf) and
-   * sort_meta = {|S|, fallback flag} */
-  uint8_t *changed;
-  uint32_t *blk_changed;
-  uint32_t *rkeys;
-  int *rvals;
-  unsigned long long *sbuf;
-  uint32_t *sort_meta;
+  uint32_t *keys_new;                  /* keys the step kernel (or k_hash) wrote, slot order */
+  uint32_t *keys_sorted[2];            /* sorted keys, alternating per step (old/new) */
+  int *idx_sorted;                     /* sorted order as original slots */
+  uint8_t *changed;                    /* key changed in the last step kernel */
+  unsigned long long *sbuf;            /* appended (key<<32 | slot) of changed slots */
+  unsigned long long *skey;            /* changed set sorted by (key, slot) */
+  uint32_t *sslot;                     /* changed slots in slot order */
+  uint32_t *meta;                      /* META_* words */
+  uint32_t *rk[2];                     /* full radix sort ping-pong */
+  int *rv[2];
+  uint32_t *hist;                      /* full radix digit counters */
+  uint32_t *part;                      /* full radix scan partials (<= 1024) */
   uint2 *cells;                        /* (start, end) per cell, reference format */
   /* tangential history, indexed by P_ID, double-buffered across steps */
   int *hcount[2]; HistRec *hrec[2];
@@ -41,24 +41,46 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX = 256;
+constexpr int SMALL_MAX = 16384;       /* changed slots one workgroup sorts in LDS */
+enum { META_APPEND = 0, META_COUNT = 1, META_FALLBACK = 2, META_WORDS = 4 };
 
-inline int sort_blocks(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
+inline __host__ __device__ int sort_blocks(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 int key_bits(uint32_t max_key);
 int sort_passes(uint32_t max_key);
 
+/* Everything the sort kernels touch for one step. */
+struct SortArgs {
+  int n;
+  uint32_t C;
+  int passes;
+  const uint32_t* keys_new;
+  const uint8_t* changed;
+  const unsigned long long* sbuf;
+  unsigned long long* skey;
+  uint32_t* sslot;
+  const uint32_t* okeys;               /* previous step's sorted keys */
+  uint32_t* keys_sorted;
+  int* idx_sorted;
+  uint32_t* rk[2];
+  int* rv[2];
+  uint32_t* hist;
+  uint32_t* part;
+  uint32_t* meta;
+  uint2* cells;
+  uint32_t* status;
+  const float4 *pos_cur, *vel_cur, *omg_cur, *quat_cur;
+  float4 *pos_srt, *vel_srt, *omg_srt, *quat_srt;
+};
+
 /* Individual stages (all asynchronous on `s`). */
 void launch_hash(const DevParams* dP, const StepBuffers& b, hipStream_t s);
-/* Stable sort of keys[0] (payload: slot index) for keys <= max_key.
- * incremental: keys[0] came from the step kernel together with its changed
- * flags; otherwise a full radix sort runs.  Returns the index (0 or 1) of the
- * buffer pair holding the result. */
-int launch_sort(const StepBuffers& b, uint32_t max_key, bool incremental, hipStream_t s);
-void launch_gather(const StepBuffers& b, const int* idx, bool orient, hipStream_t s);
-void launch_cells(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys,
-                  uint32_t ncells, hipStream_t s);
+/* Sort + gather + cell table.  incremental: keys_new came from the step
+ * kernel together with its changed list; otherwise (or as the in-stream
+ * fallback) the cooperative full radix sort runs. */
+hipError_t launch_sort(const SortArgs& a, bool incremental, hipStream_t s, hipEvent_t* ev = nullptr);
 /* Fused contact search + pair force + walls + integration, srt -> cur.
- * Writes next-step keys into keys[0] and the changed flags/counts against
- * sorted_keys (this step's sorted keys). */
+ * Writes next-step keys, the changed flags (against sorted_keys, this step's
+ * sorted keys) and appends changed slots to sbuf. */
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
                  bool orient, int hist_par, bool do_kill, bool want_wall_force, hipStream_t s);
 

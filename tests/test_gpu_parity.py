@@ -101,7 +101,9 @@ def test_step_states_match(pair, steps):
         print(f"{name} {steps} steps {key}: max rel err {err:.3e}, bit-exact {exact:.4f}")
         assert err <= TOL, f"{name} {key}: {err}"
     # the dense random packs eject particles: both sides must agree on it
-    assert bool(dev.status() & bdem.lib.WARN_OUT_OF_GRID) == bool(ora.warnings() & 1)
+    st = dev.status()
+    assert bool(st & bdem.lib.WARN_OUT_OF_GRID) == bool(ora.warnings() & 1)
+    assert not st & bdem.lib.WARN_SORT
 
 
 def test_incremental_sort_matches_full_sort():
