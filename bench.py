@@ -40,10 +40,10 @@ def make_device(bdem, deck):
 
 def step_kernel_bytes(n, k, ncells):
     """Algorithmic bytes of one fused-step launch (history law), per the
-    stage I/O: reads state 48 + sorted key 4 + spring count 4 + springs 16k
-    + wall-spring count 4; writes state 48 + key 4 + changed flag 1 + spring
-    count 4 + springs 16k + wall-spring count 4; plus the cell table 8C once."""
-    per = 48 + 4 + 4 + 16 * k + 4 + 48 + 4 + 1 + 4 + 16 * k + 4
+    stage I/O: reads state 48 + sorted key 4 + packed spring counts 4 +
+    springs 16k; writes state 48 + key 4 + changed flag 1 + spring counts 4 +
+    springs 16k; plus the cell table 8C once."""
+    per = 48 + 4 + 4 + 16 * k + 48 + 4 + 1 + 4 + 16 * k
     return n * per + 8 * ncells
 
 
@@ -188,6 +188,14 @@ def main():
                      achieved=round(tot_b / (ms_per_step * 1e-3) / 1e9, 1),
                      frac=round(tot_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 
+    # size of the incremental sort's changed list over a few more (untimed) steps
+    chg = []
+    for _ in range(8):
+        dev.step(1)
+        chg.append(dev.sort_changed())
+    changed = float(np.mean(chg))
+    log(f"[bench] changed list per step (untimed sample): mean {changed:.0f}, max {max(chg)}")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (FP32 restatement)...")
@@ -212,7 +220,8 @@ def main():
                                "tangential history, dt 5e-6 s, outlet open",
                    "particles": n_live, "cells": ncells, "mean_contacts": round(k, 3),
                    "settle_steps": args.settle, "settle_dt": args.settle_dt,
-                   "stage_ms_per_step": {s: round(v / nrec, 5) for s, v in stages.items()}},
+                   "stage_ms_per_step": {s: round(v / nrec, 5) for s, v in stages.items()},
+                   "changed_per_step": round(changed)},
         "roofline": roof,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,

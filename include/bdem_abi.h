@@ -155,10 +155,14 @@ int bdem_sync(bdem_ctx* ctx);
  * max_steps bdem_update_sim calls.  No reference counterpart (the reference
  * has no timers). */
 int bdem_timing_begin(bdem_ctx* ctx, int max_steps);
-/* Wait, then return the summed milliseconds of the 4 stages
- * {hash+sort, gather, cell table, fused step kernel} and the number of
- * steps recorded; releases the events. */
+/* Wait, then return the summed milliseconds of the 2 stages
+ * {sort (hash if needed, resort or full sort, gather, cell table), fused step
+ * kernel} and the number of steps recorded; releases the events. */
 int bdem_timing_end(bdem_ctx* ctx, double* stage_ms, int* nsteps);
+/* Wait, then return how many particles the last sort had to move: the size
+ * of the changed list for an incremental resort, N for a full sort.  No
+ * reference counterpart. */
+int bdem_sort_changed(bdem_ctx* ctx, unsigned int* changed);
 
 /* ---- stage outputs for parity checks (synchronous; state unchanged) -------- */
 /* Keys and identity index of the current state (Kernel_SpatialDecomp_CalcPHash). */

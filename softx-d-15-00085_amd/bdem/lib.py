@@ -53,6 +53,7 @@ _PROTOS = {
     "bdem_get_energy": (C.c_int, [_P, _fp]),
     "bdem_get_status": (C.c_int, [_P, _up]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
+    "bdem_sort_changed": (C.c_int, [_P, _up]),
     "bdem_num_particles": (C.c_int, [_P]),
     "bdem_num_cells": (C.c_longlong, [_P]),
     "bdem_update_dobject_positions": (C.c_int, [_P, C.POINTER(T.DOBJ)]),
@@ -198,6 +199,12 @@ class Device:
         self._check(self.lib.bdem_get_kill_count(self.h, C.byref(k)))
         return k.value
 
+    def sort_changed(self):
+        """Particles the last sort moved (changed list size, N after a full sort)."""
+        k = C.c_uint(0)
+        self._check(self.lib.bdem_sort_changed(self.h, C.byref(k)))
+        return k.value
+
     def num_cells(self):
         return int(self.lib.bdem_num_cells(self.h))
 
@@ -218,13 +225,13 @@ class Device:
         self._check(self.lib.bdem_drum_rotation(self.h, 1 if on else 0))
 
     # -- measurement ------------------------------------------------------
-    STAGES = ("sort", "place", "cells", "step")
+    STAGES = ("sort", "step")
 
     def timing_begin(self, max_steps):
         self._check(self.lib.bdem_timing_begin(self.h, int(max_steps)))
 
     def timing_end(self):
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * len(self.STAGES))()
         n = C.c_int(0)
         self._check(self.lib.bdem_timing_end(self.h, ms, C.byref(n)))
         return dict(zip(self.STAGES, list(ms))), n.value

@@ -44,6 +44,8 @@ struct bdem_ctx {
   bool flags_valid = false;  /* ... and the step kernel's changed flags for them */
   int hist_par = 0;
   int sorted_par = 0;        /* keys_sorted[sorted_par] = latest sorted keys */
+  int append_par = 0;        /* meta[META_APPEND + append_par] counts the changed list */
+  bool full_next = false;    /* a discharge check ran: many keys changed at once */
   int stepc = 0;
   /* stage timing: one event per stage boundary per recorded step */
   std::vector<hipEvent_t> tev;
@@ -51,9 +53,9 @@ struct bdem_ctx {
 };
 
 /* Stage boundaries recorded per step when timing is on:
- * 0 start | hash + changed-set sort | place + gather | cell table (+ fallback)
+ * 0 start | sort (hash if needed, resort or full sort, gather, cell table)
  * | step (fused kernel) */
-static const int BDEM_NSTAGE = 4;
+static const int BDEM_NSTAGE = 2;
 
 namespace {
 
@@ -234,10 +236,8 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     for (int i = 0; i < n; ++i)
       if (pid[i] >= c->b.hist_cap)
         return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
-    for (int h = 0; h < 2; ++h) {
-      launch_fill_u32((uint32_t*)c->b.hcount[h], c->b.hist_cap, 0u, c->stream);
-      launch_fill_u32((uint32_t*)c->b.wcount[h], c->b.hist_cap, 0u, c->stream);
-    }
+    launch_fill_u32(c->b.hwc_cur, c->b.n, 0u, c->stream);
+    launch_fill_u32(c->b.hwc_srt, c->b.n, 0u, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   c->keys_valid = false;
@@ -268,7 +268,7 @@ hipEvent_t* timing_slot(bdem_ctx* c) {
  * On return b.keys_sorted[c->sorted_par] and b.idx_sorted hold the order. */
 int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   if (ev) (void)hipEventRecord(ev[0], c->stream);
-  const bool incremental = c->keys_valid && c->flags_valid;
+  const bool incremental = c->keys_valid && c->flags_valid && !c->full_next;
   if (!c->keys_valid) launch_hash(c->dp, c->b, c->stream);
   const int prev = c->sorted_par;
   const int next = incremental ? prev ^ 1 : prev;
@@ -281,7 +281,6 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   a.changed = b.changed;
   a.sbuf = b.sbuf;
   a.skey = b.skey;
-  a.sslot = b.sslot;
   a.okeys = b.keys_sorted[prev];
   a.keys_sorted = b.keys_sorted[next];
   a.idx_sorted = b.idx_sorted;
@@ -296,9 +295,14 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   a.quat_cur = c->orient ? b.quat_cur : nullptr;
   a.pos_srt = b.pos_srt; a.vel_srt = b.vel_srt; a.omg_srt = b.omg_srt;
   a.quat_srt = c->orient ? b.quat_srt : nullptr;
-  hipError_t e = launch_sort(a, incremental, c->stream, ev ? ev + 1 : nullptr);
+  a.hwc_cur = b.hwc_cur;
+  a.hwc_srt = b.hwc_srt;
+  hipError_t e = launch_sort(a, incremental, c->append_par, c->stream);
+  if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("sort launch: ") + hipGetErrorString(e));
   c->sorted_par = next;
+  c->append_par ^= 1;
+  c->full_next = false;
   /* the step kernel must produce new keys and flags before the next sort */
   c->keys_valid = false;
   c->flags_valid = false;
@@ -399,8 +403,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.idx_sorted, n);
   ALLOC(b.changed, n);
   ALLOC(b.sbuf, n);
-  ALLOC(b.skey, SMALL_MAX);
-  ALLOC(b.sslot, SMALL_MAX);
+  ALLOC(b.skey, n);
   ALLOC(b.meta, META_WORDS);
   ALLOC(b.rk[0], n); ALLOC(b.rk[1], n); ALLOC(b.rv[0], n); ALLOC(b.rv[1], n);
   ALLOC(b.hist, (size_t)RADIX * sort_blocks(n));
@@ -452,16 +455,16 @@ int bdem_set_ext_params(bdem_ctx* c, const bdem_ext_params* ext) {
   c->ext = *ext;
   c->has_ext = true;
   const int n = c->b.n;
-  if (need_history(c) && !c->b.hcount[0]) {
+  if (need_history(c) && !c->b.hwc_cur) {
     c->b.hist_cap = n;
+    if ((rc = dalloc(c, &c->b.hwc_cur, n)) != BDEM_OK) return rc;
+    if ((rc = dalloc(c, &c->b.hwc_srt, n)) != BDEM_OK) return rc;
     for (int h = 0; h < 2; ++h) {
-      if ((rc = dalloc(c, &c->b.hcount[h], n)) != BDEM_OK) return rc;
       if ((rc = dalloc(c, &c->b.hrec[h], (size_t)n * HMAX)) != BDEM_OK) return rc;
-      if ((rc = dalloc(c, &c->b.wcount[h], n)) != BDEM_OK) return rc;
       if ((rc = dalloc(c, &c->b.wrec[h], (size_t)n * WMAX)) != BDEM_OK) return rc;
-      launch_fill_u32((uint32_t*)c->b.hcount[h], n, 0u, c->stream);
-      launch_fill_u32((uint32_t*)c->b.wcount[h], n, 0u, c->stream);
     }
+    launch_fill_u32(c->b.hwc_cur, n, 0u, c->stream);
+    launch_fill_u32(c->b.hwc_srt, n, 0u, c->stream);
   }
   return upload_params(c);
 }
@@ -624,6 +627,16 @@ int bdem_get_kill_count(bdem_ctx* c, int* k) {
   return BDEM_OK;
 }
 
+int bdem_sort_changed(bdem_ctx* c, unsigned int* changed) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!changed) return fail(c, BDEM_EINVAL, "null output");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(changed, c->b.meta + META_COUNT, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return BDEM_OK;
+}
+
 int bdem_num_particles(const bdem_ctx* c) { return (c && c->configured) ? c->b.n : 0; }
 long long bdem_num_cells(const bdem_ctx* c) { return (c && c->configured) ? (long long)c->hp.C : 0; }
 
@@ -698,10 +711,11 @@ int bdem_update_sim(bdem_ctx* c) {
     if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
     else c->stepc += 1;
   }
-  launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par, do_kill,
-              c->record_wf, c->stream);
+  launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
+              c->append_par, do_kill, c->record_wf, c->stream);
+  if (do_kill) c->full_next = true;
   if (ev) {
-    (void)hipEventRecord(ev[4], c->stream);
+    (void)hipEventRecord(ev[2], c->stream);
     c->t_used += 1;
   }
   HIPCHK(c, hipGetLastError());
@@ -746,7 +760,7 @@ int bdem_timing_end(bdem_ctx* c, double* stage_ms, int* nsteps) {
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double acc[BDEM_NSTAGE] = {0, 0, 0, 0};
+  double acc[BDEM_NSTAGE] = {};
   for (int s = 0; s < c->t_used; ++s) {
     hipEvent_t* ev = &c->tev[(size_t)s * (BDEM_NSTAGE + 1)];
     for (int k = 0; k < BDEM_NSTAGE; ++k) {

@@ -34,6 +34,17 @@ enum : uint32_t {
   ST_SORT_ERROR    = 8u,   /* incremental sort produced an out-of-range position */
 };
 
+/* MI355X dispatches workgroup w to XCD w % 8, and each XCD has its own L2.
+ * Renumbering so that XCD x runs one contiguous eighth of the logical blocks
+ * keeps the neighbour rows a block reads (one y-layer of cells either side,
+ * about 19 blocks of 256 away in sorted order) in the same L2 that loaded
+ * them for the blocks next to it.  A bijection on [0, nb) for any nb. */
+constexpr int NUM_XCD = 8;
+__device__ __forceinline__ int xcd_block(int w, int nb) {
+  const int x = w % NUM_XCD, q = w / NUM_XCD, per = nb / NUM_XCD, rem = nb % NUM_XCD;
+  return x * per + (x < rem ? x : rem) + q;
+}
+
 struct f3 { float x, y, z; };
 __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -148,98 +159,11 @@ __device__ __forceinline__ bool overlaps(f3 pa, float ra, float4 pb) {
   const float d2 = dot(d, d);
   const float s = ra + pb.w;
   const float s2 = s * s;
-  if (d2 < s2 * 0.99999904632568359375f) return true;   /* 1 - 2^-20 */
-  if (d2 > s2 * 1.00000095367431640625f) return false;  /* 1 + 2^-20 */
-  return __builtin_sqrtf(d2) - s < 0.0f;
-}
-
-/* Wave-cooperative form of the same traversal.  The 64 lanes are 64
- * consecutive sorted particles, so for one stencil row offset (dy, dz) their
- * cell runs and candidate runs overlap almost entirely: the wave loads the
- * union of its cell entries (<= 64) and of its candidate positions
- * (<= STAGE_POS) into LDS with coalesced loads, then each lane scans its own
- * range from LDS in the reference order.  Rows whose union is wider fall back
- * to global loads.  Every lane of the wave must call it; `active` false for
- * lanes without a live particle.  Same visits, same order as
- * for_each_contact. */
-constexpr int STAGE_POS = 128;
-struct WaveStage {
-  float4 pos[STAGE_POS];
-  uint2 cell[64];
-};
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
-}
-
-template <class Visit>
-__device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const uint2* __restrict__ cells,
-                                                     const float4* __restrict__ pos_r, int i, f3 pa,
-                                                     float ra, bool active, WaveStage& st,
-                                                     Visit&& visit) {
-  const int lane = threadIdx.x & 63;
-  int g[3];
-  grid_pos(P, pa, g);
-  const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
-  int count = 0;
-  for (int dy = -1; dy <= 1; ++dy) {
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int y = g[1] + dy, z = g[2] + dz;
-      bool rv = active && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
-      const uint32_t h0 = rv ? hash_of(P, xs, y, z) : 0u;
-      const uint32_t h1 = rv ? hash_of(P, xe, y, z) : 0u;
-      if (rv && h1 >= P.C) rv = false;
-      /* stage the wave's cell entries */
-      const uint32_t cmin = wave_min_u32(rv ? h0 : 0xFFFFFFFFu);
-      const uint32_t cmax = wave_max_u32(rv ? h1 : 0u);
-      if (cmin > cmax) continue;                    /* no lane has this row */
-      const bool cst = cmax - cmin < 64u;
-      if (cst) {
-        if (cmin + (uint32_t)lane <= cmax) st.cell[lane] = cells[cmin + lane];
-        __builtin_amdgcn_wave_barrier();
-      }
-      uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-      if (rv) {
-        for (uint32_t h = h0; h <= h1; ++h) {
-          const uint2 c = cst ? st.cell[h - cmin] : cells[h];
-          if (c.x != 0xFFFFFFFFu) {
-            lo = min(lo, c.x);
-            hi = c.y;
-          }
-        }
-      }
-      const bool ne = lo != 0xFFFFFFFFu;
-      /* stage the wave's candidate positions */
-      const uint32_t pmin = wave_min_u32(ne ? lo : 0xFFFFFFFFu);
-      const uint32_t pmax = wave_max_u32(ne ? hi : 0u);
-      __builtin_amdgcn_wave_barrier();
-      if (pmin >= pmax) continue;
-      const bool pst = pmax - pmin <= (uint32_t)STAGE_POS;
-      if (pst) {
-        for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
-        __builtin_amdgcn_wave_barrier();
-      }
-      if (ne) {
-        for (uint32_t j = lo; j < hi; ++j) {
-          if ((int)j == i) continue;
-          const float4 pb = pst ? st.pos[j - pmin] : pos_r[j];
-          if (overlaps(pa, ra, pb)) {
-            visit((int)j, pb, count);
-            ++count;
-          }
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
-  }
-  return count;
+  bool hit = d2 < s2 * 0.99999904632568359375f;                          /* 1 - 2^-20 */
+  const bool band = !hit && !(d2 > s2 * 1.00000095367431640625f);        /* 1 + 2^-20 */
+  /* a branch on the rare band only, so the root is skipped by whole waves */
+  if (__builtin_expect(band, 0)) hit = __builtin_sqrtf(d2) - s < 0.0f;
+  return hit;
 }
 
 template <class Visit>
@@ -279,6 +203,144 @@ __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2*
   }
   return count;
 }
+
+/* Wave-cooperative form of the same traversal.  The 64 lanes are 64
+ * consecutive sorted particles, so their cells lie in a short key range
+ * [Hmin, Hmax].  For a stencil row offset `off` (dy, dz) every lane's three
+ * cells are inside [Hmin + off - 1, Hmax + off + 1]: the wave loads that run
+ * of cell entries once (coalesced, no per-row reduction), takes the first
+ * and last occupied entry by ballot, and loads the contiguous run of
+ * candidates between them into LDS.  Each lane then scans its own cells and
+ * candidates from LDS in the reference order.  A wave whose key range is too
+ * wide, or a row with too many candidates, reads global memory instead; a
+ * live particle outside the grid (key C) takes the per-lane traversal.  Every
+ * lane of the wave must call it; `active` false for lanes without a live
+ * particle.  Same visits, same order as for_each_contact. */
+constexpr int STAGE_POS = 128;
+constexpr int STAGE_CELL = 128;
+struct WaveStage {
+  float4 pos[STAGE_POS];
+  uint2 cell[STAGE_CELL];
+};
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+/* Candidates [lo, hi) of one lane, read from `src` (LDS stage or global). */
+template <class Visit>
+__device__ __forceinline__ void scan_candidates(const float4* src, uint32_t lo, uint32_t hi, uint32_t shift,
+                                                int i, f3 pa, float ra, int& count, Visit& visit) {
+  for (uint32_t j = lo; j < hi; ++j) {
+    const float4 pb = src[j - shift];
+    if ((int)j != i && overlaps(pa, ra, pb)) {
+      visit((int)j, pb, count);
+      ++count;
+    }
+  }
+}
+
+template <class Visit>
+__device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const uint2* __restrict__ cells,
+                                                     const float4* __restrict__ pos_r, uint32_t n, int i,
+                                                     f3 pa, float ra, bool active, WaveStage& st,
+                                                     Visit&& visit) {
+  const int lane = threadIdx.x & 63;
+  int g[3];
+  grid_pos(P, pa, g);
+  const bool ingrid = active && g[0] >= 0 && g[1] >= 0 && g[2] >= 0 && g[0] < P.nx && g[1] < P.ny &&
+                      g[2] < P.nz && hash_of(P, g[0], g[1], g[2]) < P.C;
+  const uint32_t H = ingrid ? hash_of(P, g[0], g[1], g[2]) : 0u;
+  const uint32_t hmin = wave_min_u32(ingrid ? H : 0xFFFFFFFFu);
+  const uint32_t hmax = wave_max_u32(ingrid ? H : 0u);
+  const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
+  int count = 0;
+  if (hmin <= hmax) {   /* some lane is in the grid (wave-uniform) */
+    const bool cst = hmax - hmin + 3u <= (uint32_t)STAGE_CELL;
+    for (int dy = -1; dy <= 1; ++dy) {
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int y = g[1] + dy, z = g[2] + dz;
+        const bool rv0 = ingrid && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
+        const uint32_t h0 = rv0 ? hash_of(P, xs, y, z) : 0u;
+        const uint32_t h1 = rv0 ? hash_of(P, xe, y, z) : 0u;
+        const bool rv = rv0 && h1 < P.C;
+        /* wave window of cells for this row (uniform) */
+        const long long off = (long long)dz * P.nz + (long long)dy * P.nx * P.nz;
+        const long long wl = max((long long)hmin + off - 1, 0ll);
+        const long long wh = min((long long)hmax + off + 1, (long long)P.C - 1);
+        if (wl > wh) continue;
+        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+        uint32_t pmin = 0u, pmax = 0u;
+        if (cst) {
+          const uint32_t w0 = (uint32_t)wl, wn = (uint32_t)(wh - wl) + 1u;
+          const uint2 e0 = (uint32_t)lane < wn ? cells[w0 + lane] : make_uint2(0xFFFFFFFFu, 0u);
+          const uint2 e1 = (uint32_t)lane + 64u < wn ? cells[w0 + 64u + lane] : make_uint2(0xFFFFFFFFu, 0u);
+          const uint64_t m0 = __ballot(e0.x != 0xFFFFFFFFu);
+          const uint64_t m1 = __ballot(e1.x != 0xFFFFFFFFu);
+          if ((m0 | m1) == 0ull) continue;   /* no particle in the window */
+          const int f = m0 ? __ffsll((unsigned long long)m0) - 1 : 64 + __ffsll((unsigned long long)m1) - 1;
+          const int l = m1 ? 127 - __clzll((long long)m1) : 63 - __clzll((long long)m0);
+          pmin = f < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.x, f)
+                        : (uint32_t)__builtin_amdgcn_readlane((int)e1.x, f - 64);
+          pmax = l < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.y, l)
+                        : (uint32_t)__builtin_amdgcn_readlane((int)e1.y, l - 64);
+          st.cell[lane] = e0;
+          st.cell[lane + 64] = e1;
+          __builtin_amdgcn_wave_barrier();
+          if (rv) {
+            for (uint32_t h = h0; h <= h1; ++h) {
+              const uint2 c = st.cell[h - w0];
+              if (c.x != 0xFFFFFFFFu) {
+                lo = min(lo, c.x);
+                hi = c.y;
+              }
+            }
+          }
+        } else {
+          if (rv) {
+            for (uint32_t h = h0; h <= h1; ++h) {
+              const uint2 c = cells[h];
+              if (c.x != 0xFFFFFFFFu) {
+                lo = min(lo, c.x);
+                hi = c.y;
+              }
+            }
+          }
+          const bool ne = lo != 0xFFFFFFFFu;
+          pmin = wave_min_u32(ne ? lo : 0xFFFFFFFFu);
+          pmax = wave_max_u32(ne ? hi : 0u);
+          if (pmin >= pmax) continue;
+        }
+        pmax = min(pmax, n);   /* table entries never exceed n; kept in range regardless */
+        if (pmin >= pmax) continue;
+        if (lo == 0xFFFFFFFFu) { lo = pmin; hi = pmin; }
+        hi = min(hi, n);
+        if (pmax - pmin <= (uint32_t)STAGE_POS) {
+          for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
+          __builtin_amdgcn_wave_barrier();
+          /* a consistent cell table puts every lane's run inside the stage;
+           * checked, so LDS is never indexed outside it whatever the table holds */
+          if (lo >= pmin && hi <= pmax) scan_candidates(st.pos, lo, hi, pmin, i, pa, ra, count, visit);
+          else scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+        } else {
+          scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+  }
+  /* a live particle outside the grid (key C): the per-lane traversal */
+  if (active && !ingrid) count = for_each_contact(P, cells, pos_r, i, pa, ra, visit);
+  return count;
+}
+
 
 /* ------------------------------------------------------------------------ */
 /* a6: reference pair law on A from B.                                       */

@@ -110,7 +110,7 @@ __device__ __forceinline__ void particle_contact(const DevParams& P, int j, cons
  * body then runs max(contacts) times instead of max(candidates) times.
  * Particles with more than CLIST contacts (beyond the reference's 32-entry
  * list) finish the tail with a second scan, in the same order. */
-constexpr int CLIST = 24;   /* LDS list entries per thread (VGPR and LDS occupancy both allow 4 blocks per CU) */
+constexpr int CLIST = 16;   /* LDS list entries per thread: 16 KiB + wave staging keeps 5 blocks per CU */
 constexpr int STEP_THREADS = 256;
 
 template <int LAW>
@@ -148,13 +148,15 @@ struct WaveScratch {
   float f[6][64];
   int off[65];
 };
+constexpr int WAVE_LDS_BYTES =
+    sizeof(WaveStage) > sizeof(WaveScratch) ? (int)sizeof(WaveStage) : (int)sizeof(WaveScratch);
 
 template <int LAW>
 __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float4* __restrict__ pos_r,
                                                  const float4* __restrict__ vel_m,
                                                  const float4* __restrict__ omg_t,
-                                                 const uint32_t* clist, int nl,
-                                                 const int* __restrict__ hc_in,
+                                                 const uint32_t* clist, int nl, int base_i,
+                                                 const uint32_t* __restrict__ hwc_in,
                                                  const HistRec* __restrict__ hr_in, HistRec* hr_out,
                                                  int hist_cap, WaveScratch& ws, f3& F, f3& T) {
   const int lane = threadIdx.x & 63;
@@ -181,16 +183,16 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
       }
       const int k = p - ws.off[lo];
       const int otid = wbase_tid + lo;
-      const int oi = blockIdx.x * STEP_THREADS + otid;
+      const int oi = base_i + otid;
       const int j = (int)clist[k * STEP_THREADS + otid];
       const float4 po = pos_r[oi], vo = vel_m[oi], oo = omg_t[oi];
       const int opid = packed_pid(oo.w);
       const HistRec* hin = nullptr;
       int hin_cnt = 0;
       HistRec* rec = nullptr;
-      if (LAW != BDEM_LAW_REFERENCE && opid < hist_cap) {
+      if (LAW != BDEM_LAW_REFERENCE && opid >= 0 && opid < hist_cap) {
         hin = hr_in + (size_t)opid * HMAX;
-        hin_cnt = hc_in[opid];
+        hin_cnt = min((int)(hwc_in[oi] & 0xFFFFu), HMAX);
         if (k < HMAX) rec = hr_out + (size_t)opid * HMAX + k;
       }
       contact_force<LAW>(P, j, pos_r, vel_m, omg_t, xyz(po), po.w, xyz(vo), vo.w, xyz(oo),
@@ -221,18 +223,24 @@ __global__ __launch_bounds__(256) void k_step(
     float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ omg_out,
     float4* __restrict__ quat_out, uint32_t* __restrict__ keys_out,
     const uint32_t* __restrict__ sorted_keys, uint8_t* __restrict__ changed,
-    unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ meta,
-    const int* __restrict__ hc_in, const HistRec* __restrict__ hr_in, int* __restrict__ hc_out,
-    HistRec* __restrict__ hr_out, const int* __restrict__ wc_in, const HistRec* __restrict__ wr_in,
-    int* __restrict__ wc_out, HistRec* __restrict__ wr_out, int hist_cap,
+    unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ append,
+    const uint32_t* __restrict__ hwc_in, uint32_t* __restrict__ hwc_out,
+    const HistRec* __restrict__ hr_in, HistRec* __restrict__ hr_out,
+    const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out, int hist_cap,
     float* __restrict__ wall_force, int do_kill, uint32_t* status, int* kill_count) {
   const DevParams& P = *Pp;
   __shared__ uint32_t clist[CLIST * STEP_THREADS];
-  __shared__ WaveScratch wscr[STEP_THREADS / 64];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  /* per wave: candidate staging in phase 1, pair results in phase 2 */
+  __shared__ __align__(16) unsigned char wlds[STEP_THREADS / 64][WAVE_LDS_BYTES];
+  const int base_i = xcd_block(blockIdx.x, gridDim.x) * STEP_THREADS;
+  const int i = base_i + threadIdx.x;
   const bool in = i < n;
   float4 pr = make_float4(0.f, 0.f, 0.f, 0.f), vm = pr, ot = pr;
-  if (in) { pr = pos_in[i]; vm = vel_in[i]; ot = omg_in[i]; }
+  uint32_t hwc = 0u;
+  if (in) {
+    pr = pos_in[i]; vm = vel_in[i]; ot = omg_in[i];
+    if (LAW != BDEM_LAW_REFERENCE) hwc = hwc_in[i];
+  }
   int pid = in ? packed_pid(ot.w) : -1;
   const int ta = packed_type(ot.w);
   const bool alive = in && pid >= 0;
@@ -240,18 +248,18 @@ __global__ __launch_bounds__(256) void k_step(
   const float r = pr.w, m = vm.w;
 
   /* phase 1: contact partners in the reference order, into this thread's list */
-  int nc = 0;
-  if (alive) {
-    nc = for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
-      if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
-    });
-  }
+  const int nc = for_each_contact_wave(
+      P, cells, pos_in, (uint32_t)n, i, p, r, alive, *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]),
+      [&](int j, float4, int k) {
+        if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
+      });
+  __builtin_amdgcn_wave_barrier();
   /* phase 2: gravity first (DEM_Kernels_Compute.cuh:654), then the contacts */
   f3 Fpp = alive ? cvt(P.gravity) * m : mk3(0.f, 0.f, 0.f);
   f3 Tpp = mk3(0.f, 0.f, 0.f);
   const int nl = nc < CLIST ? nc : CLIST;
-  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, hc_in, hr_in, hr_out, hist_cap,
-                        wscr[threadIdx.x >> 6], Fpp, Tpp);
+  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, base_i, hwc_in, hr_in, hr_out, hist_cap,
+                        *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp);
 
   bool chg = false;
   if (alive) {
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(256) void k_step(
       PairHistCtx hc{nullptr, 0, 0, nullptr, HMAX};   /* springs looked up, not stored */
       if (hist_slot) {
         hc.in = hr_in + (size_t)pid * HMAX;
-        hc.in_cnt = hc_in[pid];
+        hc.in_cnt = min((int)(hwc & 0xFFFFu), HMAX);
       }
       for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
         if (k >= CLIST) particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc);
@@ -276,7 +284,7 @@ __global__ __launch_bounds__(256) void k_step(
     if (LAW != BDEM_LAW_REFERENCE) {
       if (hist_slot) {
         wh.in = wr_in + (size_t)pid * WMAX;
-        wh.in_cnt = wc_in[pid];
+        wh.in_cnt = min((int)(hwc >> 16), WMAX);
         wh.out = wr_out + (size_t)pid * WMAX;
       } else {
         wh.out = scratch_wall; wh.out_cnt = WMAX;   /* no slot: springs are not kept */
@@ -287,10 +295,8 @@ __global__ __launch_bounds__(256) void k_step(
     if (wall_force) {
       wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
     }
-    if (hist_slot) {
-      hc_out[pid] = nc < HMAX ? nc : HMAX;
-      wc_out[pid] = wh.out_cnt;
-    }
+    if (LAW != BDEM_LAW_REFERENCE)
+      hwc_out[i] = hist_slot ? (uint32_t)(nc < HMAX ? nc : HMAX) | ((uint32_t)wh.out_cnt << 16) : 0u;
 
     const float inv_inertia = P.per_particle ? 1.0f / (0.4f * r * r * m) : P.t_inertia[ta];
     QuatF q{1.f, 0.f, 0.f, 0.f};
@@ -313,6 +319,7 @@ __global__ __launch_bounds__(256) void k_step(
     vel_out[i] = vm;
     omg_out[i] = ot;
     if (ORIENT) quat_out[i] = quat_in[i];
+    if (LAW != BDEM_LAW_REFERENCE) hwc_out[i] = 0u;
     if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
   }
   uint32_t new_key = 0u;
@@ -328,35 +335,38 @@ __global__ __launch_bounds__(256) void k_step(
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((unsigned long long)mask) - 1;
     uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(&meta[META_APPEND], (uint32_t)__popcll(mask));
+    if (lane == leader) base = atomicAdd(append, (uint32_t)__popcll(mask));
     base = __shfl(base, leader, 64);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    if (chg) sbuf[base + (uint32_t)__popcll(mask & lt)] = ((unsigned long long)new_key << 32) | (uint32_t)i;
+    const uint32_t at = base + (uint32_t)__popcll(mask & lt);
+    if (chg) {
+      if (at < (uint32_t)n) sbuf[at] = ((unsigned long long)new_key << 32) | (uint32_t)i;
+      else atomicOr(status, ST_SORT_ERROR);   /* a counter left uncleared: never write past the list */
+    }
   }
 }
 
 template <int LAW, bool ORIENT>
-static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp,
+static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
                           bool do_kill, bool want_wf, hipStream_t s) {
   hipLaunchKernelGGL((k_step<LAW, ORIENT>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n,
                      b.cells, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
-                     b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta, b.hcount[hp],
-                     b.hrec[hp], b.hcount[hp ^ 1], b.hrec[hp ^ 1], b.wcount[hp], b.wrec[hp],
-                     b.wcount[hp ^ 1], b.wrec[hp ^ 1], b.hist_cap, want_wf ? b.wall_force : nullptr,
-                     do_kill ? 1 : 0, b.status, b.kill_count);
+                     b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta + META_APPEND + ap,
+                     b.hwc_srt, b.hwc_cur, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp], b.wrec[hp ^ 1],
+                     b.hist_cap, want_wf ? b.wall_force : nullptr, do_kill ? 1 : 0, b.status, b.kill_count);
 }
 
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int law, bool orient,
-                 int hp, bool do_kill, bool want_wf, hipStream_t s) {
+                 int hp, int ap, bool do_kill, bool want_wf, hipStream_t s) {
   if (law == BDEM_LAW_REFERENCE) {
-    if (orient) launch_step_t<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, do_kill, want_wf, s);
+    if (orient) launch_step_t<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    else launch_step_t<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
   } else if (law == BDEM_LAW_LSD_COULOMB) {
-    if (orient) launch_step_t<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, do_kill, want_wf, s);
+    if (orient) launch_step_t<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    else launch_step_t<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
   } else {
-    if (orient) launch_step_t<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, do_kill, want_wf, s);
+    if (orient) launch_step_t<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    else launch_step_t<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
   }
 }
 
@@ -393,9 +403,9 @@ template <int LAW>
 __global__ __launch_bounds__(256) void k_forces(
     const DevParams* __restrict__ Pp, int n, const uint2* __restrict__ cells,
     const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
-    const float4* __restrict__ omg_in, const int* __restrict__ hc_in,
+    const float4* __restrict__ omg_in, const uint32_t* __restrict__ hwc_in,
     const HistRec* __restrict__ hr_in, HistRec* __restrict__ hr_out,
-    const int* __restrict__ wc_in, const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out,
+    const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out,
     int hist_cap, float* __restrict__ fpp, float* __restrict__ tpp, float* __restrict__ fw,
     float* __restrict__ tw) {
   const DevParams& P = *Pp;
@@ -418,8 +428,11 @@ __global__ __launch_bounds__(256) void k_forces(
     HistRec scratch_wall[1];
     if (LAW != BDEM_LAW_REFERENCE) {
       if (pid < hist_cap) {
-        hc.in = hr_in + (size_t)pid * HMAX; hc.in_cnt = hc_in[pid]; hc.out = hr_out + (size_t)pid * HMAX;
-        wh.in = wr_in + (size_t)pid * WMAX; wh.in_cnt = wc_in[pid]; wh.out = wr_out + (size_t)pid * WMAX;
+        const uint32_t hwc = hwc_in[i];
+        hc.in = hr_in + (size_t)pid * HMAX; hc.in_cnt = min((int)(hwc & 0xFFFFu), HMAX);
+        hc.out = hr_out + (size_t)pid * HMAX;
+        wh.in = wr_in + (size_t)pid * WMAX; wh.in_cnt = min((int)(hwc >> 16), WMAX);
+        wh.out = wr_out + (size_t)pid * WMAX;
       } else {
         hc.out = scratch_pair; hc.out_cnt = HMAX;
         wh.out = scratch_wall; wh.out_cnt = WMAX;
@@ -441,16 +454,16 @@ void launch_forces(const DevParams* dP, const StepBuffers& b, int law, int hp, f
   /* springs are written to the idle half so the state's history is untouched */
   if (law == BDEM_LAW_REFERENCE)
     hipLaunchKernelGGL(k_forces<BDEM_LAW_REFERENCE>, g, blk, 0, s, dP, b.n, b.cells, b.pos_srt,
-                       b.vel_srt, b.omg_srt, b.hcount[hp], b.hrec[hp], b.hrec[hp ^ 1], b.wcount[hp],
-                       b.wrec[hp], b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
+                       b.vel_srt, b.omg_srt, b.hwc_srt, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp],
+                       b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
   else if (law == BDEM_LAW_LSD_COULOMB)
     hipLaunchKernelGGL(k_forces<BDEM_LAW_LSD_COULOMB>, g, blk, 0, s, dP, b.n, b.cells, b.pos_srt,
-                       b.vel_srt, b.omg_srt, b.hcount[hp], b.hrec[hp], b.hrec[hp ^ 1], b.wcount[hp],
-                       b.wrec[hp], b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
+                       b.vel_srt, b.omg_srt, b.hwc_srt, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp],
+                       b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
   else
     hipLaunchKernelGGL(k_forces<BDEM_LAW_HERTZ_MINDLIN>, g, blk, 0, s, dP, b.n, b.cells, b.pos_srt,
-                       b.vel_srt, b.omg_srt, b.hcount[hp], b.hrec[hp], b.hrec[hp ^ 1], b.wcount[hp],
-                       b.wrec[hp], b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
+                       b.vel_srt, b.omg_srt, b.hwc_srt, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp],
+                       b.wrec[hp ^ 1], b.hist_cap, fpp, tpp, fw, tw);
 }
 
 }  // namespace bdem

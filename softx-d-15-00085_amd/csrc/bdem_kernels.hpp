@@ -20,17 +20,19 @@ struct StepBuffers {
   int *idx_sorted;                     /* sorted order as original slots */
   uint8_t *changed;                    /* key changed in the last step kernel */
   unsigned long long *sbuf;            /* appended (key<<32 | slot) of changed slots */
-  unsigned long long *skey;            /* changed set sorted by (key, slot) */
-  uint32_t *sslot;                     /* changed slots in slot order */
+  unsigned long long *skey;            /* n composites of scratch for oversized windows */
   uint32_t *meta;                      /* META_* words */
   uint32_t *rk[2];                     /* full radix sort ping-pong */
   int *rv[2];
   uint32_t *hist;                      /* full radix digit counters */
   uint32_t *part;                      /* full radix scan partials (<= 1024) */
   uint2 *cells;                        /* (start, end) per cell, reference format */
-  /* tangential history, indexed by P_ID, double-buffered across steps */
-  int *hcount[2]; HistRec *hrec[2];
-  int *wcount[2]; HistRec *wrec[2];
+  /* tangential history: records indexed by P_ID, double-buffered across
+   * steps; their counts (pair | wall << 16) travel with the state in slot
+   * order, so they are read and written coalesced */
+  uint32_t *hwc_cur, *hwc_srt;
+  HistRec *hrec[2];
+  HistRec *wrec[2];
   int hist_cap;                        /* number of P_ID slots in the history tables */
   float *wall_force;                   /* optional n*3 output of the wall force */
   uint32_t *status;                    /* ST_* bits */
@@ -41,8 +43,14 @@ constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
 constexpr int RADIX = 256;
-constexpr int SMALL_MAX = 16384;       /* changed slots one workgroup sorts in LDS */
-enum { META_APPEND = 0, META_COUNT = 1, META_FALLBACK = 2, META_WORDS = 4 };
+/* incremental resort: 1024 threads own 4096 consecutive slots; the changed
+ * composites of a block's window are sorted in LDS up to RS_LCAP */
+constexpr int RS_THREADS = 1024;
+constexpr int RS_TILE = 4096;
+constexpr int RS_LCAP = 4096;
+/* meta words: two append counters used on alternate steps, the size of the
+ * last changed set */
+enum { META_APPEND = 0, META_COUNT = 2, META_WORDS = 4 };
 
 inline __host__ __device__ int sort_blocks(int n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 int key_bits(uint32_t max_key);
@@ -57,7 +65,6 @@ struct SortArgs {
   const uint8_t* changed;
   const unsigned long long* sbuf;
   unsigned long long* skey;
-  uint32_t* sslot;
   const uint32_t* okeys;               /* previous step's sorted keys */
   uint32_t* keys_sorted;
   int* idx_sorted;
@@ -70,19 +77,24 @@ struct SortArgs {
   uint32_t* status;
   const float4 *pos_cur, *vel_cur, *omg_cur, *quat_cur;
   float4 *pos_srt, *vel_srt, *omg_srt, *quat_srt;
+  const uint32_t* hwc_cur;             /* history counts, gathered with the state when set */
+  uint32_t* hwc_srt;
 };
 
 /* Individual stages (all asynchronous on `s`). */
 void launch_hash(const DevParams* dP, const StepBuffers& b, hipStream_t s);
 /* Sort + gather + cell table.  incremental: keys_new came from the step
- * kernel together with its changed list; otherwise (or as the in-stream
- * fallback) the cooperative full radix sort runs. */
-hipError_t launch_sort(const SortArgs& a, bool incremental, hipStream_t s, hipEvent_t* ev = nullptr);
+ * kernel together with its changed list, counted in meta[META_APPEND + par];
+ * otherwise the cooperative full radix sort runs.  Either way the counter
+ * meta[META_APPEND + (par ^ 1)] is cleared for the next step kernel. */
+hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t s);
 /* Fused contact search + pair force + walls + integration, srt -> cur.
  * Writes next-step keys, the changed flags (against sorted_keys, this step's
- * sorted keys) and appends changed slots to sbuf. */
+ * sorted keys) and appends changed slots to sbuf, counted in
+ * meta[META_APPEND + append_par]. */
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
-                 bool orient, int hist_par, bool do_kill, bool want_wall_force, hipStream_t s);
+                 bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force,
+                 hipStream_t s);
 
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,

@@ -7,28 +7,37 @@
  *
  * Result: keys ascending, ties in ascending slot order -- exactly what a
  * stable sort of (key, slot) produces -- so the sorted order is bit-exact.
+ * Elements are compared as composites c = key << 32 | slot, which are unique.
  *
- * Incremental path (every step after the first).  The slots are in the
- * previous step's sorted order; the step kernel appended every particle
- * whose key changed to a small list S.  Unchanged particles already form a
- * sorted subsequence R, so with S sorted both by (key, slot) and by slot:
- *   unchanged slot i with key k:  pos = i - #{S: slot < i} + #{S: (key,slot) < (k,i)}
- *   changed  slot i with key k:   pos = rank of (k,i) in S + #{R: (key,slot) < (k,i)},
- *     where #{R < (k,i)} = #{all: (oldkey,slot) < (k,i)} - #{S: (oldkey,slot) < (k,i)}
- *     (old keys are the previous sorted keys; unchanged elements keep theirs).
- *   k_sort_small   one workgroup bitonic-sorts S by (key<<32|slot) and by slot in
- *                  LDS, and clears the cells the changed particles left
- *   k_place        every slot computes its position (binary searches within
- *                  the block's window of S), writes the sorted key and slot,
- *                  and gathers the three state streams
- *   k_bin          writes (start, end) of every occupied cell
- * Three launches, ~110 bytes per particle, no full-table clear.
+ * Incremental path (every step after the first): one kernel, k_resort.
+ * The slots are in the previous step's sorted order, so the old composites
+ * increase with the slot.  The step kernel flagged every particle whose key
+ * changed and appended its new composite to a list S (any order).  Unchanged
+ * particles keep their composite and their relative order.
+ *
+ * Block b owns slots [s0, s1).  Let W_lo be the composite of the first
+ * unchanged slot >= s0 (0 for block 0) and W_hi that of the first unchanged
+ * slot >= s1 (infinity past the end).  The windows [W_lo, W_hi) partition
+ * the composite space, so every element of the new order belongs to exactly
+ * one block: its unchanged slots U_b and the changed composites L_b that
+ * fall in its window.  With below = #{S < W_lo} and chg_before = #{S: slot < s0}
+ * the block's output starts at base = (s0 - chg_before) + below, and
+ *   unchanged u:  pos = base + #{U_b: slot < u} + #{L_b < c_u}
+ *   changed  l:   pos = base + #{L_b < l}      + #{U_b < l}
+ * Each block reads all of S once (it is small), sorts L_b in LDS, writes
+ * its keys, slots and gathered state, then the cell bounds of its output:
+ * the neighbours across its output edges are the largest element below its
+ * window and the smallest above it, both known from the same pass over S.
+ * A cell is cleared only when it becomes empty: every particle of its old
+ * run changed key and no changed particle arrives in it.  The block owning
+ * the first slot of that run decides it, so no cell has two writers.
+ * When L_b exceeds LDS the block sorts it in global scratch instead (slow,
+ * correct): the kernel handles any S.
  *
  * Full path: one cooperative kernel does the LSD radix sort (8-bit digits,
- * wave64 ballot ranking), the gather, the table clear and the binning.  It
- * runs on the first step after a host upload and, gated on a device flag, as
- * the fallback when S does not fit one workgroup; otherwise it returns at
- * once, so the fallback costs one empty launch and no host round trip.
+ * wave64 ballot ranking), the gather, the table clear and the binning.  The
+ * host runs it on the first step after an upload and after a discharge
+ * check, when many particles change key at once.
  */
 #include <hip/hip_cooperative_groups.h>
 #include "bdem_kernels.hpp"
@@ -45,140 +54,318 @@ int key_bits(uint32_t max_key) {
 
 int sort_passes(uint32_t max_key) { return (key_bits(max_key) + 7) / 8; }
 
-/* ------------------------------------------------------------------------ */
-/* Incremental path                                                          */
-/* ------------------------------------------------------------------------ */
-__global__ __launch_bounds__(1024) void k_sort_small(SortArgs a) {
-  __shared__ unsigned long long v[SMALL_MAX];
-  const uint32_t ns = a.meta[META_APPEND];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    a.meta[META_APPEND] = 0u;          /* the next step appends afresh */
-    a.meta[META_COUNT] = ns;
-    a.meta[META_FALLBACK] = ns > (uint32_t)SMALL_MAX ? 1u : 0u;
-  }
-  if (ns > (uint32_t)SMALL_MAX || ns == 0u) return;
-  uint32_t m = 1u;
-  while (m < ns) m <<= 1;
-  /* pass 1: by (key, slot) */
-  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) v[i] = i < ns ? a.sbuf[i] : ~0ull;
-  __syncthreads();
-  for (int pass = 0; pass < 2; ++pass) {
-    for (uint32_t k = 2; k <= m; k <<= 1) {
-      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-        for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
-          const uint32_t l = i ^ j;
-          if (l > i) {
-            const unsigned long long x = v[i], y = v[l];
-            if ((x > y) == ((i & k) == 0)) { v[i] = y; v[l] = x; }
-          }
-        }
-        __syncthreads();
-      }
-    }
-    if (pass == 0) {
-      for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) a.skey[i] = v[i];
-      __syncthreads();
-      /* pass 2: by slot (as 64-bit values slot<<32) */
-      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
-        v[i] = i < ns ? ((a.sbuf[i] & 0xFFFFFFFFull) << 32) : ~0ull;
-      __syncthreads();
-    } else {
-      for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) a.sslot[i] = (uint32_t)(v[i] >> 32);
-    }
-  }
-  /* clear the cells the changed particles left; k_bin rewrites every occupied cell */
-  for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
-    const uint32_t slot = (uint32_t)(a.sbuf[i] & 0xFFFFFFFFull);
-    const uint32_t ok = a.okeys[slot];
-    if (ok < a.C) a.cells[ok] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-  }
-}
-
-__device__ __forceinline__ uint32_t lb_u64(const unsigned long long* __restrict__ arr, uint32_t lo,
-                                           uint32_t hi, unsigned long long x) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (arr[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-__device__ __forceinline__ uint32_t lb_u32(const uint32_t* __restrict__ arr, uint32_t lo, uint32_t hi,
-                                           uint32_t x) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (arr[mid] < x) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
 __device__ __forceinline__ unsigned long long comp(uint32_t k, uint32_t s) {
   return ((unsigned long long)k << 32) | s;
 }
+constexpr unsigned long long COMP_INF = ~0ull;
 
-/* Position of every slot in the new order, then the gather.  The S windows
- * of a block are found once: slots [b0, b0+256) bound the slot searches,
- * and the block's unchanged elements (sorted by (key, slot)) bound the key
- * searches between its first and last unchanged element. */
-__global__ __launch_bounds__(256) void k_place(SortArgs a) {
-  if (a.meta[META_FALLBACK] != 0u) return;
-  const uint32_t ns = a.meta[META_COUNT];
+/* ------------------------------------------------------------------------ */
+/* Incremental path                                                          */
+/* ------------------------------------------------------------------------ */
+constexpr int RS_ITEMS = RS_TILE / RS_THREADS;
+constexpr int RS_UNROLL = 4;
+static_assert(RS_ITEMS * (RS_THREADS / 64) == 64, "one wave scans the per-(item row, wave) counts");
+
+/* Ascending sort of v[0, cnt) by a bitonic network whose first stage of
+ * every merge compares i with i ^ (k - 1): every comparator puts the
+ * smaller value at the lower index, so indices >= cnt act as +infinity and
+ * are skipped.  V is an LDS or global array; the caller's threads all call. */
+template <class V>
+__device__ __forceinline__ void block_sort_u64(V v, uint32_t cnt) {
+  uint32_t m = 1u;
+  while (m < cnt) m <<= 1;
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t l = (j == (k >> 1)) ? (i ^ (k - 1u)) : (i ^ j);
+        if (l > i && l < cnt) {
+          const unsigned long long x = v[i], y = v[l];
+          if (x > y) { v[i] = y; v[l] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <class V>
+__device__ __forceinline__ uint32_t lower_bound_u64(V v, uint32_t cnt, unsigned long long x) {
+  uint32_t lo = 0u, hi = cnt;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+  return v;
+}
+
+/* First slot >= from (step +1) or last slot <= from (step -1) that is
+ * unchanged, scanning 64 flags per wave iteration; -1 if none.  One wave. */
+__device__ __forceinline__ int wave_find_unchanged(const uint8_t* __restrict__ changed, int n, int from,
+                                                   int dir) {
+  const int lane = threadIdx.x & 63;
+  for (int base = from; base >= 0 && base < n; base += 64 * dir) {
+    const int s = base + dir * lane;
+    const bool u = s >= 0 && s < n && changed[s] == 0;
+    const uint64_t m = __ballot(u);
+    if (m) return base + dir * (__ffsll((unsigned long long)m) - 1);
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_resort(SortArgs a, int par) {
+  __shared__ uint32_t okey_l[RS_TILE];           /* old keys of the block's slots */
+  __shared__ uint16_t ucnt[RS_TILE + 1];         /* exclusive prefix of unchanged slots */
+  __shared__ uint8_t cand[RS_TILE];              /* 1: candidate cell to clear, 2: refilled */
+  __shared__ unsigned long long Ls[RS_LCAP];     /* L_b */
+  __shared__ uint32_t wcnt[RS_ITEMS * (RS_THREADS / 64)];   /* unchanged per (item row, wave) */
+  __shared__ uint32_t wpre[RS_ITEMS * (RS_THREADS / 64)];
+  __shared__ unsigned long long s_wlo, s_whi, s_upred, s_maxb, s_mina;
+  __shared__ uint32_t s_below, s_chgb, s_nl, s_n2, s_ncand;
+
   const int n = a.n;
-  const int b0 = blockIdx.x * blockDim.x;
-  const int i = b0 + threadIdx.x;
-  __shared__ uint32_t win[4];
-  __shared__ unsigned long long ends[2];
-  const bool in = i < n;
-  const uint32_t k = in ? a.keys_new[i] : 0u;
-  const bool chg = in && a.changed[i] != 0;
-  if (threadIdx.x == 0) { ends[0] = ~0ull; ends[1] = 0ull; }
-  __syncthreads();
-  if (in && !chg) {
-    atomicMin(&ends[0], comp(k, (uint32_t)i));
-    atomicMax(&ends[1], comp(k, (uint32_t)i));
+  const uint32_t C = a.C;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int s0 = blockIdx.x * RS_TILE;
+  const int cnt = min(RS_TILE, n - s0);
+  uint32_t ns = a.meta[META_APPEND + par];
+  if (ns > (uint32_t)n) {   /* each slot appends at most once: never read past the list */
+    if (tid == 0) atomicOr(a.status, ST_SORT_ERROR);
+    ns = (uint32_t)n;
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    a.meta[META_APPEND + (par ^ 1)] = 0u;     /* the next step kernel appends there */
+    a.meta[META_COUNT] = ns;
+  }
+
+  /* 1. the block's slots: old keys, changed flags, unchanged prefix.  Item
+   * it of thread tid is slot t = it * RS_THREADS + tid, so every load and
+   * store of a wave covers 64 consecutive slots. */
+  uint32_t ok[RS_ITEMS];
+  bool ch[RS_ITEMS];
+  uint64_t bal[RS_ITEMS];
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int t = it * RS_THREADS + tid;
+    ok[it] = t < cnt ? a.okeys[s0 + t] : 0xFFFFFFFFu;
+    ch[it] = t < cnt ? a.changed[s0 + t] != 0 : true;
+    if (t < cnt) okey_l[t] = ok[it];
+    cand[t] = 0;
+    bal[it] = __ballot(!ch[it]);
+    if (lane == 0) wcnt[it * (RS_THREADS / 64) + wave] = (uint32_t)__popcll(bal[it]);
+  }
+  if (tid == 0) {
+    s_below = 0u; s_chgb = 0u; s_nl = 0u; s_n2 = 0u; s_ncand = 0u;
+    s_maxb = 0ull; s_mina = COMP_INF;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int b1 = min(b0 + (int)blockDim.x, n);
-    win[0] = lb_u32(a.sslot, 0u, ns, (uint32_t)b0);
-    win[1] = lb_u32(a.sslot, win[0], ns, (uint32_t)b1);
-    win[2] = lb_u64(a.skey, 0u, ns, ends[0]);
-    win[3] = ends[1] >= ends[0] ? lb_u64(a.skey, win[2], ns, ends[1]) : win[2];
-  }
-  __syncthreads();
-  if (!in) return;
-  uint32_t pos;
-  const unsigned long long c = comp(k, (uint32_t)i);
-  if (!chg) {
-    pos = (uint32_t)i - lb_u32(a.sslot, win[0], win[1], (uint32_t)i) + lb_u64(a.skey, win[2], win[3], c);
-  } else {
-    const uint32_t q = lb_u64(a.skey, 0u, ns, c);
-    /* all slots with (oldkey, slot) < (k, i): old keys are sorted in slot order */
-    uint32_t lo = 0u, hi = (uint32_t)n;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (comp(a.okeys[mid], mid) < c) lo = mid + 1; else hi = mid;
+
+  if (wave == 3) {
+    /* exclusive scan of the 64 (item row, wave) counts, in slot order */
+    const uint32_t v = wcnt[lane];
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= o) incl += u;
     }
-    const uint32_t all_before = lo;
-    /* changed slots with (oldkey, slot) < (k, i): sslot is in slot order, so
-     * (oldkey, slot) is increasing along it too */
-    lo = 0u; hi = ns;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      const uint32_t s = a.sslot[mid];
-      if (comp(a.okeys[s], s) < c) lo = mid + 1; else hi = mid;
+    wpre[lane] = incl - v;
+    if (lane == 63) ucnt[RS_TILE] = (uint16_t)incl;
+  } else if (wave == 0) {
+    /* window ends and the last unchanged element before the block */
+    unsigned long long wlo = COMP_INF;
+    if (blockIdx.x == 0) {
+      wlo = 0ull;
+    } else {
+      const int u = wave_find_unchanged(a.changed, n, s0, 1);
+      if (u >= 0) wlo = comp(a.okeys[u], (uint32_t)u);
     }
-    pos = q + (all_before - lo);
+    if (lane == 0) s_wlo = wlo;
+  } else if (wave == 1) {
+    const int u = wave_find_unchanged(a.changed, n, s0 + cnt, 1);
+    if (lane == 0) s_whi = u >= 0 ? comp(a.okeys[u], (uint32_t)u) : COMP_INF;
+  } else if (wave == 2) {
+    const int u = s0 > 0 ? wave_find_unchanged(a.changed, n, s0 - 1, -1) : -1;
+    if (lane == 0) s_upred = u >= 0 ? comp(a.okeys[u], (uint32_t)u) + 1ull : 0ull;   /* +1; 0: none */
   }
-  if (pos >= (uint32_t)n) {   /* cannot happen for a consistent changed set; never write out of range */
-    atomicOr(a.status, ST_SORT_ERROR);
+  /* cells the changed particles may leave empty: first slot of an old run,
+   * every slot of the run changed (the run may continue past the block) */
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int t = it * RS_THREADS + tid, s = s0 + t;
+    if (t >= cnt || !ch[it] || ok[it] >= C) continue;
+    if (s > 0 && (t > 0 ? okey_l[t - 1] : a.okeys[s - 1]) == ok[it]) continue;
+    bool all = true;
+    for (int j = s + 1; j < n && a.okeys[j] == ok[it]; ++j)
+      if (a.changed[j] == 0) { all = false; break; }
+    if (all) { cand[t] = 1; atomicAdd(&s_ncand, 1u); }
+  }
+  __syncthreads();
+  uint32_t myc[RS_ITEMS];   /* unchanged slots of the block before my items */
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    myc[it] = wpre[it * (RS_THREADS / 64) + wave] + (uint32_t)__popcll(bal[it] & lt);
+    ucnt[it * RS_THREADS + tid] = (uint16_t)myc[it];
+  }
+
+  /* 2. one pass over S */
+  const unsigned long long wlo = s_wlo, whi = s_whi;
+  const uint32_t kfirst = okey_l[0], klast = okey_l[cnt - 1];
+  uint32_t below = 0u, chgb = 0u;
+  unsigned long long maxb = 0ull, mina = COMP_INF;   /* maxb holds c + 1; 0: none */
+  const bool any_cand = s_ncand != 0u;
+  /* RS_UNROLL independent loads in flight per thread */
+  for (uint32_t q0 = 0; q0 < ns; q0 += RS_THREADS * RS_UNROLL) {
+    unsigned long long cc[RS_UNROLL];
+#pragma unroll
+    for (int u = 0; u < RS_UNROLL; ++u) {
+      const uint32_t q = q0 + (uint32_t)(u * RS_THREADS + tid);
+      cc[u] = q < ns ? a.sbuf[q] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < RS_UNROLL; ++u) {
+      if (q0 + (uint32_t)(u * RS_THREADS + tid) >= ns) continue;
+      const unsigned long long c = cc[u];
+      const uint32_t k = (uint32_t)(c >> 32), s = (uint32_t)c;
+      chgb += s < (uint32_t)s0 ? 1u : 0u;
+      if (c < wlo) {
+        ++below;
+        maxb = c + 1ull > maxb ? c + 1ull : maxb;
+      } else if (c < whi) {
+        const uint32_t idx = atomicAdd(&s_nl, 1u);
+        if (idx < (uint32_t)RS_LCAP) Ls[idx] = c;
+      } else {
+        mina = c < mina ? c : mina;
+      }
+      if (any_cand && k >= kfirst && k <= klast && k < C) {
+        uint32_t lo = 0u, hi = (uint32_t)cnt;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (okey_l[mid] < k) lo = mid + 1; else hi = mid;
+        }
+        if (lo < (uint32_t)cnt && okey_l[lo] == k && cand[lo] != 0) cand[lo] = 2;
+      }
+    }
+  }
+  below = wave_sum_u32(below);
+  chgb = wave_sum_u32(chgb);
+  maxb = wave_max_u64(maxb);
+  mina = wave_min_u64(mina);
+  if (lane == 0) {
+    if (below) atomicAdd(&s_below, below);
+    if (chgb) atomicAdd(&s_chgb, chgb);
+    if (maxb) atomicMax(&s_maxb, maxb);
+    if (mina != COMP_INF) atomicMin(&s_mina, mina);
+  }
+  __syncthreads();
+
+  /* 3. sort L_b: in LDS, or in global scratch past the LDS capacity */
+  const uint32_t nl = s_nl;
+  below = s_below;
+  if (s_chgb > (uint32_t)s0 || below + nl > ns) {   /* inconsistent list (uniform: return together) */
+    if (tid == 0) atomicOr(a.status, ST_SORT_ERROR);
     return;
   }
-  a.keys_sorted[pos] = k;
-  a.idx_sorted[pos] = i;
-  a.pos_srt[pos] = a.pos_cur[i];
-  a.vel_srt[pos] = a.vel_cur[i];
-  a.omg_srt[pos] = a.omg_cur[i];
-  if (a.quat_cur) a.quat_srt[pos] = a.quat_cur[i];
+  const bool big = nl > (uint32_t)RS_LCAP;
+  unsigned long long* gl = a.skey + below;   /* the block's own range of the scratch */
+  if (!big) {
+    block_sort_u64(Ls, nl);
+  } else {
+    for (uint32_t q = tid; q < ns; q += RS_THREADS) {
+      const unsigned long long c = a.sbuf[q];
+      if (c >= wlo && c < whi) gl[atomicAdd(&s_n2, 1u)] = c;
+    }
+    __syncthreads();
+    block_sort_u64(gl, nl);
+  }
+  __syncthreads();
+
+  /* 4. positions, keys, slots and the gather */
+  const uint32_t base = (uint32_t)s0 - s_chgb + below;
+  const uint32_t nu_b = ucnt[RS_TILE];
+  const uint32_t total = nu_b + nl;
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int t = it * RS_THREADS + tid, s = s0 + t;
+    if (t >= cnt || ch[it]) continue;
+    const unsigned long long c = comp(ok[it], (uint32_t)s);
+    const uint32_t r = nl == 0u ? 0u : (big ? lower_bound_u64(gl, nl, c) : lower_bound_u64(Ls, nl, c));
+    const uint32_t pos = base + myc[it] + r;
+    if (pos >= (uint32_t)n) { atomicOr(a.status, ST_SORT_ERROR); continue; }
+    a.keys_sorted[pos] = ok[it];
+    a.idx_sorted[pos] = s;
+    a.pos_srt[pos] = a.pos_cur[s];
+    a.vel_srt[pos] = a.vel_cur[s];
+    a.omg_srt[pos] = a.omg_cur[s];
+    if (a.quat_cur) a.quat_srt[pos] = a.quat_cur[s];
+    if (a.hwc_cur) a.hwc_srt[pos] = a.hwc_cur[s];
+  }
+  for (uint32_t r = tid; r < nl; r += RS_THREADS) {
+    const unsigned long long c = big ? gl[r] : Ls[r];
+    const uint32_t k = (uint32_t)(c >> 32), s = (uint32_t)c;
+    /* #{U_b < c}: old composites increase along the block's slots */
+    uint32_t lo = 0u, hi = (uint32_t)cnt;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (comp(okey_l[mid], (uint32_t)s0 + mid) < c) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t pos = base + r + ucnt[lo];
+    if (pos >= (uint32_t)n || s >= (uint32_t)n) { atomicOr(a.status, ST_SORT_ERROR); continue; }
+    a.keys_sorted[pos] = k;
+    a.idx_sorted[pos] = (int)s;
+    a.pos_srt[pos] = a.pos_cur[s];
+    a.vel_srt[pos] = a.vel_cur[s];
+    a.omg_srt[pos] = a.omg_cur[s];
+    if (a.quat_cur) a.quat_srt[pos] = a.quat_cur[s];
+    if (a.hwc_cur) a.hwc_srt[pos] = a.hwc_cur[s];
+  }
+  __syncthreads();   /* the block's keys_sorted writes are visible to the block */
+
+  /* 5. cell bounds of the block's output range */
+  if (base + total <= (uint32_t)n) {
+    const unsigned long long pred = s_maxb > s_upred ? s_maxb : s_upred;   /* composite + 1; 0: none */
+    const unsigned long long succ = s_mina < whi ? s_mina : whi;            /* INF: none */
+    for (uint32_t q = tid; q < total; q += RS_THREADS) {
+      const uint32_t p = base + q;
+      const uint32_t k = a.keys_sorted[p];
+      if (k >= C) continue;
+      bool first, last;
+      if (q > 0) first = a.keys_sorted[p - 1] != k;
+      else first = p == 0 || pred == 0ull || (uint32_t)((pred - 1ull) >> 32) != k;
+      if (q + 1 < total) last = a.keys_sorted[p + 1] != k;
+      else last = succ == COMP_INF || (uint32_t)(succ >> 32) != k;
+      if (first) a.cells[k].x = p;
+      if (last) a.cells[k].y = p + 1;
+    }
+  }
+  /* 6. cells left empty */
+#pragma unroll
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    const int t = it * RS_THREADS + tid;
+    if (t < cnt && cand[t] == 1) a.cells[ok[it]] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+  }
 }
 
 /* Kernel_SpatialDecomp_BinData (Collision_Detection_BroadPhase.cuh:279-325):
@@ -191,17 +378,10 @@ __device__ __forceinline__ void bin_one(int n, int i, const uint32_t* __restrict
   if (i == n - 1 || keys[i + 1] != k) cells[k].y = (uint32_t)(i + 1);
 }
 
-__global__ __launch_bounds__(256) void k_bin_gated(SortArgs a) {
-  if (a.meta[META_FALLBACK] != 0u) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.n) bin_one(a.n, i, a.keys_sorted, a.cells, a.C);
-}
-
 /* ------------------------------------------------------------------------ */
 /* Full path: one cooperative kernel                                          */
 /* ------------------------------------------------------------------------ */
-__global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int gated) {
-  if (gated && a.meta[META_FALLBACK] == 0u) return;   /* uniform across the grid */
+__global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int par) {
   cg::grid_group grid = cg::this_grid();
   __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
   __shared__ uint32_t red[SORT_THREADS];
@@ -212,6 +392,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int gate
   const int wave = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
   const int m = RADIX * ntiles;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.meta[META_APPEND + (par ^ 1)] = 0u;   /* the next step kernel appends there */
+    a.meta[META_COUNT] = (uint32_t)n;
+  }
 
   const uint32_t* kin = a.keys_new;
   const int* vin = nullptr;
@@ -357,6 +541,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int gate
     a.vel_srt[i] = a.vel_cur[j];
     a.omg_srt[i] = a.omg_cur[j];
     if (a.quat_cur) a.quat_srt[i] = a.quat_cur[j];
+    if (a.hwc_cur) a.hwc_srt[i] = a.hwc_cur[j];
   }
   for (uint32_t c = blockIdx.x * SORT_THREADS + threadIdx.x; c < a.C; c += (uint32_t)gsize)
     a.cells[c] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
@@ -381,34 +566,16 @@ static int full_sort_grid() {
   return g;
 }
 
-hipError_t launch_full_sort(const SortArgs& a, bool gated, hipStream_t s) {
+hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t s) {
+  if (incremental) {
+    hipLaunchKernelGGL(k_resort, dim3((a.n + RS_TILE - 1) / RS_TILE), dim3(RS_THREADS), 0, s, a, par);
+    return hipGetLastError();
+  }
   SortArgs args = a;
-  int gflag = gated ? 1 : 0;
-  void* params[] = {&args, &gflag};
+  int p = par;
+  void* params[] = {&args, &p};
   return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_full_sort), dim3(full_sort_grid()),
                                     dim3(SORT_THREADS), params, 0, s);
-}
-
-__global__ void k_reset_append(uint32_t* meta) { meta[META_APPEND] = 0u; }
-
-hipError_t launch_sort(const SortArgs& a, bool incremental, hipStream_t s, hipEvent_t* ev) {
-  if (!incremental) {
-    /* the step kernel will append afresh */
-    hipLaunchKernelGGL(k_reset_append, dim3(1), dim3(1), 0, s, a.meta);
-    if (ev) { (void)hipEventRecord(ev[0], s); (void)hipEventRecord(ev[1], s); }
-    hipError_t e = launch_full_sort(a, false, s);
-    if (ev) (void)hipEventRecord(ev[2], s);
-    return e;
-  }
-  hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, s, a);
-  if (ev) (void)hipEventRecord(ev[0], s);
-  const dim3 g((a.n + 255) / 256);
-  hipLaunchKernelGGL(k_place, g, dim3(256), 0, s, a);
-  if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_bin_gated, g, dim3(256), 0, s, a);
-  hipError_t e = launch_full_sort(a, true, s);
-  if (ev) (void)hipEventRecord(ev[2], s);
-  return e;
 }
 
 }  // namespace bdem

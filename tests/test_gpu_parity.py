@@ -107,8 +107,9 @@ def test_step_states_match(pair, steps):
 
 
 def test_incremental_sort_matches_full_sort():
-    """After steps, the next sort runs the incremental path (partition + LDS
-    sort + merge); its order must equal the oracle's stable sort."""
+    """After steps, the next sort runs the incremental path (one resort
+    kernel over the changed list); its order must equal the oracle's stable
+    sort."""
     deck = S.random_deck(6000, seed=21, L=14.0, law="lsd", polydisperse=True, vmax=400.0)
     dev, ora = make_pair(deck)
     for rounds in range(3):
@@ -123,9 +124,56 @@ def test_incremental_sort_matches_full_sort():
     ora.close()
 
 
-def test_sort_fallback_when_many_particles_change_cell():
-    """Fast particles: more than 16384 change cell per step, so the
-    incremental path hands over to the full radix sort inside the step."""
+def test_incremental_cell_tables_bit_exact():
+    """The resort updates the cell table in place: occupied cells get their
+    new bounds, cells left empty are cleared, the rest is untouched.  The
+    table must equal the oracle's from-scratch binning after every step."""
+    deck = S.random_deck(6000, seed=23, L=14.0, law="hm", polydisperse=True, vmax=400.0)
+    dev, ora = make_pair(deck)
+    for rounds in range(4):
+        dev.step(5)
+        ora.step(5)
+        sg, eg = dev.debug_cells()
+        so, eo = ora.cells()
+        assert np.array_equal(sg, so), f"round {rounds}: cell starts differ"
+        assert np.array_equal(eg, eo), f"round {rounds}: cell ends differ"
+    assert not dev.status() & bdem.lib.WARN_SORT
+    dev.close()
+    ora.close()
+
+
+def test_resort_window_beyond_lds():
+    """Half the particles leave the grid in one step, so one block's window
+    holds ~10000 changed particles (key C): more than the LDS list, so that
+    block sorts its window in global memory.  Order and cell table must
+    still be the oracle's."""
+    deck = S.random_deck(20000, seed=24, L=20.0, radius=0.3, vmax=50.0)
+    vel = deck["vel"].copy()
+    up = deck["pos"][:, 1] > 10.0
+    vel[up, 1] = 2.0e6          # 20 cm in one 1e-5 s step: out through the top
+    deck["vel"] = vel
+    dev, ora = make_pair(deck)
+    print(f"[case] {int(up.sum())} particles leave the grid", flush=True)
+    for rounds in range(2):
+        dev.step(1)
+        ora.step(1)
+        kg, ig = dev.debug_sorted()
+        ko, io = ora.sorted()
+        assert np.array_equal(kg, ko), f"round {rounds}: keys differ"
+        assert np.array_equal(ig, io), f"round {rounds}: order differs"
+        dev.step(1)
+        ora.step(1)
+        sg, eg = dev.debug_cells()
+        so, eo = ora.cells()
+        assert np.array_equal(sg, so) and np.array_equal(eg, eo), f"round {rounds}: cells differ"
+    assert not dev.status() & bdem.lib.WARN_SORT
+    dev.close()
+    ora.close()
+
+
+def test_sort_many_particles_change_cell():
+    """Fast particles: most particles change cell every step, the changed
+    list is large and the resort still reproduces the stable sort."""
     deck = S.random_deck(40000, seed=22, L=30.0, radius=0.3, vmax=2.0e5, dt=1e-5)
     dev, ora = make_pair(deck)
     dev.step(3)
