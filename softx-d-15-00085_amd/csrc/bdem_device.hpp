@@ -234,6 +234,41 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
+/* Wave window of cell entries for stencil row r = 3 (dy + 1) + (dz + 1):
+ * [hmin + off - 1, hmax + off + 1] clipped to the table (uniform). */
+struct RowWin { uint32_t w0, wn; bool ok; };
+__device__ __forceinline__ RowWin row_window(const DevParams& P, uint32_t hmin, uint32_t hmax, int r) {
+  const int dy = r / 3 - 1, dz = r % 3 - 1;
+  const long long off = (long long)dz * P.nz + (long long)dy * P.nx * P.nz;
+  const long long wl = max((long long)hmin + off - 1, 0ll);
+  const long long wh = min((long long)hmax + off + 1, (long long)P.C - 1);
+  RowWin w;
+  w.ok = r < 9 && wl <= wh;
+  w.w0 = w.ok ? (uint32_t)wl : 0u;
+  w.wn = w.ok ? (uint32_t)(wh - wl) + 1u : 0u;
+  return w;
+}
+
+/* Candidate run of a row window from its two cell entries per lane: the
+ * start of the first occupied entry and the end of the last (by ballot). */
+struct RowStage { uint32_t pmin, pmax; bool any, staged; };
+__device__ __forceinline__ RowStage row_stage(uint2 e0, uint2 e1, uint32_t n) {
+  RowStage s{0u, 0u, false, false};
+  const uint64_t m0 = __ballot(e0.x != 0xFFFFFFFFu);
+  const uint64_t m1 = __ballot(e1.x != 0xFFFFFFFFu);
+  if ((m0 | m1) == 0ull) return s;
+  const int f = m0 ? __ffsll((unsigned long long)m0) - 1 : 64 + __ffsll((unsigned long long)m1) - 1;
+  const int l = m1 ? 127 - __clzll((long long)m1) : 63 - __clzll((long long)m0);
+  s.pmin = f < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.x, f)
+                  : (uint32_t)__builtin_amdgcn_readlane((int)e1.x, f - 64);
+  s.pmax = l < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.y, l)
+                  : (uint32_t)__builtin_amdgcn_readlane((int)e1.y, l - 64);
+  s.pmax = min(s.pmax, n);   /* table entries never exceed n; kept in range regardless */
+  s.any = s.pmin < s.pmax;
+  s.staged = s.any && s.pmax - s.pmin <= (uint32_t)STAGE_POS;
+  return s;
+}
+
 /* Candidates [lo, hi) of one lane, read from `src` (LDS stage or global). */
 template <class Visit>
 __device__ __forceinline__ void scan_candidates(const float4* src, uint32_t lo, uint32_t hi, uint32_t shift,
@@ -263,47 +298,87 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
   const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
   int count = 0;
   if (hmin <= hmax) {   /* some lane is in the grid (wave-uniform) */
-    const bool cst = hmax - hmin + 3u <= (uint32_t)STAGE_CELL;
-    for (int dy = -1; dy <= 1; ++dy) {
-      for (int dz = -1; dz <= 1; ++dz) {
-        const int y = g[1] + dy, z = g[2] + dz;
-        const bool rv0 = ingrid && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
-        const uint32_t h0 = rv0 ? hash_of(P, xs, y, z) : 0u;
-        const uint32_t h1 = rv0 ? hash_of(P, xe, y, z) : 0u;
-        const bool rv = rv0 && h1 < P.C;
-        /* wave window of cells for this row (uniform) */
-        const long long off = (long long)dz * P.nz + (long long)dy * P.nx * P.nz;
-        const long long wl = max((long long)hmin + off - 1, 0ll);
-        const long long wh = min((long long)hmax + off + 1, (long long)P.C - 1);
-        if (wl > wh) continue;
-        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-        uint32_t pmin = 0u, pmax = 0u;
-        if (cst) {
-          const uint32_t w0 = (uint32_t)wl, wn = (uint32_t)(wh - wl) + 1u;
-          const uint2 e0 = (uint32_t)lane < wn ? cells[w0 + lane] : make_uint2(0xFFFFFFFFu, 0u);
-          const uint2 e1 = (uint32_t)lane + 64u < wn ? cells[w0 + 64u + lane] : make_uint2(0xFFFFFFFFu, 0u);
-          const uint64_t m0 = __ballot(e0.x != 0xFFFFFFFFu);
-          const uint64_t m1 = __ballot(e1.x != 0xFFFFFFFFu);
-          if ((m0 | m1) == 0ull) continue;   /* no particle in the window */
-          const int f = m0 ? __ffsll((unsigned long long)m0) - 1 : 64 + __ffsll((unsigned long long)m1) - 1;
-          const int l = m1 ? 127 - __clzll((long long)m1) : 63 - __clzll((long long)m0);
-          pmin = f < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.x, f)
-                        : (uint32_t)__builtin_amdgcn_readlane((int)e1.x, f - 64);
-          pmax = l < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)e0.y, l)
-                        : (uint32_t)__builtin_amdgcn_readlane((int)e1.y, l - 64);
-          st.cell[lane] = e0;
-          st.cell[lane + 64] = e1;
-          __builtin_amdgcn_wave_barrier();
-          if (rv) {
+    if (hmax - hmin + 3u <= (uint32_t)STAGE_CELL) {
+      /* Software pipeline over the nine rows r = 3 (dy + 1) + (dz + 1): while
+       * row r is scanned from LDS, the candidates of row r + 1 and the cell
+       * entries of row r + 2 are already in flight. */
+      const uint2 empty_cell = make_uint2(0xFFFFFFFFu, 0u);
+      RowWin wc = row_window(P, hmin, hmax, 0), w1 = row_window(P, hmin, hmax, 1);
+      /* (plain ifs: a select between a load and a constant struct would go
+       * through a private copy and a flat load) */
+      uint2 ce0 = empty_cell, ce1 = empty_cell, ne0 = empty_cell, ne1 = empty_cell;
+      if (wc.ok && (uint32_t)lane < wc.wn) ce0 = cells[wc.w0 + lane];
+      if (wc.ok && (uint32_t)lane + 64u < wc.wn) ce1 = cells[wc.w0 + 64u + lane];
+      if (w1.ok && (uint32_t)lane < w1.wn) ne0 = cells[w1.w0 + lane];
+      if (w1.ok && (uint32_t)lane + 64u < w1.wn) ne1 = cells[w1.w0 + 64u + lane];
+      RowStage sc = row_stage(ce0, ce1, n);
+      float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+      if (sc.staged) {
+        if ((uint32_t)lane < sc.pmax - sc.pmin) q0 = pos_r[sc.pmin + lane];
+        if ((uint32_t)lane + 64u < sc.pmax - sc.pmin) q1 = pos_r[sc.pmin + 64u + lane];
+      }
+      for (int r = 0; r < 9; ++r) {
+        /* 1. row r into LDS */
+        st.cell[lane] = ce0;
+        st.cell[lane + 64] = ce1;
+        if (sc.staged) {
+          st.pos[lane] = q0;
+          st.pos[lane + 64] = q1;
+        }
+        __builtin_amdgcn_wave_barrier();
+        /* 2. issue row r + 1's candidates and row r + 2's cells */
+        const RowStage sn = row_stage(ne0, ne1, n);
+        float4 nq0 = make_float4(0.f, 0.f, 0.f, 0.f), nq1 = nq0;
+        if (r + 1 < 9 && sn.staged) {
+          if ((uint32_t)lane < sn.pmax - sn.pmin) nq0 = pos_r[sn.pmin + lane];
+          if ((uint32_t)lane + 64u < sn.pmax - sn.pmin) nq1 = pos_r[sn.pmin + 64u + lane];
+        }
+        const RowWin w2 = row_window(P, hmin, hmax, r + 2);
+        uint2 nn0 = empty_cell, nn1 = empty_cell;
+        if (r + 2 < 9 && w2.ok) {
+          if ((uint32_t)lane < w2.wn) nn0 = cells[w2.w0 + lane];
+          if ((uint32_t)lane + 64u < w2.wn) nn1 = cells[w2.w0 + 64u + lane];
+        }
+        /* 3. scan row r: each lane's cells, then its candidates */
+        if (wc.ok && sc.any) {
+          const int y = g[1] + (r / 3 - 1), z = g[2] + (r % 3 - 1);
+          const bool rv0 = ingrid && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
+          const uint32_t h0 = rv0 ? hash_of(P, xs, y, z) : 0u;
+          const uint32_t h1 = rv0 ? hash_of(P, xe, y, z) : 0u;
+          uint32_t lo = sc.pmin, hi = sc.pmin;
+          if (rv0 && h1 < P.C) {
+            uint32_t l2 = 0xFFFFFFFFu, h2 = 0u;
             for (uint32_t h = h0; h <= h1; ++h) {
-              const uint2 c = st.cell[h - w0];
+              const uint2 c = st.cell[h - wc.w0];
               if (c.x != 0xFFFFFFFFu) {
-                lo = min(lo, c.x);
-                hi = c.y;
+                l2 = min(l2, c.x);
+                h2 = c.y;
               }
             }
+            if (l2 != 0xFFFFFFFFu) { lo = l2; hi = min(h2, n); }
           }
-        } else {
+          /* a consistent cell table puts every lane's run inside the stage;
+           * checked, so LDS is never indexed outside it whatever the table holds */
+          if (sc.staged && lo >= sc.pmin && hi <= sc.pmax)
+            scan_candidates(st.pos, lo, hi, sc.pmin, i, pa, ra, count, visit);
+          else
+            scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+        }
+        __builtin_amdgcn_wave_barrier();
+        ce0 = ne0; ce1 = ne1; ne0 = nn0; ne1 = nn1;
+        sc = sn; q0 = nq0; q1 = nq1;
+        wc = w1; w1 = w2;
+      }
+    } else {
+      /* keys too spread for one stage: per-lane cells, shared candidates when they fit */
+      for (int dy = -1; dy <= 1; ++dy) {
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int y = g[1] + dy, z = g[2] + dz;
+          const bool rv0 = ingrid && y >= 0 && y < P.ny && z >= 0 && z < P.nz && xs <= xe;
+          const uint32_t h0 = rv0 ? hash_of(P, xs, y, z) : 0u;
+          const uint32_t h1 = rv0 ? hash_of(P, xe, y, z) : 0u;
+          const bool rv = rv0 && h1 < P.C;
+          uint32_t lo = 0xFFFFFFFFu, hi = 0u;
           if (rv) {
             for (uint32_t h = h0; h <= h1; ++h) {
               const uint2 c = cells[h];
@@ -314,25 +389,21 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
             }
           }
           const bool ne = lo != 0xFFFFFFFFu;
-          pmin = wave_min_u32(ne ? lo : 0xFFFFFFFFu);
-          pmax = wave_max_u32(ne ? hi : 0u);
+          const uint32_t pmin = wave_min_u32(ne ? lo : 0xFFFFFFFFu);
+          const uint32_t pmax = min(wave_max_u32(ne ? hi : 0u), n);
           if (pmin >= pmax) continue;
-        }
-        pmax = min(pmax, n);   /* table entries never exceed n; kept in range regardless */
-        if (pmin >= pmax) continue;
-        if (lo == 0xFFFFFFFFu) { lo = pmin; hi = pmin; }
-        hi = min(hi, n);
-        if (pmax - pmin <= (uint32_t)STAGE_POS) {
-          for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
+          if (!ne) { lo = pmin; hi = pmin; }
+          hi = min(hi, n);
+          if (pmax - pmin <= (uint32_t)STAGE_POS) {
+            for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
+            __builtin_amdgcn_wave_barrier();
+            if (lo >= pmin && hi <= pmax) scan_candidates(st.pos, lo, hi, pmin, i, pa, ra, count, visit);
+            else scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+          } else {
+            scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+          }
           __builtin_amdgcn_wave_barrier();
-          /* a consistent cell table puts every lane's run inside the stage;
-           * checked, so LDS is never indexed outside it whatever the table holds */
-          if (lo >= pmin && hi <= pmax) scan_candidates(st.pos, lo, hi, pmin, i, pa, ra, count, visit);
-          else scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
-        } else {
-          scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
         }
-        __builtin_amdgcn_wave_barrier();
       }
     }
   }

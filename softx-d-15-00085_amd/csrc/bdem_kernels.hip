@@ -142,11 +142,12 @@ __device__ __forceinline__ int particle_pair_forces(const DevParams& P, const ui
  * lane, into LDS, and every owner then adds its own results in contact
  * order -- the same additions in the same order as the per-particle loop,
  * with the wave's force body running ceil(T/64) times instead of
- * max(contacts) times.  All lanes of the wave must call it (count 0 for
- * idle lanes). */
+ * max(contacts) times.  A map from contact number to (owner lane, index)
+ * replaces a search per contact.  All lanes of the wave must call it
+ * (count 0 for idle lanes). */
 struct WaveScratch {
   float f[6][64];
-  int off[65];
+  uint16_t own[64 * CLIST];   /* contact q -> owner lane | index << 6 */
 };
 constexpr int WAVE_LDS_BYTES =
     sizeof(WaveStage) > sizeof(WaveScratch) ? (int)sizeof(WaveStage) : (int)sizeof(WaveScratch);
@@ -169,20 +170,15 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
   }
   const int excl = incl - nl;
   const int total = __shfl(incl, 63, 64);
-  ws.off[lane] = excl;
-  if (lane == 63) ws.off[64] = total;
+  for (int k = 0; k < nl; ++k) ws.own[excl + k] = (uint16_t)(lane | (k << 6));
   __builtin_amdgcn_wave_barrier();
   for (int base = 0; base < total; base += 64) {
     const int p = base + lane;
     f3 Fc = mk3(0.f, 0.f, 0.f), Tc = Fc;
     if (p < total) {
-      int lo = 0, hi = 63;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ws.off[mid] <= p) lo = mid; else hi = mid - 1;
-      }
-      const int k = p - ws.off[lo];
-      const int otid = wbase_tid + lo;
+      const uint32_t e = ws.own[p];
+      const int k = (int)(e >> 6);
+      const int otid = wbase_tid + (int)(e & 63u);
       const int oi = base_i + otid;
       const int j = (int)clist[k * STEP_THREADS + otid];
       const float4 po = pos_r[oi], vo = vel_m[oi], oo = omg_t[oi];
@@ -280,14 +276,13 @@ __global__ __launch_bounds__(256) void k_step(
     if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
 
     WallHist wh{nullptr, 0, nullptr, 0};
-    HistRec scratch_wall[1];
     if (LAW != BDEM_LAW_REFERENCE) {
       if (hist_slot) {
         wh.in = wr_in + (size_t)pid * WMAX;
         wh.in_cnt = min((int)(hwc >> 16), WMAX);
         wh.out = wr_out + (size_t)pid * WMAX;
       } else {
-        wh.out = scratch_wall; wh.out_cnt = WMAX;   /* no slot: springs are not kept */
+        wh.out = nullptr; wh.out_cnt = WMAX;   /* no slot: springs are not kept (full count: never written) */
       }
     }
     f3 Fw, Tw;
@@ -424,8 +419,6 @@ __global__ __launch_bounds__(256) void k_forces(
     F = cvt(P.gravity) * m;
     PairHistCtx hc{nullptr, 0, 0, nullptr, 0};
     WallHist wh{nullptr, 0, nullptr, 0};
-    HistRec scratch_pair[1];
-    HistRec scratch_wall[1];
     if (LAW != BDEM_LAW_REFERENCE) {
       if (pid < hist_cap) {
         const uint32_t hwc = hwc_in[i];
@@ -434,8 +427,8 @@ __global__ __launch_bounds__(256) void k_forces(
         wh.in = wr_in + (size_t)pid * WMAX; wh.in_cnt = min((int)(hwc >> 16), WMAX);
         wh.out = wr_out + (size_t)pid * WMAX;
       } else {
-        hc.out = scratch_pair; hc.out_cnt = HMAX;
-        wh.out = scratch_wall; wh.out_cnt = WMAX;
+        hc.out = nullptr; hc.out_cnt = HMAX;   /* no slot: full counts, never written */
+        wh.out = nullptr; wh.out_cnt = WMAX;
       }
     }
     particle_pair_forces<LAW>(P, cells, pos_in, vel_in, omg_in, i, p, r, v, m, w, ta, F, T, &hc, clist);
