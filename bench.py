@@ -92,8 +92,8 @@ def main():
     ap.add_argument("--settle-dt", type=float, default=2e-5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="after the timed steps, copy 256 MiB device-to-device three times (a plain "
-                         "copy kernel of known size, to scale HBM byte counters)")
+                    help="after the timed steps, run the library's plain copy kernel (k_copy_u4, "
+                         "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of settling "
@@ -192,13 +192,8 @@ def main():
                      frac=round(tot_b / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 
     if args.copy_calibration:
-        import torch
-        a = torch.empty(64 * 2**20, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("[bench] copy calibration: 3 x 256 MiB device-to-device copies")
+        dev.calibration_copy(256 * 2**20, 3)
+        log("[bench] copy calibration: 3 x k_copy_u4 over 256 MiB")
 
     # size of the incremental sort's changed list over a few more (untimed) steps
     chg = []

@@ -163,6 +163,12 @@ int bdem_timing_end(bdem_ctx* ctx, double* stage_ms, int* nsteps);
  * of the changed list for an incremental resort, N for a full sort.  No
  * reference counterpart. */
 int bdem_sort_changed(bdem_ctx* ctx, unsigned int* changed);
+/* Run a plain copy kernel (k_copy_u4: 16-byte loads and stores) over two
+ * scratch buffers of `bytes` each, `repeats` times, and wait.  A kernel of
+ * known traffic, to scale hardware byte counters (FETCH_SIZE/WRITE_SIZE).
+ * Allocates and frees its scratch; never call it in a timed region.  No
+ * reference counterpart. */
+int bdem_calibration_copy(bdem_ctx* ctx, unsigned long long bytes, int repeats);
 
 /* ---- stage outputs for parity checks (synchronous; state unchanged) -------- */
 /* Keys and identity index of the current state (Kernel_SpatialDecomp_CalcPHash). */

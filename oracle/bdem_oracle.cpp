@@ -158,6 +158,22 @@ double bo_time_steps(void* h, int nsteps) {
 
 int bo_set_active_world_objects(void* h, int n) { WITH(h, o.num_wobj_active = n); return 0; }
 int bo_set_drum_rotation(void* h, int on) { WITH(h, o.cylinder_rotation = on != 0); return 0; }
+/* Per-particle lifter contact type kept between steps (Contact_Info::cont_type),
+ * in the current (sorted) slot order.  The setter exists for tests that start
+ * a particle with a remembered contact. */
+int bo_lifter_types(void* h, int* out) {
+  WITH(h, for (int i = 0; i < o.N; ++i) out[i] = o.lifter_type[i]);
+  return 0;
+}
+int bo_set_lifter_types(void* h, const int* in) {
+  WITH(h, for (int i = 0; i < o.N; ++i) o.lifter_type[i] = in[i]);
+  return 0;
+}
+/* The lifter table (Device_Update_DObjectPostions): nd objects, 0 removes them. */
+int bo_set_dobj(void* h, const bdem_DOBJ* d, int nd) {
+  WITH(h, o.dobj.assign(d, d + (d ? nd : 0)));
+  return 0;
+}
 
 /* Bit 0: live particle outside the grid; bit 1: history overflow. */
 int bo_warnings(void* h) {

@@ -93,6 +93,7 @@ class Oracle {
   bdem_SimulationInfo sim{};
   std::vector<bdem_WOBJ> wobj;
   std::vector<bdem_POBJ> pobj;
+  std::vector<bdem_DOBJ> dobj;      /* lifters (DynamicObject[32]), host-rotated */
   bdem_ext_params ext{};
   bool has_ext = false;
   int num_wobj_active = 0;           /* Num_WorldObjects, decremented by Hatch */
@@ -108,6 +109,7 @@ class Oracle {
   std::vector<R> radius, mass, inv_inertia;
   std::vector<uint32_t> ptype;
   std::vector<int> pid;
+  std::vector<int> lifter_type;      /* Contact_Info::cont_type of the last lifter check, travels with the particle */
   /* history, indexed by P_ID */
   std::vector<int> hcount, wcount;
   std::vector<PairHist<R>> hist, whist;
@@ -116,7 +118,7 @@ class Oracle {
   std::vector<uint32_t> keys;
   std::vector<int> idx;
   std::vector<uint32_t> cell_start, cell_end;   /* reference format, 0xFFFFFFFF empty */
-  std::vector<V3<R>> f_pp, t_pp, f_w, t_w;
+  std::vector<V3<R>> f_pp, t_pp, f_w, t_w, f_l, t_l;
   int stepc = 0;
   long long steps_done = 0;
   int warn_out_of_grid = 0;
@@ -190,6 +192,7 @@ class Oracle {
     N = n;
     pos.resize(n); vel.resize(n); avel.resize(n); quat.resize(n);
     radius.resize(n); mass.resize(n); inv_inertia.resize(n); ptype.resize(n); pid.resize(n);
+    lifter_type.assign(n, 0);
     for (int i = 0; i < n; ++i) {
       pos[i] = mk<R>(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2]);
       vel[i] = v3 ? mk<R>(v3[3 * i], v3[3 * i + 1], v3[3 * i + 2]) : mk<R>(0, 0, 0);
@@ -217,7 +220,7 @@ class Oracle {
     wcount.assign(std::max(maxid, n), 0);
     hist.assign(size_t(std::max(maxid, n)) * HMAX, PairHist<R>{-1, mk<R>(0, 0, 0)});
     whist.assign(size_t(std::max(maxid, n)) * WMAX, PairHist<R>{-1, mk<R>(0, 0, 0)});
-    f_pp.assign(n, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp;
+    f_pp.assign(n, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp; f_l = f_pp; t_l = f_pp;
   }
 
   /* ------------------------------------------------------------------ */
@@ -280,6 +283,7 @@ class Oracle {
   void reorder() {
     permute(pos); permute(vel); permute(avel); permute(quat);
     permute(radius); permute(mass); permute(inv_inertia); permute(ptype); permute(pid);
+    permute(lifter_type);
   }
 
   /* a4: cudaMemset(0xFF) + Kernel_SpatialDecomp_BinData
@@ -479,18 +483,22 @@ class Oracle {
   /* ------------------------------------------------------------------ */
   struct Coll { bool hit; R dis; V3<R> cp; V3<R> n; };
 
-  /* is_PointOnSurface, Device/Utilities.cuh:20-53 */
-  bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, V3<R> point) const {
+  /* is_PointOnSurface, Device/Utilities.cuh:20-53 (vertices of a world or
+   * dynamic object) */
+  bool point_on_surface(const bdem_Surface& s, const bdem_float3* vtx, V3<R> point) const {
     R area = R(0);
     for (int i = 0; i < s.num_vertex - 1; ++i) {
-      V3<R> V1 = point - cvt<R>(w.vertex[s.vertex_Order[i]]);
-      V3<R> V2 = point - cvt<R>(w.vertex[s.vertex_Order[i + 1]]);
+      V3<R> V1 = point - cvt<R>(vtx[s.vertex_Order[i]]);
+      V3<R> V2 = point - cvt<R>(vtx[s.vertex_Order[i + 1]]);
       area = area + R(0.500000f) * length(cross(V1, V2));
     }
-    V3<R> V1 = point - cvt<R>(w.vertex[s.vertex_Order[s.num_vertex - 1]]);
-    V3<R> V2 = point - cvt<R>(w.vertex[s.vertex_Order[0]]);
+    V3<R> V1 = point - cvt<R>(vtx[s.vertex_Order[s.num_vertex - 1]]);
+    V3<R> V2 = point - cvt<R>(vtx[s.vertex_Order[0]]);
     area = area + R(0.500000f) * length(cross(V1, V2));
     return (std::fabs(area - R(s.area)) / R(s.area)) * R(100.0000f) <= R(s.Tol);
+  }
+  bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, V3<R> point) const {
+    return point_on_surface(s, w.vertex, point);
   }
 
   /* Collision_Detection_Sphere_Surface / _Plane (Collision_Detection_Spheres.cuh:6-100) */
@@ -713,11 +721,133 @@ class Oracle {
     }
   }
 
+  /* ------------------------------------------------------------------ */
+  /* a9: lifters (VolumeObject_InteractionParticle, DEM_Kernels_Compute.cuh:12-246) */
+  /* ------------------------------------------------------------------ */
+  struct DColl { bool hit; int type; R dis; V3<R> n; V3<R> cp; };
+
+  /* Collision_Detection_Sphere_DObject (Collision_Detection_Spheres.cuh:108-225):
+   * faces first unless the previous lifter contact was an edge, then the
+   * edges as half-lines from their first vertex. */
+  DColl sphere_dobject(int i, const bdem_DOBJ& D, int prev_type) const {
+    DColl c{false, 0, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    const R r = radius[i];
+    const V3<R> p = pos[i];
+    if (prev_type != 2) {
+      for (int j = 0; j < D.num_faces; ++j) {
+        const bdem_Surface& S = D.faces[j];
+        const V3<R> nrm = cvt<R>(S.normal);
+        const V3<R> Pt = p - r * nrm;
+        R dis = dot(nrm, Pt - cvt<R>(S.centroid));
+        if (dis < R(0.000000f) && dis > -r * R(0.50f)) {
+          dis = std::fabs(dis);
+          const V3<R> point = Pt + nrm * dis;
+          if (point_on_surface(S, D.vertex, point)) {
+            c.hit = true; c.type = 1; c.dis = dis; c.cp = Pt; c.n = nrm;
+            return c;
+          }
+        }
+      }
+    }
+    for (int e = 0; e < D.num_edges; ++e) {
+      const V3<R> P0 = cvt<R>(D.vertex[D.edge[e].point_vindex[0]]);
+      const V3<R> dir = cvt<R>(D.vertex[D.edge[e].point_vindex[1]]) - P0;
+      const V3<R> Edir = dir / length(dir);
+      const V3<R> vpc = p - P0;
+      const R dVec = dot(Edir, vpc);
+      if (dVec > R(0)) {
+        const V3<R> P2 = P0 + dVec * Edir;
+        const V3<R> d2 = p - P2;
+        const R d = length(d2);
+        if (d < r) {
+          const R dis = std::fabs(d - r);
+          if (dis < r) {
+            V3<R> nn = p - P2;
+            nn = nn * (R(1.0f) / length(nn));
+            c.hit = true; c.type = 2; c.dis = dis; c.n = nn; c.cp = mk<R>(0, 0, 0);
+            return c;
+          }
+        }
+      }
+    }
+    return c;
+  }
+
+  /* Collision_Response_Particle_VolumeObject (Collision_Response.cuh:241-363),
+   * lifter without spin (R_Body_COMVOBJ = VelVOBJ_Ang = 0). */
+  void resp_dobject(int i, const DColl& c, V3<R> Vl, V3<R>& Ftr, V3<R>& Tq) const {
+    const bdem_POBJ& P = pobj[ptype[i]];
+    const V3<R> Rc = c.cp - pos[i];
+    const V3<R> zero = mk<R>(0, 0, 0);
+    const V3<R> RVel = vel[i] - Vl;
+    const V3<R> Vnormal = dot(RVel, c.n) * c.n;
+    const R Fnormal = R(P.Dsurface_Kn) * c.dis;
+    const V3<R> Fdiss = R(P.Dsurface_Cn) * Vnormal;
+    Ftr = Fnormal * c.n - Fdiss;
+    const V3<R> Fdir = Ftr / length(Ftr);
+    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    const V3<R> V_CPA = vel[i] + cross(avel[i], Rc);
+    const V3<R> V_CPB = Vl + cross(zero, zero);
+    const V3<R> V_CPNA = dot(V_CPA, c.n) * c.n;
+    const V3<R> V_CPNB = dot(V_CPB, c.n) * c.n;
+    const V3<R> Vt_A = V_CPA - V_CPNA;
+    const V3<R> Vt_B = V_CPB - V_CPNB;
+    const V3<R> Vt_Rel = Vt_A - Vt_B;
+    R mag = dot(Vt_Rel, Vt_Rel);
+    V3<R> Tang = zero;
+    if (mag > R(0)) {
+      mag = rsqrt_(mag);
+      R Velf = dot(RVel, RVel);
+      Velf = Velf > R(0) ? rsqrt_(Velf) : R(1.0f);
+      const R tan_f = (R(0.050f) * (mass[i] / R(sim.InitalDelta_t))) *
+                      (R(P.Dsurface_Fric_kinetic) * (mag / Velf));
+      Tang = (R(-1.0f) * tan_f) * (Vt_Rel / mag);
+    }
+    Ftr = Ftr + Tang;
+    Tq = sim.Rotation ? cross(Rc, Ftr) : zero;
+  }
+
+  /* Lifter force and torque on particle i; returns the contact type kept
+   * for the next step (0 none, 1 face, 2 edge).  Only the first lifter in
+   * contact acts (`break`, DEM_Kernels_Compute.cuh:224). */
+  int lifter_forces(int i, V3<R>& F, V3<R>& T) const {
+    F = mk<R>(0, 0, 0);
+    T = mk<R>(0, 0, 0);
+    if (dobj.empty()) return 0;
+    const R r = radius[i];
+    const V3<R> p = pos[i];
+    DColl res{false, 0, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    const bool check = dobj[0].is_translating || !sim.isMillSim || sim.Rotating_WObject ||
+                       length(mk<R>(p.x, p.y, R(0.0)) - cvt<R>(sim.cent)) > (R(sim.DObject_CylinderBoundR) - R(2.0f) * r);
+    if (check) {
+      for (size_t d = 0; d < dobj.size(); ++d) {
+        const bdem_DOBJ& D = dobj[d];
+        const V3<R> BV = (p - cvt<R>(D.COM)) * cvt<R>(D.Axis);
+        if (length(BV) < (R(D.boundR) + r)) {
+          res = sphere_dobject(i, D, lifter_type[i]);
+          if (res.hit) {
+            if (res.type == 2) res.cp = p;   /* no spin for an edge contact */
+            V3<R> Vl = mk<R>(0, 0, 0);
+            if (D.is_attached_DWorld) Vl = drum_velocity(res.cp);
+            else if (D.is_translating) Vl = cvt<R>(D.velocity);
+            V3<R> Ftr, Tq;
+            resp_dobject(i, res, Vl, Ftr, Tq);
+            F = F + Ftr;
+            if (sim.Rotation) T = T + Tq;
+            break;
+          }
+        }
+      }
+    }
+    return res.type;
+  }
+
   /* Pair forces (including gravity, DEM_Kernels_Compute.cuh:654) and wall
    * forces of every live particle on the sorted state. */
   void compute_forces() {
-    f_pp.assign(N, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp;
+    f_pp.assign(N, mk<R>(0, 0, 0)); t_pp = f_pp; f_w = f_pp; t_w = f_pp; f_l = f_pp; t_l = f_pp;
     const bool use_hist = law_kind() != BDEM_LAW_REFERENCE;
+    std::vector<int> new_ltype(lifter_type);
     std::vector<int> hc_new, wc_new;
     std::vector<PairHist<R>> h_new, w_new;
     if (use_hist) {
@@ -745,6 +875,12 @@ class Oracle {
       wall_forces(i, Fw, Tw, &wout);
       f_w[i] = Fw;
       t_w[i] = Tw;
+      if (!dobj.empty()) {
+        V3<R> Fl, Tl;
+        new_ltype[i] = lifter_forces(i, Fl, Tl);
+        f_l[i] = Fl;
+        t_l[i] = Tl;
+      }
       if (use_hist) {
         int p = pid[i];
         hc_new[p] = int(out.size());
@@ -754,6 +890,7 @@ class Oracle {
       }
     }
     if (hov) warn_hist_overflow = 1;
+    lifter_type.swap(new_ltype);
     if (use_hist) {
       hcount.swap(hc_new); hist.swap(h_new);
       wcount.swap(wc_new); whist.swap(w_new);
@@ -767,11 +904,13 @@ class Oracle {
     for (int i = 0; i < N; ++i) {
       if (pid[i] < 0) continue;
       V3<R> acc = f_w[i] + f_pp[i];
+      if (!dobj.empty()) acc = acc + f_l[i];   /* Integration.cuh:41-44 */
       acc = acc * (R(1.0f) / mass[i]);
       vel[i] = vel[i] + acc * dt;
       pos[i] = pos[i] + vel[i] * dt;
       if (sim.Mode2D) vel[i].z = R(0);
       V3<R> fl = t_pp[i] + t_w[i];
+      if (!dobj.empty()) fl = fl + t_l[i];     /* Integration.cuh:92-95 */
       if (sim.Rotation) {
         if (dot(fl, fl) > R(0)) {
           V3<R> aa = inv_inertia[i] * fl;

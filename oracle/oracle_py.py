@@ -43,6 +43,9 @@ def load():
     lib.bo_time_steps.argtypes = [P, C.c_int]
     lib.bo_set_active_world_objects.argtypes = [P, C.c_int]
     lib.bo_set_drum_rotation.argtypes = [P, C.c_int]
+    lib.bo_set_dobj.argtypes = [P, C.c_void_p, C.c_int]
+    lib.bo_lifter_types.argtypes = [P, ip]
+    lib.bo_set_lifter_types.argtypes = [P, ip]
     lib.bo_warnings.argtypes = [P]
     lib.bo_history.argtypes = [P, C.c_int, ip, dp]
     _lib = lib
@@ -68,6 +71,8 @@ class Oracle:
                                     ctypes.byref(ext) if ext is not None else None)
         self.n = deck["sim"].Num_Particles
         self.bits = bits
+        if deck.get("dobj"):
+            self.set_dobj(deck["dobj"])
         self.set_state(deck)
 
     def set_state(self, deck):
@@ -135,6 +140,24 @@ class Oracle:
 
     def time_steps(self, n):
         return float(self.lib.bo_time_steps(self.h, int(n)))
+
+    def set_dobj(self, dobj):
+        """Lifter table (list of DOBJ), as Device_Update_DObjectPostions uploads it."""
+        arr = (dobj[0].__class__ * len(dobj))(*dobj) if dobj else None
+        self._d = arr
+        self.lib.bo_set_dobj(self.h, arr, len(dobj) if dobj else 0)
+
+    def lifter_types(self):
+        t = np.zeros(self.n, np.int32)
+        self.lib.bo_lifter_types(self.h, _p(t, C.c_int))
+        return t
+
+    def set_lifter_types(self, t):
+        t = np.ascontiguousarray(t, np.int32)
+        self.lib.bo_set_lifter_types(self.h, _p(t, C.c_int))
+
+    def set_drum_rotation(self, on):
+        self.lib.bo_set_drum_rotation(self.h, 1 if on else 0)
 
     def set_active_world_objects(self, k):
         self.lib.bo_set_active_world_objects(self.h, int(k))

@@ -54,6 +54,7 @@ _PROTOS = {
     "bdem_get_status": (C.c_int, [_P, _up]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
+    "bdem_calibration_copy": (C.c_int, [_P, C.c_ulonglong, C.c_int]),
     "bdem_num_particles": (C.c_int, [_P]),
     "bdem_num_cells": (C.c_longlong, [_P]),
     "bdem_update_dobject_positions": (C.c_int, [_P, C.POINTER(T.DOBJ)]),
@@ -204,6 +205,16 @@ class Device:
         k = C.c_uint(0)
         self._check(self.lib.bdem_sort_changed(self.h, C.byref(k)))
         return k.value
+
+    def calibration_copy(self, nbytes, repeats=3):
+        """Plain copy kernel over nbytes, `repeats` times (counter calibration)."""
+        self._check(self.lib.bdem_calibration_copy(self.h, int(nbytes), int(repeats)))
+
+    def update_dobj(self, dobj):
+        """Upload moved lifters (Device_Update_DObjectPostions)."""
+        arr = (T.DOBJ * len(dobj))(*dobj)
+        self._dobj = arr
+        self._check(self.lib.bdem_update_dobject_positions(self.h, arr))
 
     def num_cells(self):
         return int(self.lib.bdem_num_cells(self.h))

@@ -386,3 +386,176 @@ def _fcc_fill(n, d, rmax, g):
     pts = np.concatenate(out)
     pts = pts[np.lexsort((pts[:, 2], pts[:, 0], pts[:, 1]))]
     return pts[:n].astype(np.float32)
+
+
+# --------------------------------------------------------------------------
+# Rotating drum with lifters (ball-mill geometry)
+# --------------------------------------------------------------------------
+def _quad_face(s, V, order, tol):
+    """Centroid, normal and area of a quad face as ReadDObject derives them
+    (Input/KSimulationData.cpp:1706-1760): n = normalise(cross(V2, V1)),
+    area = |cross(V2,V1)|/2 + |cross(V3,V4)|/2, all in float arithmetic."""
+    f = np.float32
+    s.Tol = tol
+    s.num_vertex = len(order)
+    total = np.zeros(3, np.float32)
+    for q, o in enumerate(order):
+        s.vertex_Order[q] = o
+        total = (total + V[o]).astype(np.float32)
+    V1 = (V[order[1]] - V[order[0]]).astype(np.float32)
+    V2 = (V[order[2]] - V[order[1]]).astype(np.float32)
+    V3 = (V[order[3]] - V[order[2]]).astype(np.float32)
+    V4 = (V[order[0]] - V[order[3]]).astype(np.float32)
+    n = np.cross(V2, V1).astype(np.float32)
+    ln = f(np.sqrt(f(np.dot(n, n))))
+    n2 = np.cross(V3, V4).astype(np.float32)
+    s.area = float(f(f(f(0.5) * ln) + f(f(0.5) * f(np.sqrt(f(np.dot(n2, n2)))))))
+    s.normal = T.f3(*(n / ln).astype(np.float32))
+    s.centroid = T.f3(*((f(1.0) / f(len(order))) * total).astype(np.float32))
+
+
+def lifter_dobj(verts, com, bound_r, tol=0.1, attached=True):
+    """A rectangular lifter with the layout of Projects/3.BallMill/LifterObjects/
+    Square_Lifter16.DOBJ: 8 vertices (base y0 at the drum wall, top y1), the
+    two side faces and the top face, and the two top edges along the axis."""
+    d = T.DOBJ()
+    d.btype = 1
+    d.Axis = T.f3(1.0, 1.0, 0.0)
+    d.COM = T.f3(*com)
+    d.boundR = float(bound_r)
+    V = np.asarray(verts, np.float32)
+    d.num_vertex = 8
+    for k in range(8):
+        d.vertex[k] = T.f3(*V[k])
+    d.num_faces = 3
+    for j, order in enumerate([(0, 4, 5, 1), (2, 6, 7, 3), (6, 5, 4, 7)]):
+        _quad_face(d.faces[j], V, order, tol)
+    d.num_edges = 2
+    for k, (a, b) in enumerate([(4, 5), (6, 7)]):
+        d.edge[k].point_vindex[0] = a
+        d.edge[k].point_vindex[1] = b
+    d.is_attached_DWorld = bool(attached)
+    d.is_translating = False
+    return d
+
+
+def rotate_dobj(d, about, angle):
+    """Rotate a lifter about the z axis through `about` and re-derive its faces,
+    as Rotate_VolObject_Inc does each step (Input/Processing_Mills.h:7-83):
+    x' = cos*x + sin*y, y' = -sin*x + cos*y, in float arithmetic."""
+    f = np.float32
+    c, s = f(math.cos(angle)), f(math.sin(angle))
+    ab = np.asarray(about, np.float32)
+    V = np.array([[d.vertex[k].x, d.vertex[k].y, d.vertex[k].z] for k in range(d.num_vertex)], np.float32)
+    out = T.DOBJ.from_buffer_copy(d)
+    for k in range(d.num_vertex):
+        p = (V[k] - ab).astype(np.float32)
+        q = np.array([f(f(c * p[0]) + f(s * p[1])), f(f(-s * p[0]) + f(c * p[1])), p[2]], np.float32)
+        V[k] = (q + ab).astype(np.float32)
+        out.vertex[k] = T.f3(*V[k])
+    for j in range(d.num_faces):
+        order = [d.faces[j].vertex_Order[q] for q in range(d.faces[j].num_vertex)]
+        _quad_face(out.faces[j], V, order, d.faces[j].Tol)
+    p = (np.array([d.COM.x, d.COM.y, d.COM.z], np.float32) - ab).astype(np.float32)
+    q = np.array([f(f(c * p[0]) + f(s * p[1])), f(f(-s * p[0]) + f(c * p[1])), p[2]], np.float32)
+    out.COM = T.f3(*(q + ab).astype(np.float32))
+    return out
+
+
+def mill_deck(n=3000, seed=3, R=15.0, L=10.0, n_lifters=8, lifter_w=1.5, lifter_h=1.5, rpm=30.0,
+              law="ref", radius=0.5, polydisperse=False, rmin=0.4, rmax=0.6, density=2700.0,
+              kn=2e5, e=0.4, mu=0.6, dt=1e-5, vmax=20.0):
+    """Drum of radius R and length L (axis z, centre (R, R)) with n_lifters
+    square lifters attached to the wall, rotating at `rpm`; spheres on a
+    jittered lattice in the lower half of the drum.  Lifter l is the bottom
+    lifter rotated by 2 pi l / n_lifters about the axis (Repeat_Lifters,
+    Input/KSimulationData.cpp:1962-2028)."""
+    f = np.float32
+    rmax_ = rmax if polydisperse else radius
+    cell = 2.0 * rmax_
+    x0, x1 = R - lifter_w / 2, R + lifter_w / 2
+    base = lifter_dobj([(x0, 0, 0), (x0, 0, L), (x1, 0, L), (x1, 0, 0),
+                        (x0, lifter_h, 0), (x0, lifter_h, L), (x1, lifter_h, L), (x1, lifter_h, 0)],
+                       (R, lifter_h / 2, L / 2), max(lifter_w, lifter_h))
+    about = (R, R, 0.0)
+    dobj = [base] + [rotate_dobj(base, about, 2.0 * math.pi * k / n_lifters) for k in range(1, n_lifters)]
+    # spheres: lattice inside the drum below its centre, clear of the lifters
+    d = 2.0 * rmax_ * 1.05
+    pts = []
+    inner = R - lifter_h - rmax_ * 1.5
+    g = np.arange(R - inner + rmax_, R + inner, d)
+    zs = np.arange(rmax_ * 1.2, L - rmax_ * 1.2, d)
+    for y in np.arange(R - inner + rmax_, R + 0.8 * inner, d):
+        for x in g:
+            if (x - R) ** 2 + (y - R) ** 2 > (inner - rmax_) ** 2:
+                continue
+            for z in zs:
+                pts.append((x, y, z))
+                if len(pts) == n:
+                    break
+            if len(pts) == n:
+                break
+        if len(pts) == n:
+            break
+    if len(pts) < n:
+        raise ValueError(f"drum holds only {len(pts)} of {n} spheres on this lattice")
+    pos = np.asarray(pts, np.float64)
+    pos += uniform(seed, STREAM_POS, 3 * n, -0.02, 0.02).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+    vel[:, 2] = 0.0
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (2 * R + 1, 2 * R + 1, L + 1), cell,
+                   rotation=True, roll_res=0.0, sim_type=1)
+    sim.isMillSim = True
+    sim.Rotating_WObject = False
+    omega = rpm * 2.0 * math.pi / 60.0
+    sim.MillRotVel = float(f(-omega))
+    sim.DObject_CylinderBoundR = float(f(f(R) - f(2.0) * f(base.COM.y)))
+    sim.cent = T.f3(R, R, 0.0)
+    pobj = [sphere_pobj(radius, density, e, kn, mu)]
+    ext = None
+    rad = mass = None
+    if law != "ref" or polydisperse:
+        ext = T.ExtParams() if law == "ref" else lsd_ext(kn, e, mu)
+        if law == "hm":
+            ext = hm_ext(1e5, 0.3, e, mu)
+    if polydisperse:
+        ext.per_particle_radius = 1
+        rad = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+        mass = np.array([sphere_mass(float(r), density) for r in rad], np.float32)
+    drum = cylinder_wobj(R, L, (0.0, 0.0, 1.0), (0.0, 0.0, 0.0), top=True, bot=True)
+    deck = _deck(sim, [drum], pobj, pos, vel, ext=ext, radius=rad, mass=mass, name=f"mill{n}-{law}")
+    deck["dobj"] = dobj
+    deck["lifter_about"] = about
+    deck["rad_per_step"] = -omega * dt
+    return deck
+
+
+def place_at_lifters(deck, seed=5, overlap=0.05):
+    """Move the first spheres of a mill deck onto every lifter: one on each
+    face and one at each top edge, overlapping by `overlap`, at seeded
+    positions along the lifter.  Exercises face and edge contacts from the
+    first step."""
+    pos = deck["pos"].copy()
+    rad = deck["radius"] if deck.get("radius") is not None else np.full(len(pos), deck["pobj"][0].radius, np.float32)
+    k = 0
+    u = uniform(seed, STREAM_JITTER, 64 * len(deck["dobj"]), 0.2, 0.8)
+    for D in deck["dobj"]:
+        V = np.array([[D.vertex[q].x, D.vertex[q].y, D.vertex[q].z] for q in range(D.num_vertex)])
+        for j in range(D.num_faces):
+            F = D.faces[j]
+            n = np.array([F.normal.x, F.normal.y, F.normal.z])
+            c = np.array([F.centroid.x, F.centroid.y, F.centroid.z])
+            c[2] = V[:, 2].min() + u[k] * (V[:, 2].max() - V[:, 2].min())
+            pos[k] = c + n * (rad[k] - overlap)
+            k += 1
+        ntop = np.array([D.faces[2].normal.x, D.faces[2].normal.y, D.faces[2].normal.z])
+        for e, side in zip(range(D.num_edges), (0, 1)):
+            a, b = D.edge[e].point_vindex[0], D.edge[e].point_vindex[1]
+            ns = np.array([D.faces[side].normal.x, D.faces[side].normal.y, D.faces[side].normal.z])
+            dirn = (ntop + ns) / np.linalg.norm(ntop + ns)
+            p = V[a] + u[k] * (V[b] - V[a])
+            pos[k] = p + dirn * (rad[k] - overlap)
+            k += 1
+    out = dict(deck)
+    out["pos"] = pos.astype(np.float32)
+    return out

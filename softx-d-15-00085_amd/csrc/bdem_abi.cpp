@@ -176,6 +176,16 @@ void build_params(bdem_ctx* c) {
     for (int k = 0; k < c->ext.num_cones && k < BDEM_MAX_CONES; ++k) P.cone[k] = c->ext.cone[k];
   }
   for (size_t k = 0; k < c->wobj.size() && k < size_t(MAX_WOBJ); ++k) P.wobj[k] = c->wobj[k];
+  P.num_dobj = int(c->dobj.size() < size_t(MAX_DOBJ) ? c->dobj.size() : size_t(MAX_DOBJ));
+  for (int k = 0; k < P.num_dobj; ++k) P.dobj[k] = c->dobj[k];
+  P.mill_sim = s.isMillSim ? 1 : 0;
+  P.mill_cent = s.cent;
+  P.dobj_bound_r = s.DObject_CylinderBoundR;
+  for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
+    P.d_kn[t] = c->pobj[t].Dsurface_Kn;
+    P.d_cn[t] = c->pobj[t].Dsurface_Cn;
+    P.d_fk[t] = c->pobj[t].Dsurface_Fric_kinetic;
+  }
 }
 
 int upload_params(bdem_ctx* c) {
@@ -187,6 +197,40 @@ int upload_params(bdem_ctx* c) {
 
 bool need_history(const bdem_ctx* c) {
   return c->has_ext && c->ext.contact_law != BDEM_LAW_REFERENCE;
+}
+
+/* The per-particle word carried with the state: spring counts (history
+ * laws) and the last lifter contact type (Contact_Info::cont_type). */
+int ensure_counts(bdem_ctx* c) {
+  if (c->b.hwc_cur || (!need_history(c) && c->dobj.empty())) return BDEM_OK;
+  int rc;
+  if ((rc = dalloc(c, &c->b.hwc_cur, c->b.n)) != BDEM_OK) return rc;
+  if ((rc = dalloc(c, &c->b.hwc_srt, c->b.n)) != BDEM_OK) return rc;
+  launch_fill_u32(c->b.hwc_cur, c->b.n, 0u, c->stream);
+  launch_fill_u32(c->b.hwc_srt, c->b.n, 0u, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BDEM_OK;
+}
+
+/* DOBJ tables (KSimulationObjects.h:318-342): index ranges the kernels use. */
+int check_dobj(bdem_ctx* c, const bdem_DOBJ* d, int nd) {
+  for (int k = 0; k < nd; ++k) {
+    const bdem_DOBJ& D = d[k];
+    if (D.num_vertex < 0 || D.num_vertex > 8 || D.num_faces < 0 || D.num_faces > 3 || D.num_edges < 0 ||
+        D.num_edges > 4)
+      return fail(c, BDEM_EINVAL, "dynamic object vertex/face/edge count out of range");
+    for (int f = 0; f < D.num_faces; ++f) {
+      const bdem_Surface& S = D.faces[f];
+      if (S.num_vertex < 1 || S.num_vertex > 8) return fail(c, BDEM_EINVAL, "lifter face vertex count out of range");
+      for (int q = 0; q < S.num_vertex; ++q)
+        if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= D.num_vertex)
+          return fail(c, BDEM_EINVAL, "lifter face vertex index out of range");
+    }
+    for (int e = 0; e < D.num_edges; ++e)
+      if (D.edge[e].point_vindex[0] >= (unsigned)D.num_vertex || D.edge[e].point_vindex[1] >= (unsigned)D.num_vertex)
+        return fail(c, BDEM_EINVAL, "lifter edge vertex index out of range");
+  }
+  return BDEM_OK;
 }
 
 int require(bdem_ctx* c) {
@@ -236,6 +280,8 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     for (int i = 0; i < n; ++i)
       if (pid[i] >= c->b.hist_cap)
         return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
+  }
+  if (c->b.hwc_cur) {   /* spring counts and lifter contact types restart */
     launch_fill_u32(c->b.hwc_cur, c->b.n, 0u, c->stream);
     launch_fill_u32(c->b.hwc_srt, c->b.n, 0u, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -357,7 +403,12 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     return fail(c, BDEM_EINVAL, "num_NNCells must be positive");
   const unsigned long long C = (unsigned long long)sim->num_NNCells.x * sim->num_NNCells.y * sim->num_NNCells.z;
   if (C + 2 > 0xFFFFFFF0ull) return fail(c, BDEM_EINVAL, "grid too large for 32-bit keys");
-  if (num_dobj > 0) return fail(c, BDEM_ENOTSUP, "lifters (dynamic objects) are not implemented yet");
+  if (num_dobj < 0 || num_dobj > MAX_DOBJ) return fail(c, BDEM_EINVAL, "num_dobj must be in [0, 32]");
+  if (num_dobj > 0 && !dobj) return fail(c, BDEM_EINVAL, "null dynamic-object table");
+  if (num_dobj > 0) {
+    int rc0 = check_dobj(c, dobj, num_dobj);
+    if (rc0) return rc0;
+  }
   for (int k = 0; k < num_wobj; ++k) {
     const bdem_WOBJ& w = wobj[k];
     if (w.surface_type != BDEM_SURFACE_PLANE && w.surface_type != BDEM_SURFACE_CYLINDER)
@@ -419,6 +470,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   HIPCHK(c, hipMemset(b.pos_cur, 0, (size_t)n * sizeof(float4)));
   c->configured = true;
   if ((rc = upload_params(c)) != BDEM_OK) return rc;
+  if ((rc = ensure_counts(c)) != BDEM_OK) return rc;
 
   /* Set_Init_Config_Device (GPU_Memory_Transfer.cuh:205-271): velocity =
    * InitVel[0], angular velocity = 0.  InitVel[0] is init->velocity[0] for a
@@ -455,17 +507,14 @@ int bdem_set_ext_params(bdem_ctx* c, const bdem_ext_params* ext) {
   c->ext = *ext;
   c->has_ext = true;
   const int n = c->b.n;
-  if (need_history(c) && !c->b.hwc_cur) {
+  if (need_history(c) && !c->b.hrec[0]) {
     c->b.hist_cap = n;
-    if ((rc = dalloc(c, &c->b.hwc_cur, n)) != BDEM_OK) return rc;
-    if ((rc = dalloc(c, &c->b.hwc_srt, n)) != BDEM_OK) return rc;
     for (int h = 0; h < 2; ++h) {
       if ((rc = dalloc(c, &c->b.hrec[h], (size_t)n * HMAX)) != BDEM_OK) return rc;
       if ((rc = dalloc(c, &c->b.wrec[h], (size_t)n * WMAX)) != BDEM_OK) return rc;
     }
-    launch_fill_u32(c->b.hwc_cur, n, 0u, c->stream);
-    launch_fill_u32(c->b.hwc_srt, n, 0u, c->stream);
   }
+  if ((rc = ensure_counts(c)) != BDEM_OK) return rc;
   return upload_params(c);
 }
 
@@ -637,6 +686,25 @@ int bdem_sort_changed(bdem_ctx* c, unsigned int* changed) {
   return BDEM_OK;
 }
 
+int bdem_calibration_copy(bdem_ctx* c, unsigned long long bytes, int repeats) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (bytes == 0 || bytes % 16 != 0 || repeats < 1) return fail(c, BDEM_EINVAL, "bytes must be a positive multiple of 16");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  void *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, bytes) != hipSuccess) return fail(c, BDEM_ENOMEM, "calibration buffer");
+  if (hipMalloc(&dst, bytes) != hipSuccess) { (void)hipFree(src); return fail(c, BDEM_ENOMEM, "calibration buffer"); }
+  launch_fill_u32(static_cast<uint32_t*>(src), bytes / 4, 0x3F800000u, c->stream);
+  for (int r = 0; r < repeats; ++r)
+    launch_copy_u4(static_cast<uint4*>(dst), static_cast<const uint4*>(src), bytes / 16, c->stream);
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("calibration copy: ") + hipGetErrorString(e));
+  return BDEM_OK;
+}
+
 int bdem_num_particles(const bdem_ctx* c) { return (c && c->configured) ? c->b.n : 0; }
 long long bdem_num_cells(const bdem_ctx* c) { return (c && c->configured) ? (long long)c->hp.C : 0; }
 
@@ -644,8 +712,12 @@ int bdem_update_dobject_positions(bdem_ctx* c, const bdem_DOBJ* d) {
   int rc = require(c);
   if (rc) return rc;
   if (c->dobj.empty()) return BDEM_OK;
-  (void)d;
-  return fail(c, BDEM_ENOTSUP, "lifters are not implemented yet");
+  if (!d) return fail(c, BDEM_EINVAL, "null dynamic-object table");
+  if ((rc = check_dobj(c, d, int(c->dobj.size()))) != BDEM_OK) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dobj.assign(d, d + c->dobj.size());
+  return upload_params(c);
 }
 
 int bdem_update_world_object_positions(bdem_ctx* c, const bdem_WOBJ* w) {

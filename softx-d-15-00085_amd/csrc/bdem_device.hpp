@@ -25,6 +25,7 @@ constexpr int HMAX = 16;          /* pair tangential-history records per particl
 constexpr int WMAX = 4;           /* wall tangential-history records per particle */
 constexpr int MAX_TYPES = 8;      /* __constant__ POBJ ParticleObject[8], GPU_Memory_Transfer.cuh:81 */
 constexpr int MAX_WOBJ = 8;       /* __constant__ WOBJ WorldObject[8], GPU_Memory_Transfer.cuh:80 */
+constexpr int MAX_DOBJ = 32;      /* __constant__ DOBJ DynamicObject[32], GPU_Memory_Transfer.cuh:82 */
 
 /* Status bits raised by kernels into the context's status word. */
 enum : uint32_t {
@@ -107,6 +108,13 @@ struct DevParams {
   LawPair law_w[MAX_TYPES];
   bdem_cone cone[BDEM_MAX_CONES];
   bdem_WOBJ wobj[MAX_WOBJ];
+  /* lifters (VolumeObject_InteractionParticle) */
+  int num_dobj;
+  int mill_sim;               /* SimulationInfo::isMillSim */
+  bdem_float3 mill_cent;      /* SimulationInfo::cent */
+  float dobj_bound_r;         /* SimulationInfo::DObject_CylinderBoundR */
+  float d_kn[MAX_TYPES], d_cn[MAX_TYPES], d_fk[MAX_TYPES];   /* POBJ Dsurface_* */
+  bdem_DOBJ dobj[MAX_DOBJ];
 };
 
 /* ------------------------------------------------------------------------ */
@@ -502,18 +510,21 @@ __device__ __forceinline__ f3 history_law(const DevParams& P, f3 n, float delta,
 struct Coll { bool hit; float dis; f3 cp; f3 n; };
 
 /* is_PointOnSurface, Device/Utilities.cuh:20-53 */
-__device__ __forceinline__ bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, f3 point) {
+__device__ __forceinline__ bool point_on_surface(const bdem_Surface& s, const bdem_float3* vtx, f3 point) {
   float area = 0.0f;
   const int nv = s.num_vertex;
   for (int i = 0; i < nv - 1; ++i) {
-    const f3 V1 = point - cvt(w.vertex[s.vertex_Order[i]]);
-    const f3 V2 = point - cvt(w.vertex[s.vertex_Order[i + 1]]);
+    const f3 V1 = point - cvt(vtx[s.vertex_Order[i]]);
+    const f3 V2 = point - cvt(vtx[s.vertex_Order[i + 1]]);
     area = area + 0.500000f * length(cross(V1, V2));
   }
-  const f3 V1 = point - cvt(w.vertex[s.vertex_Order[nv - 1]]);
-  const f3 V2 = point - cvt(w.vertex[s.vertex_Order[0]]);
+  const f3 V1 = point - cvt(vtx[s.vertex_Order[nv - 1]]);
+  const f3 V2 = point - cvt(vtx[s.vertex_Order[0]]);
   area = area + 0.500000f * length(cross(V1, V2));
   return (fabsf(area - s.area) / s.area) * 100.0000f <= s.Tol;
+}
+__device__ __forceinline__ bool point_on_surface(const bdem_Surface& s, const bdem_WOBJ& w, f3 point) {
+  return point_on_surface(s, w.vertex, point);
 }
 
 /* Collision_Detection_Sphere_Surface / _Plane, Collision_Detection_Spheres.cuh:6-100 */
@@ -776,15 +787,136 @@ __device__ __forceinline__ QuatF qnormalise(QuatF a) {
   return QuatF{a.w * n, a.x * n, a.y * n, a.z * n};
 }
 
+/* ------------------------------------------------------------------------ */
+/* a9: lifters, VolumeObject_InteractionParticle (DEM_Kernels_Compute.cuh:12-246) */
+/* ------------------------------------------------------------------------ */
+struct DColl { bool hit; int type; float dis; f3 n; f3 cp; };
+
+/* Collision_Detection_Sphere_DObject, Collision_Detection_Spheres.cuh:108-225:
+ * faces first unless the previous lifter contact was an edge, then the edges
+ * as half-lines from their first vertex. */
+__device__ __forceinline__ DColl sphere_dobject(const bdem_DOBJ& D, f3 p, float r, int prev_type) {
+  DColl c{false, 0, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  if (prev_type != 2) {
+    for (int j = 0; j < D.num_faces; ++j) {
+      const bdem_Surface& S = D.faces[j];
+      const f3 nrm = cvt(S.normal);
+      const f3 Pt = p - r * nrm;
+      float dis = dot(nrm, Pt - cvt(S.centroid));
+      if (dis < 0.000000f && dis > -r * 0.50f) {
+        dis = fabsf(dis);
+        const f3 point = Pt + nrm * dis;
+        if (point_on_surface(S, D.vertex, point)) {
+          c.hit = true; c.type = 1; c.dis = dis; c.cp = Pt; c.n = nrm;
+          return c;
+        }
+      }
+    }
+  }
+  for (int e = 0; e < D.num_edges; ++e) {
+    const f3 P0 = cvt(D.vertex[D.edge[e].point_vindex[0]]);
+    const f3 dir = cvt(D.vertex[D.edge[e].point_vindex[1]]) - P0;
+    const f3 Edir = dir / length(dir);
+    const f3 vpc = p - P0;
+    const float dVec = dot(Edir, vpc);
+    if (dVec > 0.0f) {
+      const f3 P2 = P0 + dVec * Edir;
+      const f3 d2 = p - P2;
+      const float d = length(d2);
+      if (d < r) {
+        const float dis = fabsf(d - r);
+        if (dis < r) {
+          f3 nn = p - P2;
+          nn = nn * (1.0f / length(nn));
+          c.hit = true; c.type = 2; c.dis = dis; c.n = nn; c.cp = mk3(0.f, 0.f, 0.f);
+          return c;
+        }
+      }
+    }
+  }
+  return c;
+}
+
+/* Collision_Response_Particle_VolumeObject, Collision_Response.cuh:241-363,
+ * for a lifter without spin (R_Body_COMVOBJ = VelVOBJ_Ang = 0). */
+__device__ __forceinline__ void resp_dobject(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
+                                             const DColl& c, f3 Vl, f3& Ftr, f3& Tq) {
+  const f3 Rc = c.cp - p;
+  const f3 zero = mk3(0.f, 0.f, 0.f);
+  const f3 RVel = v - Vl;
+  const f3 Vnormal = dot(RVel, c.n) * c.n;
+  const float Fnormal = P.d_kn[t] * c.dis;
+  const f3 Fdiss = P.d_cn[t] * Vnormal;
+  Ftr = Fnormal * c.n - Fdiss;
+  const f3 Fdir = Ftr / length(Ftr);
+  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  const f3 V_CPA = v + cross(w, Rc);
+  const f3 V_CPB = Vl + cross(zero, zero);
+  const f3 V_CPNA = dot(V_CPA, c.n) * c.n;
+  const f3 V_CPNB = dot(V_CPB, c.n) * c.n;
+  const f3 Vt_A = V_CPA - V_CPNA;
+  const f3 Vt_B = V_CPB - V_CPNB;
+  const f3 Vt_Rel = Vt_A - Vt_B;
+  float mag = dot(Vt_Rel, Vt_Rel);
+  f3 Tang = zero;
+  if (mag > 0.0f) {
+    mag = __builtin_sqrtf(mag);
+    float Velf = dot(RVel, RVel);
+    Velf = Velf > 0.0f ? __builtin_sqrtf(Velf) : 1.0f;
+    const float tan_f = (0.050f * (m / P.dt)) * (P.d_fk[t] * (mag / Velf));
+    Tang = (-1.0f * tan_f) * (Vt_Rel / mag);
+  }
+  Ftr = Ftr + Tang;
+  Tq = P.rotation ? cross(Rc, Ftr) : zero;
+}
+
+/* Lifter force and torque on one particle; returns the contact type kept
+ * for the next step (0 none, 1 face, 2 edge).  Only the first lifter in
+ * contact acts (`break`, DEM_Kernels_Compute.cuh:224). */
+__device__ __forceinline__ int lifter_forces(const DevParams& P, int t, f3 p, float r, f3 v, f3 w,
+                                             float m, int prev_type, f3& F, f3& T) {
+  F = mk3(0.f, 0.f, 0.f);
+  T = F;
+  DColl res{false, 0, 0.0f, F, F};
+  const bool check = P.dobj[0].is_translating || !P.mill_sim || P.rotating_wobj ||
+                     length(mk3(p.x, p.y, 0.0f) - cvt(P.mill_cent)) > (P.dobj_bound_r - 2.0f * r);
+  if (check) {
+    for (int d = 0; d < P.num_dobj; ++d) {
+      const bdem_DOBJ& D = P.dobj[d];
+      const f3 BV = (p - cvt(D.COM)) * cvt(D.Axis);
+      if (length(BV) < (D.boundR + r)) {
+        res = sphere_dobject(D, p, r, prev_type);
+        if (res.hit) {
+          if (res.type == 2) res.cp = p;   /* no spin for an edge contact */
+          f3 Vl = mk3(0.f, 0.f, 0.f);
+          if (D.is_attached_DWorld) Vl = drum_velocity(P, res.cp);
+          else if (D.is_translating) Vl = cvt(D.velocity);
+          f3 Ftr, Tq;
+          resp_dobject(P, t, p, v, w, m, res, Vl, Ftr, Tq);
+          F = F + Ftr;
+          if (P.rotation) T = T + Tq;
+          break;
+        }
+      }
+    }
+  }
+  return res.type;
+}
+
+/* Integrate_Euler_NonSymmetry (Integration.cuh:6-184); the lifter terms are
+ * added only when lifters exist (Integration.cuh:41-44, 92-95). */
 __device__ __forceinline__ void integrate(const DevParams& P, float inv_inertia, float m, f3 Fpp,
-                                          f3 Tpp, f3 Fw, f3 Tw, f3& p, f3& v, f3& w, QuatF* q) {
+                                          f3 Tpp, f3 Fw, f3 Tw, f3 Fl, f3 Tl, f3& p, f3& v, f3& w,
+                                          QuatF* q) {
   const float dt = P.dt;
   f3 acc = Fw + Fpp;
+  if (P.num_dobj > 0) acc = acc + Fl;
   acc = acc * (1.0f / m);
   v = v + acc * dt;
   p = p + v * dt;
   if (P.mode2d) v.z = 0.0f;
-  const f3 fl = Tpp + Tw;
+  f3 fl = Tpp + Tw;
+  if (P.num_dobj > 0) fl = fl + Tl;
   if (P.rotation) {
     if (dot(fl, fl) > 0.0f) {
       const f3 aa = inv_inertia * fl;

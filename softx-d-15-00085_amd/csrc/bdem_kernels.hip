@@ -32,6 +32,21 @@ void launch_fill_u32(uint32_t* p, size_t count, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, count, v);
 }
 
+/* Plain 16-byte copy: the calibration kernel for hardware byte counters. */
+__global__ __launch_bounds__(256) void k_copy_u4(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                 size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) dst[i] = src[i];
+}
+
+void launch_copy_u4(uint4* dst, const uint4* src, size_t count, hipStream_t s) {
+  if (count == 0) return;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_copy_u4, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, count);
+}
+
 /* ------------------------------------------------------------------------ */
 /* a1: Kernel_SpatialDecomp_CalcPHash (Collision_Detection_BroadPhase.cuh:75-100) */
 __global__ __launch_bounds__(256) void k_hash(const DevParams* __restrict__ Pp, int n,
@@ -235,7 +250,7 @@ __global__ __launch_bounds__(256) void k_step(
   uint32_t hwc = 0u;
   if (in) {
     pr = pos_in[i]; vm = vel_in[i]; ot = omg_in[i];
-    if (LAW != BDEM_LAW_REFERENCE) hwc = hwc_in[i];
+    if (hwc_in) hwc = hwc_in[i];
   }
   int pid = in ? packed_pid(ot.w) : -1;
   const int ta = packed_type(ot.w);
@@ -279,7 +294,7 @@ __global__ __launch_bounds__(256) void k_step(
     if (LAW != BDEM_LAW_REFERENCE) {
       if (hist_slot) {
         wh.in = wr_in + (size_t)pid * WMAX;
-        wh.in_cnt = min((int)(hwc >> 16), WMAX);
+        wh.in_cnt = min((int)((hwc >> 16) & 0xFFu), WMAX);
         wh.out = wr_out + (size_t)pid * WMAX;
       } else {
         wh.out = nullptr; wh.out_cnt = WMAX;   /* no slot: springs are not kept (full count: never written) */
@@ -290,13 +305,21 @@ __global__ __launch_bounds__(256) void k_step(
     if (wall_force) {
       wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
     }
-    if (LAW != BDEM_LAW_REFERENCE)
-      hwc_out[i] = hist_slot ? (uint32_t)(nc < HMAX ? nc : HMAX) | ((uint32_t)wh.out_cnt << 16) : 0u;
+    /* a9: lifters; the contact type of the last check is kept for the next step */
+    f3 Fl = mk3(0.f, 0.f, 0.f), Tl = Fl;
+    uint32_t ltype = 0u;
+    if (P.num_dobj > 0) ltype = (uint32_t)lifter_forces(P, ta, p, r, v, w, m, (int)((hwc >> 24) & 3u), Fl, Tl);
+    if (hwc_out) {
+      uint32_t h = ltype << 24;
+      if (LAW != BDEM_LAW_REFERENCE && hist_slot)
+        h |= (uint32_t)(nc < HMAX ? nc : HMAX) | ((uint32_t)wh.out_cnt << 16);
+      hwc_out[i] = h;
+    }
 
     const float inv_inertia = P.per_particle ? 1.0f / (0.4f * r * r * m) : P.t_inertia[ta];
     QuatF q{1.f, 0.f, 0.f, 0.f};
     if (ORIENT) { const float4 q4 = quat_in[i]; q = QuatF{q4.x, q4.y, q4.z, q4.w}; }
-    integrate(P, inv_inertia, m, Fpp, Tpp, Fw, Tw, p, v, w, ORIENT ? &q : nullptr);
+    integrate(P, inv_inertia, m, Fpp, Tpp, Fw, Tw, Fl, Tl, p, v, w, ORIENT ? &q : nullptr);
 
     /* a11: Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
     if (do_kill && p.y < P.kill_height) {
@@ -314,7 +337,7 @@ __global__ __launch_bounds__(256) void k_step(
     vel_out[i] = vm;
     omg_out[i] = ot;
     if (ORIENT) quat_out[i] = quat_in[i];
-    if (LAW != BDEM_LAW_REFERENCE) hwc_out[i] = 0u;
+    if (hwc_out) hwc_out[i] = hwc;
     if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
   }
   uint32_t new_key = 0u;

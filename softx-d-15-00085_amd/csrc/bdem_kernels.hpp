@@ -102,5 +102,6 @@ void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* li
 void launch_forces(const DevParams* dP, const StepBuffers& b, int law, int hist_par, float* fpp,
                    float* tpp, float* fw, float* tw, hipStream_t s);
 void launch_fill_u32(uint32_t* p, size_t count, uint32_t v, hipStream_t s);
+void launch_copy_u4(uint4* dst, const uint4* src, size_t count, hipStream_t s);
 
 }  // namespace bdem

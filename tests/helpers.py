@@ -6,7 +6,7 @@ from oracle_py import Oracle
 
 
 def make_device(deck):
-    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"])
+    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"], dobj=deck.get("dobj"))
     if deck.get("ext") is not None:
         dev.set_ext(deck["ext"])
     if deck.get("radius") is not None:

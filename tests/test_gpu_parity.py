@@ -186,3 +186,33 @@ def test_sort_many_particles_change_cell():
     assert np.array_equal(kg, ko) and np.array_equal(ig, io)
     dev.close()
     ora.close()
+
+
+@pytest.mark.parametrize("law", ["ref", "hm"])
+def test_mill_with_lifters_matches(law):
+    """Rotating drum with eight attached lifters, spheres placed on every
+    lifter face and edge; the host rotates the lifters every step and uploads
+    them (Rotate_VolObject_Inc + Device_Update_DObjectPostions).  30 steps."""
+    deck = S.place_at_lifters(S.mill_deck(n=1800, seed=41, law=law, polydisperse=(law == "hm")))
+    print(f"[case] mill {law}", flush=True)
+    dev, ora = make_pair(deck)
+    dev.drum_rotation(True)
+    ora.set_drum_rotation(True)
+    dobj = deck["dobj"]
+    seen = set()
+    for s in range(30):
+        dev.step(1)
+        ora.step(1)
+        seen |= set(int(t) for t in ora.lifter_types())
+        dobj = [S.rotate_dobj(d, deck["lifter_about"], deck["rad_per_step"]) for d in dobj]
+        dev.update_dobj(dobj)
+        ora.set_dobj(dobj)
+    assert {1, 2} <= seen, f"lifter contact types seen: {seen}"
+    sg, so = by_pid(dev.get_state()), by_pid(ora.state())
+    for key in ("pos", "vel", "avel"):
+        err = rel_err(sg[key], so[key], 1e-3)
+        exact = float(np.mean(sg[key].astype(np.float64) == so[key]))
+        print(f"mill {law} {key}: max rel err {err:.3e}, bit-exact {exact:.4f}")
+        assert err <= TOL, f"{key}: {err}"
+    dev.close()
+    ora.close()

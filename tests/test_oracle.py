@@ -269,3 +269,104 @@ def test_discharge_kill_marks_and_sentinel():
     assert np.allclose(st["pos"][dead], [ms.x, ms.y, ms.z])
     k, _ = o.hash()
     assert np.all(k[dead] == o.num_cells() + 1)
+
+
+# --------------------------------------------------------------------------
+# a9 lifters (VolumeObject_InteractionParticle, DEM_Kernels_Compute.cuh:12-246)
+# --------------------------------------------------------------------------
+LIFT_X0, LIFT_X1, LIFT_H, LIFT_L = 9.0, 11.0, 2.0, 10.0
+
+
+def _one_sphere_lifter(pos, vel=(0.0, 0.0, 0.0), bits=64):
+    """One sphere, one horizontal lifter (top face y = LIFT_H, top edges along
+    z at x = LIFT_X0 and LIFT_X1), no gravity, not a mill (every particle is
+    checked), box walls far away."""
+    deck = S.random_deck(1, seed=2, L=20.0)
+    deck["sim"].Force_Field = T.f3(0.0, 0.0, 0.0)
+    x0, x1, h, L = LIFT_X0, LIFT_X1, LIFT_H, LIFT_L
+    deck["dobj"] = [S.lifter_dobj([(x0, 0, 0), (x0, 0, L), (x1, 0, L), (x1, 0, 0),
+                                   (x0, h, 0), (x0, h, L), (x1, h, L), (x1, h, 0)],
+                                  ((x0 + x1) / 2, h / 2, L / 2), 2.0, attached=False)]
+    deck["pos"] = np.array([pos], np.float32)
+    deck["vel"] = np.array([vel], np.float32)
+    deck["avel"] = np.zeros((1, 3), np.float32)
+    return deck, Oracle(deck, bits=bits)
+
+
+def test_lifter_face_contact_closed_form():
+    """Sphere at rest pressed onto the top face by delta: F = Dsurface_Kn * delta
+    along the face normal, no damping (v = 0), no tangential force, no torque."""
+    po = S.random_deck(1, seed=2, L=20.0)["pobj"][0]
+    r, delta = po.radius, 0.05
+    deck, o = _one_sphere_lifter((10.0, LIFT_H + r - delta, 5.0))
+    o.step(1)
+    st = o.state()
+    dt = float(np.float32(1e-5))
+    assert st["vel"][0, 1] == pytest.approx(po.Dsurface_Kn * delta / po.mass * dt, rel=1e-6)
+    assert abs(st["vel"][0, 0]) < 1e-12 and abs(st["vel"][0, 2]) < 1e-12
+    assert np.all(st["avel"][0] == 0.0)
+    assert o.lifter_types()[0] == 1
+
+
+def test_lifter_edge_contact_closed_form():
+    """Sphere diagonally off the top edge at x = LIFT_X0: neither face holds the
+    projected point, the edge does.  F = Dsurface_Kn * delta along (-1, 1, 0)/sqrt 2,
+    and no spin (contact point = centre)."""
+    po = S.random_deck(1, seed=2, L=20.0)["pobj"][0]
+    r, delta = po.radius, 0.05
+    a = (r - delta) / math.sqrt(2.0)
+    deck, o = _one_sphere_lifter((LIFT_X0 - a, LIFT_H + a, 5.0))
+    o.step(1)
+    st = o.state()
+    dt = float(np.float32(1e-5))
+    dv = po.Dsurface_Kn * delta / po.mass * dt / math.sqrt(2.0)
+    assert st["vel"][0, 0] == pytest.approx(-dv, rel=1e-5)
+    assert st["vel"][0, 1] == pytest.approx(dv, rel=1e-5)
+    assert np.all(st["avel"][0] == 0.0)
+    assert o.lifter_types()[0] == 2
+
+
+def test_lifter_edge_memory_skips_faces():
+    """Above the top face next to the edge both tests hold.  With no remembered
+    contact the face wins (normal +y, depth delta); after an edge contact the
+    faces are skipped (Collision_Detection_Spheres.cuh:121) and the edge normal
+    acts with depth r - |p - edge|."""
+    po = S.random_deck(1, seed=2, L=20.0)["pobj"][0]
+    r, delta, b = po.radius, 0.05, 0.1
+    p = (LIFT_X0 + b, LIFT_H + r - delta, 5.0)
+    dt = float(np.float32(1e-5))
+    k = po.Dsurface_Kn / po.mass * dt
+    deck, o = _one_sphere_lifter(p)
+    o.step(1)
+    v = o.state()["vel"][0]
+    assert v[1] == pytest.approx(k * delta, rel=1e-6) and abs(v[0]) < 1e-12
+    deck, o = _one_sphere_lifter(p)
+    o.set_lifter_types([2])
+    o.step(1)
+    v = o.state()["vel"][0]
+    d = math.hypot(b, r - delta)
+    assert v[0] == pytest.approx(k * (r - d) * b / d, rel=1e-5)
+    assert v[1] == pytest.approx(k * (r - d) * (r - delta) / d, rel=1e-5)
+    assert o.lifter_types()[0] == 2
+
+
+def test_lifter_attached_to_drum_moves_with_it():
+    """An attached lifter moves with the drum: a sphere resting on it in a
+    rotating mill feels tangential friction along the lifter velocity
+    cross((0,0,-MillRotVel), cp - (R,R,0)), zero for a fixed lifter."""
+    deck = S.mill_deck(n=40, seed=3)
+    deck = S.place_at_lifters(deck)
+    o_rot = Oracle(deck, bits=64)
+    fixed = dict(deck)
+    fixed["dobj"] = [S.T.DOBJ.from_buffer_copy(d) for d in deck["dobj"]]
+    for d in fixed["dobj"]:
+        d.is_attached_DWorld = False
+    o_fix = Oracle(fixed, bits=64)
+    o_rot.step(1)
+    o_fix.step(1)
+    a, b = o_rot.state(), o_fix.state()
+    va = a["vel"][np.argsort(a["pid"])]
+    vb = b["vel"][np.argsort(b["pid"])]
+    # P_ID 2 sits on lifter 0's top face: tangential push along +x
+    # (MillRotVel < 0, contact point below the drum centre)
+    assert va[2, 0] - vb[2, 0] > 0.0
