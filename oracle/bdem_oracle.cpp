@@ -196,4 +196,52 @@ int bo_history(void* h, int p, int* ids, double* xi) {
   return 0;
 }
 
+/* Polyhedra pair probe on the current slot order: the contact of slot a with
+ * slot b (Kernel_ParticleInteraction_Polyhedra_NonSymmetry's per-partner
+ * body) and the force and torque it puts on a.  out[16] = hit, class (1 VF,
+ * 2 EF, 3 FF, 4 EE, 5 SP; +8 contact-error clamp, +16 speed clamp), depth,
+ * normal, contact point, F, T. */
+int bo_poly_pair(void* h, int a, int b, double* out) {
+  WITH(h, {
+    if (!o.poly() || a < 0 || b < 0 || a >= o.N || b >= o.N) return -1;
+    const auto A = o.rotate_poly(a);
+    const auto c = o.poly_pair_contact(a, b, A);
+    auto F = c.n, T = c.n;
+    F.x = F.y = F.z = 0; T = F;
+    o.pair_force_poly(a, b, A, F, T);
+    const double v[16] = {double(c.hit), double(c.cls), double(c.dis), double(c.n.x), double(c.n.y),
+                          double(c.n.z), double(c.cp.x), double(c.cp.y), double(c.cp.z), double(F.x),
+                          double(F.y), double(F.z), double(T.x), double(T.y), double(T.z), 0.0};
+    for (int k = 0; k < 16; ++k) out[k] = v[k];
+    return 0;
+  });
+  return 0;
+}
+
+/* Get_Angular_Acc for slot i and torque tau: R(q) I0^-1 R(q)^T tau. */
+int bo_angular_acc(void* h, int i, const double* tau, double* out) {
+  WITH(h, {
+    if (!o.poly() || i < 0 || i >= o.N) return -1;
+    using RR = std::remove_reference_t<decltype(o.pos[0].x)>;
+    const auto a = o.angular_acc(i, mk<RR>(RR(tau[0]), RR(tau[1]), RR(tau[2])));
+    out[0] = double(a.x); out[1] = double(a.y); out[2] = double(a.z);
+    return 0;
+  });
+  return 0;
+}
+
+/* Polyhedra wall probe: the wall force and torque on slot i. */
+int bo_poly_walls(void* h, int i, double* out) {
+  WITH(h, {
+    if (!o.poly() || i < 0 || i >= o.N) return -1;
+    const auto A = o.rotate_poly(i);
+    auto F = A.V[0], T = A.V[0];
+    o.wall_forces_poly(i, A, F, T);
+    out[0] = double(F.x); out[1] = double(F.y); out[2] = double(F.z);
+    out[3] = double(T.x); out[4] = double(T.y); out[5] = double(T.z);
+    return 0;
+  });
+  return 0;
+}
+
 }  // extern "C"

@@ -361,13 +361,19 @@ class Oracle {
   /* Collision_Response_Particle_Particle Collision_Response.cuh:374-494 */
   /* ------------------------------------------------------------------ */
   void pair_force_ref(int a, int b, V3<R>& F, V3<R>& T) const {
-    const uint32_t ta = ptype[a], tb = ptype[b];
     V3<R> relPos = pos[a] - pos[b];
-    V3<R> RVel = vel[a] - vel[b];
     V3<R> N3 = relPos / length(relPos);
     R dis = std::fabs(length(relPos) - (radius[a] + radius[b])) + R(0.000010f);
     V3<R> RA = (R(-1.0f) * N3) * radius[a];
     V3<R> RB = N3 * radius[b];
+    pair_response(a, b, N3, dis, RA, RB, F, T);
+  }
+
+  /* Collision_Response_Particle_Particle (Collision_Response.cuh:374-494):
+   * normal N3 (towards a), depth dis, lever arms RA, RB from the centres. */
+  void pair_response(int a, int b, V3<R> N3, R dis, V3<R> RA, V3<R> RB, V3<R>& F, V3<R>& T) const {
+    const uint32_t ta = ptype[a], tb = ptype[b];
+    V3<R> RVel = vel[a] - vel[b];
     const bdem_PP_Parms& pp = pobj[ta].PP[tb];
     V3<R> Vnormal = dot(RVel, N3) * N3;
     R Fnormal = R(pp.Kn) * dis;
@@ -722,6 +728,522 @@ class Oracle {
   }
 
   /* ------------------------------------------------------------------ */
+  /* a7: polyhedra (Collision_Detection_Polyhedra.cuh, Device/Utilities.cuh) */
+  /* ------------------------------------------------------------------ */
+  bool poly() const { return sim.particle_type == 1; }
+
+  struct PolyRot { int nv, nf; V3<R> V[32]; V3<R> Fn[32]; V3<R> Fc[32]; };
+  struct PColl { bool hit; R dis; V3<R> cp; V3<R> n; int cls; };
+  enum { C_UNDEF = 0, C_VF = 1, C_EF = 2, C_FF = 3, C_EE = 4, C_SP = 5 };
+
+  V3<R> qrot(Q4<R> Q, Q4<R> QT, V3<R> v) const {
+    Q4<R> p{R(0.0f), v.x, v.y, v.z};
+    Q4<R> r = qmul(qmul(Q, p), QT);
+    return mk<R>(r.x, r.y, r.z);
+  }
+
+  /* RotatePolyhedra (Device/Utilities.cuh:420-479) */
+  PolyRot rotate_poly(int i) const {
+    PolyRot P;
+    const bdem_POBJ& po = pobj[ptype[i]];
+    P.nv = po.num_vertex;
+    P.nf = po.num_faces;
+    const V3<R> c = pos[i];
+    const Q4<R> Q = quat[i];
+    const Q4<R> QT{Q.w, -Q.x, -Q.y, -Q.z};
+    const int m = std::max(P.nf, P.nv);
+    for (int k = 0; k < m; ++k) {
+      if (sim.Rotation) {
+        if (k < P.nv) P.V[k] = qrot(Q, QT, cvt<R>(po.Local_Vertex[k])) + c;
+        if (k < P.nf) {
+          P.Fn[k] = qrot(Q, QT, cvt<R>(po.face[k].normal));
+          P.Fc[k] = qrot(Q, QT, cvt<R>(po.face[k].centroid) - cvt<R>(po.COM)) + c;
+        }
+      } else {
+        if (k < P.nv) P.V[k] = cvt<R>(po.Local_Vertex[k]) + c;
+        if (k < P.nf) {
+          P.Fn[k] = cvt<R>(po.face[k].normal);
+          P.Fc[k] = cvt<R>(po.face[k].centroid) - cvt<R>(po.COM) + c;
+        }
+      }
+    }
+    return P;
+  }
+
+  /* is_PointOnSurface over a polyhedron's rotated vertices (Device/Utilities.cuh:20-53) */
+  bool point_on_poly_face(const bdem_Surface& f, const V3<R>* V, V3<R> point, R tol) const {
+    R area = R(0.000000f);
+    for (int q = 0; q < f.num_vertex - 1; ++q) {
+      V3<R> V1 = point - V[f.vertex_Order[q]];
+      V3<R> V2 = point - V[f.vertex_Order[q + 1]];
+      area = area + R(0.500000f) * length(cross(V1, V2));
+    }
+    V3<R> V1 = point - V[f.vertex_Order[f.num_vertex - 1]];
+    V3<R> V2 = point - V[f.vertex_Order[0]];
+    area = area + R(0.500000f) * length(cross(V1, V2));
+    return (std::fabs(area - R(f.area)) / R(f.area)) * R(100.0000f) <= tol;
+  }
+
+  /* is_SP_Poly_Poly (Device/Utilities.cuh:105-156): a face of A with B's
+   * centre in front and every vertex of B in front separates them. */
+  bool sp_poly_poly(const PolyRot& A, V3<R> PosB, const PolyRot& B) const {
+    for (int f = 0; f < A.nf; ++f) {
+      if (double(dot(A.Fn[f], PosB - A.Fc[f])) > 0.0) {
+        bool contact = false;
+        for (int v = 0; v < B.nv; ++v) {
+          if (dot(A.Fn[f], B.V[v] - A.Fc[f]) <= R(0.00000f)) { contact = true; break; }
+        }
+        if (!contact) return true;
+      }
+    }
+    return false;
+  }
+
+  /* isPointinPoly (Device/Utilities.cuh:63-98) */
+  bool point_in_poly(const PolyRot& A, V3<R> point) const {
+    for (int f = 0; f < A.nf; ++f)
+      if (!(dot(A.Fn[f], point - A.Fc[f]) <= R(0.000100f))) return false;
+    return true;
+  }
+
+  struct InPoly { bool inside; int face[3]; R dist[3]; };
+  /* Collision_Detection_Vertex_InPoly_3Faces (Collision_Detection_Polyhedra.cuh:287-375) */
+  InPoly vertex_in_poly(V3<R> PosP, V3<R> v, const PolyRot& B) const {
+    InPoly r{false, {-1, -1, -1}, {R(10.0f), R(10.0f), R(10.0f)}};
+    for (int f = 0; f < B.nf; ++f) {
+      bool inside_plane = false;
+      const R vd = dot(B.Fn[f], v - B.Fc[f]);
+      if (vd < R(-0.0000100f)) {
+        inside_plane = true;
+        if (double(dot(B.Fn[f], PosP - B.Fc[f])) > 0.0) {
+          const R a = std::fabs(vd);
+          if (a < r.dist[0]) {
+            if (r.face[0] > -1) {
+              if (r.face[1] > -1) { r.dist[2] = r.dist[1]; r.face[2] = r.face[1]; }
+              r.dist[1] = r.dist[0]; r.face[1] = r.face[0];
+            }
+            r.dist[0] = a; r.face[0] = f;
+          } else if (a < r.dist[1]) {
+            if (r.face[1] > -1) { r.dist[2] = r.dist[1]; r.face[2] = r.face[1]; }
+            r.dist[1] = a; r.face[1] = f;
+          } else if (a < r.dist[2]) {
+            r.dist[2] = a; r.face[2] = f;
+          }
+        }
+      }
+      if (!inside_plane) return r;
+    }
+    r.inside = true;
+    return r;
+  }
+
+  /* The ray from PosA through `through` meets face f of B inside its polygon. */
+  bool ray_hits_face(V3<R> PosA, V3<R> through, const PolyRot& B, const bdem_POBJ& pb, int f, R tol) const {
+    V3<R> Ray = through - PosA;
+    Ray = Ray / length(Ray);
+    const R dir = dot(Ray, B.Fn[f]);
+    if (!(std::fabs(dir) > R(0))) return false;
+    const R d = dot((B.Fc[f] - PosA), B.Fn[f]) / dir;
+    return point_on_poly_face(pb.face[f], B.V, PosA + d * Ray, tol);
+  }
+
+  /* Collision_Detection_Polyhedara_Polyhedra_VertexFaces
+   * (Collision_Detection_Polyhedra.cuh:1154-1630): vertices of A inside B;
+   * one vertex -> vertex-face, two -> edge-face, more -> face-face, with a
+   * ray test choosing among the (up to) three closest faces. */
+  PColl vertex_faces(V3<R> PosA, const PolyRot& A, V3<R> PosB, const PolyRot& B, const bdem_POBJ& pb) const {
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
+    int np = 0;
+    int valid[8];
+    InPoly sel[8];
+    for (int v = 0; v < A.nv; ++v) {
+      InPoly t = vertex_in_poly(PosA, A.V[v], B);
+      if (t.inside) {
+        if (np < 8) { sel[np] = t; valid[np] = v; }
+        ++np;
+      }
+    }
+    if (np == 0) return res;
+    np = std::min(np, 8);
+    res.hit = true;
+    const InPoly& s0 = sel[0];
+    if (np == 1) {
+      res.cls = C_VF;
+      res.dis = s0.dist[0];
+      res.n = B.Fn[s0.face[0]];
+      res.cp = A.V[valid[0]];
+      if (s0.face[1] == -1) return res;
+      if (ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[0], R(0.10f))) return res;
+      if (s0.face[1] > -1 && ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[1], R(0.10f))) {
+        res.n = B.Fn[s0.face[1]];
+        res.dis = s0.dist[1];
+        return res;
+      }
+      if (s0.face[2] > -1 && ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[2], R(0.10f))) {
+        res.n = B.Fn[s0.face[2]];
+        res.dis = s0.dist[2];
+        return res;
+      }
+      return res;   /* the reference's "VF logic" fall-through keeps face 0 */
+    }
+    auto depth = [&](int f) { return std::fabs(dot(B.Fn[f], (res.cp - B.Fc[f]))); };
+    if (np == 2) {
+      res.cls = C_EF;
+      res.cp = (A.V[valid[0]] + A.V[valid[1]]) * R(0.50f);
+      res.dis = depth(s0.face[0]);
+      res.n = B.Fn[s0.face[0]];
+      if (s0.face[1] == -1 && sel[1].face[1] == -1) return res;
+      for (int q = 0; q < 3; ++q) {
+        if (q > 0 && !(s0.face[q] > -1)) continue;
+        if (ray_hits_face(PosA, res.cp, B, pb, s0.face[q], R(0.10f))) {
+          res.dis = depth(s0.face[q]);
+          res.n = B.Fn[s0.face[q]];
+          return res;
+        }
+      }
+      return res;
+    }
+    res.cls = C_FF;
+    V3<R> cp = mk<R>(R(0.0f), R(0.0f), R(0.0f));
+    for (int q = 0; q < np; ++q) cp = cp + A.V[valid[q]];
+    res.cp = cp / R(float(np));
+    const R tols[3] = {R(0.10f), R(0.010f), R(0.10f)};
+    for (int q = 0; q < 3; ++q) {
+      if (q > 0 && !(s0.face[q] > -1)) continue;
+      if (ray_hits_face(PosA, res.cp, B, pb, s0.face[q], tols[q])) {
+        res.dis = depth(s0.face[q]);
+        res.n = B.Fn[s0.face[q]];
+        return res;
+      }
+    }
+    return res;   /* "Face Logic" fall-through: normal and depth left unset */
+  }
+
+  /* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707) */
+  PColl edges(const PolyRot& A, const bdem_POBJ& pa, const PolyRot& B, const bdem_POBJ& pb, V3<R> RelPos,
+              V3<R> PosB) const {
+    PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_SP};
+    R min_edgedis = R(5.0f);
+    V3<R> selA = mk<R>(0, 0, 0), selB = selA;
+    bool found = false;
+    for (int i = 0; i < pa.num_edges; ++i) {
+      const V3<R> PA = A.V[pa.edge[i].point_vindex[0]];
+      const V3<R> DA = A.V[pa.edge[i].point_vindex[1]] - PA;
+      const R DEAD = dot(DA, DA);
+      for (int j = 0; j < pb.num_edges; ++j) {
+        const V3<R> PB = B.V[pb.edge[j].point_vindex[0]];
+        const V3<R> DB = B.V[pb.edge[j].point_vindex[1]] - PB;
+        const V3<R> dir = cross(DA, DB);
+        const R mag = dot(dir, dir);
+        if (!(mag > R(0.00000f))) continue;
+        V3<R> DV = PA - PB;
+        const R DEBD = dot(DB, DB);
+        const R DEABD = dot(DA, DB);
+        const R DEADAB = dot(DA, DV);
+        const R DEBDAB = dot(DB, DV);
+        const R detJinv = R(1.00000f) / (DEABD * DEABD - DEAD * DEBD);
+        const R dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+        if (!(dA >= R(-0.010f) && dA <= R(1.0100f))) continue;
+        const R dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+        if (!(dB >= R(-0.0100f) && dB <= R(1.010f))) continue;
+        const V3<R> pA = PA + DA * dA;
+        const V3<R> pB = PB + DB * dB;
+        DV = (pA - pB);
+        R dis = dot(DV, DV);
+        if (dis > R(0.00000f)) {
+          dis = rsqrt_(dis);
+          if (dis < min_edgedis) {
+            found = true;
+            min_edgedis = dis;
+            res.cp = pA;
+            res.n = cross(DA, DB);
+            selA = pA;
+            selB = pB;
+          }
+        }
+      }
+    }
+    if (!found) return res;
+    bool valid = true;
+    res.n = res.n * (R(1.0f) / length(res.n));
+    if (length(PosB - selB) < length(PosB - selA)) {
+      valid = false;
+      if (point_in_poly(B, selA)) valid = true;
+    }
+    if (!valid) return res;
+    res.hit = true;
+    res.cls = C_EE;
+    const R dn = dot(RelPos, res.n);
+    if (std::fabs(dn) > R(0.0f)) {
+      res.n = (dot(RelPos, res.n) / std::fabs(dot(RelPos, res.n))) * res.n;
+      res.dis = std::fabs(dot(res.n, (selB - selA))) + R(0.000010f);
+    } else {
+      res.n = RelPos / length(RelPos);
+      res.cp = PosB;
+      res.dis = std::fabs(dot(res.n, (selB - selA))) + R(0.000010f);
+    }
+    return res;
+  }
+
+  /* Kernel_ParticleInteraction_Polyhedra_NonSymmetry (DEM_Kernels_Compute.cuh:736-992):
+   * the contact of a with listed partner b (A's rotated shape given), after
+   * the reference's depth clamps. */
+  PColl poly_pair_contact(int a, int b, const PolyRot& A) const {
+    const bdem_POBJ& pa = pobj[ptype[a]];
+    const bdem_POBJ& pb = pobj[ptype[b]];
+    const V3<R> PosA = pos[a], PosB = pos[b];
+    const V3<R> RVel = vel[a] - vel[b];
+    const PolyRot B = rotate_poly(b);
+    const V3<R> RelPos = (PosA - PosB) / length(PosA - PosB);
+    bool sp = sp_poly_poly(A, PosB, B);
+    if (!sp) sp = sp_poly_poly(B, PosA, A);
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_SP};
+    if (!sp) {
+      res = vertex_faces(PosA, A, PosB, B, pb);
+      if (!res.hit) {
+        res = vertex_faces(PosB, B, PosA, A, pa);
+        res.n = res.n * R(-1.0f);   /* B into A: normal reversed */
+      }
+      if (!res.hit) {
+        res = edges(A, pa, B, pb, RelPos, PosB);
+        if (res.hit) {
+          res.cls = C_EE;
+          res.dis = std::min(R(0.025f) * R(pa.radius), res.dis);
+        }
+      }
+    }
+    if (!res.hit) return res;
+    const R rA = R(pa.radius);
+    if (res.dis > R(0.1500f) * rA) {   /* contact error: fixed small overlap along the centres */
+      res.n = RelPos / length(RelPos);
+      res.dis = R(0.025f) * rA;
+      res.cp = PosA;
+      res.cls |= 8;
+    }
+    if (res.dis > R(0.0500f) * rA) {   /* large overlap: limited by speed (first branch wins, as written) */
+      res.n = RelPos / length(RelPos);
+      const double sp_rel = double(length(RVel));
+      if (sp_rel > 50.0) res.dis = R(0.010f) * rA;
+      else if (sp_rel > 150.0) res.dis = R(0.015f) * rA;
+      else if (sp_rel > 250.0) res.dis = R(0.020f) * rA;
+      else res.dis = R(0.0030f) * rA;
+      res.cp = PosA;
+      res.cls |= 16;
+    }
+    return res;
+  }
+
+  void pair_force_poly(int a, int b, const PolyRot& A, V3<R>& F, V3<R>& T) const {
+    const PColl res = poly_pair_contact(a, b, A);
+    if (!res.hit) return;
+    pair_response(a, b, res.n, res.dis, res.cp - pos[a], res.cp - pos[b], F, T);
+  }
+
+  /* Collision_Detection_Vertcies_Surface (Collision_Detection_Polyhedra.cuh:85-185) */
+  PColl vertices_surface(const PolyRot& A, const bdem_Surface& S, const bdem_float3* svtx) const {
+    PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_UNDEF};
+    R dis_max = R(0.0);
+    int cnt = 0;
+    const V3<R> n = cvt<R>(S.normal), c = cvt<R>(S.centroid);
+    for (int v = 0; v < A.nv; ++v) {
+      const R vd = dot(n, (A.V[v] - c));
+      if (vd < R(0.000000f)) {
+        const V3<R> hit = A.V[v] + std::fabs(vd) * n;
+        if (point_on_surface(S, svtx, hit)) {
+          ++cnt;
+          res.cp = res.cp + hit;
+          if (std::fabs(vd) > dis_max) { res.hit = true; dis_max = std::fabs(vd); }
+          if (cnt > 2) {
+            res.cls = C_FF; res.n = n; res.dis = dis_max;
+            res.cp = mk<R>(R(0.0f), R(0.0f), R(0.0f));
+            return res;
+          }
+        }
+      }
+    }
+    if (!res.hit) { res.cls = C_SP; return res; }
+    res.cls = cnt == 1 ? C_VF : C_EF;
+    res.n = n;
+    res.dis = dis_max;
+    res.cp = res.cp / R(float(cnt));
+    return res;
+  }
+
+  /* Collision_Detection_Polyhedra_Surface (Collision_Detection_Polyhedra.cuh:489-529) */
+  PColl poly_surface(int i, const PolyRot& A, const bdem_Surface& S, const bdem_float3* svtx) const {
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
+    const V3<R> n = cvt<R>(S.normal), c = cvt<R>(S.centroid);
+    if (dot(n, (pos[i] - c)) > R(0.000000f)) {
+      if (dot(n, (pos[i] - c)) <= R(pobj[ptype[i]].radius)) {
+        res = vertices_surface(A, S, svtx);
+        if (res.cls == C_FF) res.cp = pos[i];   /* no moment for face-face */
+        return res;
+      }
+    }
+    return res;
+  }
+
+  /* Collision_Detection_Vertcies_Plane (Collision_Detection_Polyhedra.cuh:6-84) */
+  PColl vertices_plane(const PolyRot& A, V3<R> n, V3<R> c) const {
+    PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_SP};
+    R dis_max = R(0.0);
+    int cnt = 0;
+    for (int v = 0; v < A.nv; ++v) {
+      const R vd = dot(n, (A.V[v] - c));
+      if (vd < R(0.000000f)) {
+        ++cnt;
+        res.cp = res.cp + A.V[v];
+        if (std::fabs(vd) > dis_max) dis_max = std::fabs(vd);
+      }
+    }
+    if (cnt == 0) return res;
+    res.hit = true;
+    res.cls = cnt == 2 ? C_EF : C_FF;   /* as written: one vertex also ends as face-face */
+    res.cp = res.cp / R(float(cnt));
+    res.n = n;
+    res.dis = dis_max;
+    return res;
+  }
+
+  /* Collision_Detection_Polyhedra_Cylinder (Collision_Detection_Polyhedra.cuh:2010-2132) */
+  PColl poly_cylinder(int i, const PolyRot& A, const bdem_Macro_Cylinder& cyl) const {
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
+    const V3<R> AX = cvt<R>(cyl.AXIS), cb = cvt<R>(cyl.center_bot_cap);
+    const V3<R> Pvec = pos[i] * AX - cb;
+    const R Pvec_dis = length(Pvec) + R(pobj[ptype[i]].radius);
+    if (!(Pvec_dis > R(cyl.radius))) return res;
+    R dis = R(0.0);
+    int cnt = 0;
+    V3<R> s_point = mk<R>(R(0.0f), R(0.0f), R(0.0f));
+    auto scale = [&](R x, int k) { return R(double(x) * (1.0 / double(float(k)))); };
+    for (int v = 0; v < A.nv; ++v) {
+      const V3<R> vp = (A.V[v]) * AX - cb;
+      const R rc = length(vp);
+      if (rc > R(cyl.radius)) {
+        const R vd = rc - R(cyl.radius);
+        ++cnt;
+        s_point = s_point + A.V[v];
+        dis = dis + vd;
+        if (cnt > 2) {
+          dis = scale(dis, cnt);
+          res.hit = true;
+          res.cls = C_FF;
+          res.n = R(-1.0f) * Pvec;
+          res.n = res.n * (R(1.0f) / length(res.n));
+          res.dis = std::min(R(0.00100f), std::fabs(dis));
+          res.cp = pos[i];
+          return res;
+        }
+      }
+    }
+    if (cnt == 0) return res;
+    if (cnt == 2) {
+      dis = scale(dis, cnt);
+      res.hit = true;
+      res.cls = C_EF;
+      res.cp = s_point / R(float(cnt));
+      res.n = R(-1.0f) * Pvec;
+      res.n = res.n * (R(1.0f) / length(res.n));
+      res.dis = std::fabs(dis);
+      return res;
+    }
+    dis = scale(dis, cnt);
+    res.cp = s_point;
+    res.n = R(-1.0f) * Pvec;
+    res.n = res.n * (R(1.0f) / length(res.n));
+    res.dis = dis;
+    res.hit = true;
+    res.cls = C_VF;
+    return res;
+  }
+
+  /* WorldObject_InteractionParticle, polyhedra branches (DEM_Kernels_Compute.cuh:254-610).
+   * Infinite planes have no polyhedra method in the reference: the previous
+   * surface's result is reused there, as written. */
+  void wall_forces_poly(int i, const PolyRot& A, V3<R>& F, V3<R>& T) const {
+    F = mk<R>(0, 0, 0);
+    T = mk<R>(0, 0, 0);
+    const bool moving_planes = sim.Rotating_WObject && cylinder_rotation;
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
+    for (int k = 0; k < num_wobj_active; ++k) {
+      const bdem_WOBJ& w = wobj[k];
+      if (w.surface_type == BDEM_SURFACE_PLANE) {
+        for (int s = 0; s < w.num_surfaces; ++s) {
+          const bdem_Surface& S = w.surfaces[s];
+          if (S.area != -1.0f) res = poly_surface(i, A, S, w.vertex);
+          if (!res.hit) continue;
+          Coll c{true, res.dis, res.cp, res.n};
+          V3<R> ft = mk<R>(0, 0, 0), tq = ft;
+          if (moving_planes) ft = resp_moving(i, c, drum_velocity(res.cp), tq);
+          else resp_static(i, c, ft, tq);
+          F = F + ft;
+          if (sim.Rotation && res.cls != C_FF) T = T + tq;
+        }
+      } else if (w.surface_type == BDEM_SURFACE_CYLINDER) {
+        const bdem_Macro_Cylinder& cyl = w.cyl_geo;
+        res = poly_cylinder(i, A, cyl);
+        if (res.hit) {
+          Coll c{true, res.dis, res.cp, res.n};
+          V3<R> ft = mk<R>(0, 0, 0), tq = ft;
+          if (cylinder_rotation) ft = resp_moving(i, c, drum_velocity(res.cp), tq);
+          else resp_static(i, c, ft, tq);
+          F = ft;
+          if (sim.Rotation && res.cls != C_FF) T = T + tq;
+        }
+        if (cyl.has_top_cap) {
+          res = vertices_plane(A, cvt<R>(cyl.normal_top_cap), cvt<R>(cyl.center_top_cap));
+          if (res.hit) F = resp_endplate(i, Coll{true, res.dis, res.cp, res.n});
+        }
+        if (cyl.has_bot_cap) {
+          res = vertices_plane(A, cvt<R>(cyl.normal_bot_cap), cvt<R>(cyl.center_bot_cap));
+          if (res.hit) F = resp_endplate(i, Coll{true, res.dis, res.cp, res.n});
+        }
+      }
+    }
+  }
+
+  /* Get_Angular_Acc (Device/Utilities.cuh:486-608).  IC = RQ I0 (:511-524),
+   * then IN[i][j] = sum_k IC[k][j] RQT[i][k] (:546-560), which is RQ^T RQ I0:
+   * the rotation cancels up to rounding, so the body-frame tensor I0 acts on
+   * the world-frame torque, as written. */
+  V3<R> angular_acc(int i, V3<R> tau) const {
+    const Q4<R> Q = quat[i];
+    const float* I0 = pobj[ptype[i]].InertiaT;
+    R RQ[9];
+    RQ[0] = (Q.w * Q.w + Q.x * Q.x - Q.y * Q.y - Q.z * Q.z);
+    RQ[1] = (R(2.0f) * Q.x * Q.y - R(2.0f) * Q.w * Q.z);
+    RQ[2] = (R(2.0f) * Q.x * Q.z + R(2.0f) * Q.w * Q.y);
+    RQ[3] = (R(2.0f) * Q.x * Q.y + R(2.0f) * Q.w * Q.z);
+    RQ[4] = (Q.w * Q.w - Q.x * Q.x + Q.y * Q.y - Q.z * Q.z);
+    RQ[5] = (R(2.0f) * Q.y * Q.z - R(2.0f) * Q.w * Q.x);
+    RQ[6] = (R(2.0f) * Q.x * Q.z - R(2.0f) * Q.w * Q.y);
+    RQ[7] = (R(2.0f) * Q.y * Q.z + R(2.0f) * Q.w * Q.x);
+    RQ[8] = (Q.w * Q.w - Q.x * Q.x - Q.y * Q.y + Q.z * Q.z);
+    R IC[9], IN[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        R sum = R(0);
+        for (int k = 0; k < 3; ++k) sum += (RQ[r * 3 + k] * R(I0[k * 3 + c]));
+        IC[r * 3 + c] = sum;
+      }
+    const R RQT[9] = {RQ[0], RQ[3], RQ[6], RQ[1], RQ[4], RQ[7], RQ[2], RQ[5], RQ[8]};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        R sum = R(0);
+        for (int k = 0; k < 3; ++k) sum += ((IC[k * 3 + c]) * RQT[r * 3 + k]);
+        IN[r * 3 + c] = sum;
+      }
+    const R f[3] = {tau.x, tau.y, tau.z};
+    R out[3];
+    for (int r = 0; r < 3; ++r) {
+      R sum = R(0);
+      for (int k = 0; k < 3; ++k) sum += (IN[r * 3 + k] * (f[k]));
+      out[r] = sum;
+    }
+    return mk<R>(out[0], out[1], out[2]);
+  }
+
+  /* ------------------------------------------------------------------ */
   /* a9: lifters (VolumeObject_InteractionParticle, DEM_Kernels_Compute.cuh:12-246) */
   /* ------------------------------------------------------------------ */
   struct DColl { bool hit; int type; R dis; V3<R> n; V3<R> cp; };
@@ -857,6 +1379,28 @@ class Oracle {
       w_new.assign(whist.size(), PairHist<R>{-1, mk<R>(0, 0, 0)});
     }
     int hov = 0;
+    if (poly()) {
+      int lov = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : lov)
+      for (int i = 0; i < N; ++i) {
+        if (pid[i] < 0) continue;
+        V3<R> F = cvt<R>(sim.Force_Field) * mass[i];
+        V3<R> T = mk<R>(0, 0, 0);
+        const PolyRot A = rotate_poly(i);
+        int n = for_each_contact(i, [&](int j, int k) {
+          if (k < LIST_STRIDE) pair_force_poly(i, j, A, F, T);   /* the 32-slot list */
+        });
+        if (n > LIST_STRIDE) lov = 1;
+        f_pp[i] = F;
+        t_pp[i] = sim.Rotation ? T : mk<R>(0, 0, 0);
+        V3<R> Fw, Tw;
+        wall_forces_poly(i, A, Fw, Tw);
+        f_w[i] = Fw;
+        t_w[i] = Tw;
+      }
+      if (lov) warn_list_overflow = 1;
+      return;
+    }
 #pragma omp parallel for schedule(static) reduction(| : hov)
     for (int i = 0; i < N; ++i) {
       if (pid[i] < 0) continue;
@@ -913,11 +1457,11 @@ class Oracle {
       if (!dobj.empty()) fl = fl + t_l[i];     /* Integration.cuh:92-95 */
       if (sim.Rotation) {
         if (dot(fl, fl) > R(0)) {
-          V3<R> aa = inv_inertia[i] * fl;
+          V3<R> aa = poly() ? angular_acc(i, fl) : inv_inertia[i] * fl;
           avel[i] = avel[i] + aa * dt;
           avel[i] = avel[i] - avel[i] * R(sim.Roll_Res);
         }
-        if (sim.sphere_orient) {
+        if (poly() || sim.sphere_orient) {
           V3<R> om = avel[i];
           R mo = dot(om, om);
           Q4<R> dQ{1, 0, 0, 0};

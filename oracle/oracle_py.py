@@ -48,6 +48,9 @@ def load():
     lib.bo_set_lifter_types.argtypes = [P, ip]
     lib.bo_warnings.argtypes = [P]
     lib.bo_history.argtypes = [P, C.c_int, ip, dp]
+    lib.bo_poly_pair.argtypes = [P, C.c_int, C.c_int, dp]
+    lib.bo_angular_acc.argtypes = [P, C.c_int, dp, dp]
+    lib.bo_poly_walls.argtypes = [P, C.c_int, dp]
     _lib = lib
     return lib
 
@@ -169,3 +172,25 @@ class Oracle:
         ids = np.zeros(16, np.int32); xi = np.zeros((16, 3))
         c = self.lib.bo_history(self.h, int(pid), _p(ids, C.c_int), _p(xi, C.c_double))
         return ids[:max(c, 0)], xi[:max(c, 0)]
+
+    def poly_pair(self, a, b):
+        """Contact of slot a with slot b and the force/torque on a (fresh
+        handles: slots are input order until the first stage runs)."""
+        out = np.zeros(16)
+        if self.lib.bo_poly_pair(self.h, int(a), int(b), _p(out, C.c_double)) != 0:
+            raise ValueError("poly_pair needs a polyhedra deck and valid slots")
+        return dict(hit=bool(out[0]), cls=int(out[1]), dis=out[2], n=out[3:6], cp=out[6:9],
+                    F=out[9:12], T=out[12:15])
+
+    def angular_acc(self, i, tau):
+        t = np.ascontiguousarray(tau, np.float64)
+        out = np.zeros(3)
+        if self.lib.bo_angular_acc(self.h, int(i), _p(t, C.c_double), _p(out, C.c_double)) != 0:
+            raise ValueError("angular_acc needs a polyhedra deck")
+        return out
+
+    def poly_walls(self, i):
+        out = np.zeros(6)
+        if self.lib.bo_poly_walls(self.h, int(i), _p(out, C.c_double)) != 0:
+            raise ValueError("poly_walls needs a polyhedra deck")
+        return out[:3], out[3:]

@@ -559,3 +559,256 @@ def place_at_lifters(deck, seed=5, overlap=0.05):
     out = dict(deck)
     out["pos"] = pos.astype(np.float32)
     return out
+
+
+# --------------------------------------------------------------------------
+# Polyhedra (particle_type 1)
+# --------------------------------------------------------------------------
+CUBE_VERTS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], np.float64) * 0.5
+# outward counter-clockwise quads (0-based)
+CUBE_FACES = [[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [2, 3, 7, 6], [1, 2, 6, 5], [0, 4, 7, 3]]
+
+
+def _edges_of(faces):
+    seen, out = set(), []
+    for f in faces:
+        for q in range(len(f)):
+            a, b = f[q], f[(q + 1) % len(f)]
+            key = (min(a, b), max(a, b))
+            if key not in seen:
+                seen.add(key)
+                out.append((a, b))
+    return out
+
+
+def _tet_inertia(a, b, c):
+    """Inertia tensor (about the origin, unit density) and volume of the
+    tetrahedron (0, a, b, c), from the covariance of a simplex."""
+    vol = np.dot(a, np.cross(b, c)) / 6.0
+    S = np.zeros((3, 3))
+    P = [a, b, c]
+    for x in P:
+        S += np.outer(x, x)
+    s = a + b + c
+    S += np.outer(s, s)
+    cov = vol / 20.0 * S
+    return vol, np.trace(cov) * np.eye(3) - cov
+
+
+def poly_pobj(verts, faces, density, cor, kn, mu, cor_s=None, kn_s=None, mu_s=None, tol=0.1):
+    """A polyhedron POBJ as ReadPObject_Poly + Make_Particle_LocalInfo build
+    it (Input/KSimulationData.cpp:1180-1530): COM = vertex mean, Local_Vertex
+    = vertex - COM, bounding radius = largest |Local_Vertex|, face centroid =
+    vertex mean of the face, face normal = normalise(cross(v1-v2, v2-v0)) for
+    triangles and normalise(cross(v2-v3, v3-v0)) for polygons, area = the
+    centroid fan, all in float arithmetic.  `faces` are 0-based outward CCW
+    vertex orders.  Mass = density * volume * 1e-6 and the stored tensor is
+    the inverse of the inertia tensor about the COM (kg cm^2); the reference
+    reads a normalised tensor from its deck, so the tensor here is input data.
+    Triangles get their own area (the reference's reader leaves the previous
+    face's value there)."""
+    f = np.float32
+    V = np.asarray(verts, np.float32)
+    nv = len(V)
+    assert nv <= 32 and len(faces) <= 32
+    p = T.POBJ()
+    p.num_vertex = nv
+    total = np.zeros(3, np.float32)
+    for k in range(nv):
+        p.vertex[k] = T.f3(*V[k])
+        total = (total + V[k]).astype(np.float32)
+    com = (total * f(f(1.0) / f(nv))).astype(np.float32)
+    p.COM = T.f3(*com)
+    loc = (V - com).astype(np.float32)
+    rad = f(0.0)
+    for k in range(nv):
+        p.Local_Vertex[k] = T.f3(*loc[k])
+        ln = f(np.sqrt(f(np.dot(loc[k], loc[k]))))
+        rad = max(rad, ln)
+    p.radius = float(rad)
+    p.num_faces = len(faces)
+    vol = 0.0
+    Itot = np.zeros((3, 3))
+    cd = V.astype(np.float64) - com.astype(np.float64)
+    for j, order in enumerate(faces):
+        s = p.face[j]
+        s.num_vertex = len(order)
+        s.Tol = tol
+        tot = np.zeros(3, np.float32)
+        for q, o in enumerate(order):
+            s.vertex_Order[q] = o
+            tot = (tot + V[o]).astype(np.float32)
+        c = ((f(1.0) / f(len(order))) * tot).astype(np.float32)
+        s.centroid = T.f3(*c)
+        if len(order) == 3:
+            n = np.cross((V[order[1]] - V[order[2]]).astype(np.float32),
+                         (V[order[2]] - V[order[0]]).astype(np.float32)).astype(np.float32)
+        else:
+            n = np.cross((V[order[2]] - V[order[3]]).astype(np.float32),
+                         (V[order[3]] - V[order[0]]).astype(np.float32)).astype(np.float32)
+        ln = f(np.sqrt(f(np.dot(n, n))))
+        s.normal = T.f3(*(n / ln).astype(np.float32))
+        area = f(0.0)
+        for q in range(len(order)):
+            a = (V[order[q]] - c).astype(np.float32)
+            b = (V[order[(q + 1) % len(order)]] - c).astype(np.float32)
+            x = np.cross(a, b).astype(np.float32)
+            area = f(area + f(f(0.5) * f(np.sqrt(f(np.dot(x, x))))))
+        s.area = float(area)
+        cc = c.astype(np.float64) - com.astype(np.float64)
+        for q in range(len(order)):
+            dv, Iv = _tet_inertia(cd[order[q]], cd[order[(q + 1) % len(order)]], cc)
+            vol += dv
+            Itot += Iv
+    edges = _edges_of(faces)
+    assert len(edges) <= 32
+    p.num_edges = len(edges)
+    for k, (a, b) in enumerate(edges):
+        p.edge[k].point_vindex[0] = a
+        p.edge[k].point_vindex[1] = b
+    p.mass = float(f(density * vol * 1e-6))
+    Iinv = np.linalg.inv(Itot * density * 1e-6)
+    for g in range(9):
+        p.InertiaT[g] = float(f(Iinv.flat[g]))
+    m = float(f(p.mass))
+    kn_s = kn if kn_s is None else kn_s
+    cor_s = cor if cor_s is None else cor_s
+    mu_s = mu if mu_s is None else mu_s
+    p.PP[0].Kn = kn
+    p.PP[0].Cn = cn_from_cor(float(f(kn)), 1.0 / (1.0 / m + 1.0 / m), cor)
+    p.PP[0].Fric_static = mu
+    p.PP[0].Fric_kinetic = mu
+    p.surface_Kn = kn_s
+    p.surface_Cn = cn_from_cor(float(f(kn_s)), m, cor_s)
+    p.surface_Fric_static = mu_s
+    p.surface_Fric_kinetic = mu_s
+    p.Dsurface_Kn = kn_s
+    p.Dsurface_Cn = p.surface_Cn
+    p.Dsurface_Fric_static = mu_s
+    p.Dsurface_Fric_kinetic = mu_s
+    return p
+
+
+def cross_pair_params(pobj, kn, e, mu):
+    """PP[tb] of every type for every partner type, as Set_SpringConstants_KN
+    derives them (Input/KSimulationData.cpp:950-1045): reduced mass m/2 for
+    the same type, mA mB/(mA+mB) across types."""
+    for a, pa in enumerate(pobj):
+        for b, pb in enumerate(pobj):
+            ma, mb = float(np.float32(pa.mass)), float(np.float32(pb.mass))
+            pa.PP[b].Kn = kn
+            pa.PP[b].Cn = cn_from_cor(float(np.float32(kn)), 1.0 / (1.0 / ma + 1.0 / mb), e)
+            pa.PP[b].Fric_static = mu
+            pa.PP[b].Fric_kinetic = mu
+
+
+def hull_shape(seed, nv=12, aspect=(1.0, 0.8, 0.6), size=1.0):
+    """A convex polyhedron: the hull of nv seeded points on an ellipsoid,
+    with outward CCW triangles (nv <= 12 keeps E = 3nv-6 <= 32)."""
+    from scipy.spatial import ConvexHull
+    u = uniform(seed, STREAM_SHAPE, 2 * nv, 0.0, 1.0).reshape(nv, 2)
+    th = np.arccos(1.0 - 2.0 * u[:, 0])
+    ph = 2.0 * math.pi * u[:, 1]
+    P = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    P = P * np.asarray(aspect) * (0.5 * size)
+    h = ConvexHull(P)
+    keep = sorted(set(h.vertices.tolist()))
+    remap = {v: k for k, v in enumerate(keep)}
+    V = P[keep]
+    faces = []
+    for simp, eq in zip(h.simplices, h.equations):
+        a, b, c = (P[s] for s in simp)
+        if np.dot(np.cross(b - a, c - a), eq[:3]) < 0:
+            simp = simp[[0, 2, 1]]
+        faces.append([remap[s] for s in simp])
+    return V, faces
+
+
+def random_quats(seed, n):
+    """Uniform random unit quaternions (w, x, y, z) (Shoemake's method)."""
+    u = uniform(seed, STREAM_SHAPE + 16, 3 * n, 0.0, 1.0).reshape(n, 3).astype(np.float64)
+    a, b = np.sqrt(1 - u[:, 0]), np.sqrt(u[:, 0])
+    t1, t2 = 2 * math.pi * u[:, 1], 2 * math.pi * u[:, 2]
+    q = np.stack([b * np.cos(t2), a * np.sin(t1), a * np.cos(t1), b * np.sin(t2)], 1)
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(np.float32)
+
+
+def axis_quat(axis, deg):
+    a = np.asarray(axis, np.float64)
+    a = a / np.linalg.norm(a)
+    h = math.radians(deg) * 0.5
+    return np.array([math.cos(h), *(math.sin(h) * a)], np.float32)
+
+
+def tilt_quats(seed, n, max_deg):
+    """Rotations by a seeded angle in [0, max_deg] about seeded axes."""
+    u = uniform(seed, STREAM_SHAPE + 24, 4 * n, 0.0, 1.0).reshape(n, 4).astype(np.float64)
+    ax = np.stack([np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.cos(2 * math.pi * u[:, 1]),
+                   np.sqrt(1 - (2 * u[:, 0] - 1) ** 2) * np.sin(2 * math.pi * u[:, 1]),
+                   2 * u[:, 0] - 1], 1)
+    h = np.radians(max_deg) * u[:, 2] * 0.5
+    return np.concatenate([np.cos(h)[:, None], np.sin(h)[:, None] * ax], 1).astype(np.float32)
+
+
+def poly_deck(n, seed=31, L=12.0, shape="cube", size=1.0, density=2500.0, kn=1e5, e=0.6, mu=0.4,
+              dt=1e-5, vmax=20.0, wmax=10.0, spacing=None, overlap=0.04, rotation=True, tilt=None,
+              jitter=0.05):
+    """Polyhedra (particle_type 1) on a jittered lattice in a closed box,
+    random orientations, velocities and spins: neighbours' bounding spheres
+    overlap, so face, edge and vertex contacts of every kind occur.
+    shape = "cube" (8 V, 6 F, 12 E), "hull" (12-vertex convex hull, 20 F,
+    30 E), "mixed" (cube and hull) or "cubes2" (cubes of side 1 and 1.4);
+    with two types they alternate as a 3-D checkerboard.  tilt = None draws uniform
+    orientations; a number keeps every body within that many degrees of the
+    lattice axes, so that faces meet faces."""
+    if shape == "cube":
+        shapes = [(CUBE_VERTS * size, CUBE_FACES)]
+    elif shape == "hull":
+        shapes = [hull_shape(seed, size=size)]
+    elif shape == "cubes2":
+        shapes = [(CUBE_VERTS * size, CUBE_FACES), (CUBE_VERTS * (1.4 * size), CUBE_FACES)]
+    else:
+        shapes = [(CUBE_VERTS * size, CUBE_FACES), hull_shape(seed, size=size)]
+    pobj = [poly_pobj(V, F, density, e, kn, mu) for V, F in shapes]
+    cross_pair_params(pobj, kn, e, mu)
+    rmax = max(p.radius for p in pobj)
+    d = spacing if spacing is not None else 2.0 * rmax * (1.0 - overlap) * 0.6
+    m = int(math.floor((L - 2 * rmax - 0.2) / d)) + 1
+    assert m ** 3 >= n, "box too small for the lattice"
+    gi = np.indices((m, m, m)).reshape(3, -1).T[:n].astype(np.float64)
+    pos = rmax + 0.1 + gi * d
+    pos += uniform(seed, STREAM_JITTER, 3 * n, -jitter * d, jitter * d).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+    avel = uniform(seed, STREAM_VEL + 8, 3 * n, -wmax, wmax).reshape(n, 3)
+    cell = 2.0 * rmax
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (L, L, L), cell,
+                   rotation=rotation, roll_res=0.0)
+    sim.particle_type = 1
+    sim.Num_ParticleObjects = len(pobj)
+    ptype = (gi.sum(axis=1).astype(np.int64) % len(pobj)).astype(np.uint32)
+    out = _deck(sim, [box_walls(L)], pobj, pos, vel, ptype=ptype, name=f"poly{n}-{shape}")
+    out["avel"] = avel.astype(np.float32)
+    out["quat"] = random_quats(seed, n) if tilt is None else tilt_quats(seed, n, tilt)
+    return out
+
+
+def poly_pair_deck(pa, pb, qa, qb, shape_a=(CUBE_VERTS, CUBE_FACES), shape_b=None, va=(0, 0, 0),
+                   vb=(0, 0, 0), wa=(0, 0, 0), wb=(0, 0, 0), kn=1e5, e=0.6, mu=0.4, L=12.0, gravity=True):
+    """Two polyhedra in an otherwise empty box (type 0 = A's shape, type 1 =
+    B's shape when given), for closed-form contact checks."""
+    shapes = [shape_a] if shape_b is None else [shape_a, shape_b]
+    pobj = [poly_pobj(V, F, 2500.0, e, kn, mu) for V, F in shapes]
+    cross_pair_params(pobj, kn, e, mu)
+    rmax = max(p.radius for p in pobj)
+    sim = sim_info(2, 1e-5, (0.0, -981.0 if gravity else 0.0, 0.0), (-1.0, -1.0, -1.0), (L, L, L),
+                   2.0 * rmax, rotation=True, roll_res=0.0)
+    sim.particle_type = 1
+    sim.Num_ParticleObjects = len(pobj)
+    ptype = np.array([0, 0 if shape_b is None else 1], np.uint32)
+    d = _deck(sim, [box_walls(L)], pobj, np.array([pa, pb], np.float64), np.array([va, vb], np.float64),
+              ptype=ptype, name="poly-pair")
+    d["avel"] = np.array([wa, wb], np.float32)
+    d["quat"] = np.array([qa, qb], np.float32)
+    return d
