@@ -837,8 +837,16 @@ class Oracle {
     return r;
   }
 
+  /* Face f of B; f = -1 (no face had A's centre in front: A's centre is
+   * inside B) reads as a zero plane.  The reference indexes its face array
+   * with -1 there, which is undefined; a zero normal makes the ray tests fail
+   * and the depth 0, the same on every path of this build. */
+  V3<R> fnorm(const PolyRot& B, int f) const { return f >= 0 ? B.Fn[f] : mk<R>(R(0), R(0), R(0)); }
+  V3<R> fcent(const PolyRot& B, int f) const { return f >= 0 ? B.Fc[f] : mk<R>(R(0), R(0), R(0)); }
+
   /* The ray from PosA through `through` meets face f of B inside its polygon. */
   bool ray_hits_face(V3<R> PosA, V3<R> through, const PolyRot& B, const bdem_POBJ& pb, int f, R tol) const {
+    if (f < 0) return false;
     V3<R> Ray = through - PosA;
     Ray = Ray / length(Ray);
     const R dir = dot(Ray, B.Fn[f]);
@@ -870,34 +878,34 @@ class Oracle {
     if (np == 1) {
       res.cls = C_VF;
       res.dis = s0.dist[0];
-      res.n = B.Fn[s0.face[0]];
+      res.n = fnorm(B, s0.face[0]);
       res.cp = A.V[valid[0]];
       if (s0.face[1] == -1) return res;
       if (ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[0], R(0.10f))) return res;
       if (s0.face[1] > -1 && ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[1], R(0.10f))) {
-        res.n = B.Fn[s0.face[1]];
+        res.n = fnorm(B, s0.face[1]);
         res.dis = s0.dist[1];
         return res;
       }
       if (s0.face[2] > -1 && ray_hits_face(PosA, A.V[valid[0]], B, pb, s0.face[2], R(0.10f))) {
-        res.n = B.Fn[s0.face[2]];
+        res.n = fnorm(B, s0.face[2]);
         res.dis = s0.dist[2];
         return res;
       }
       return res;   /* the reference's "VF logic" fall-through keeps face 0 */
     }
-    auto depth = [&](int f) { return std::fabs(dot(B.Fn[f], (res.cp - B.Fc[f]))); };
+    auto depth = [&](int f) { return std::fabs(dot(fnorm(B, f), (res.cp - fcent(B, f)))); };
     if (np == 2) {
       res.cls = C_EF;
       res.cp = (A.V[valid[0]] + A.V[valid[1]]) * R(0.50f);
       res.dis = depth(s0.face[0]);
-      res.n = B.Fn[s0.face[0]];
+      res.n = fnorm(B, s0.face[0]);
       if (s0.face[1] == -1 && sel[1].face[1] == -1) return res;
       for (int q = 0; q < 3; ++q) {
         if (q > 0 && !(s0.face[q] > -1)) continue;
         if (ray_hits_face(PosA, res.cp, B, pb, s0.face[q], R(0.10f))) {
           res.dis = depth(s0.face[q]);
-          res.n = B.Fn[s0.face[q]];
+          res.n = fnorm(B, s0.face[q]);
           return res;
         }
       }
@@ -912,7 +920,7 @@ class Oracle {
       if (q > 0 && !(s0.face[q] > -1)) continue;
       if (ray_hits_face(PosA, res.cp, B, pb, s0.face[q], tols[q])) {
         res.dis = depth(s0.face[q]);
-        res.n = B.Fn[s0.face[q]];
+        res.n = fnorm(B, s0.face[q]);
         return res;
       }
     }

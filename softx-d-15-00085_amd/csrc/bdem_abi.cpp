@@ -40,6 +40,9 @@ struct bdem_ctx {
   StepBuffers b{};
   std::vector<void*> allocs;
   bool orient = false;
+  bool poly = false;         /* particle_type 1 */
+  bdem_POBJ* dpobj = nullptr;  /* device copy of the POBJ table (polyhedra) */
+  PolyBuffers pb{};
   bool keys_valid = false;   /* keys_new holds the keys of the current state */
   bool flags_valid = false;  /* ... and the step kernel's changed flags for them */
   int hist_par = 0;
@@ -88,6 +91,8 @@ void free_all(bdem_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->b = StepBuffers{};
+  c->pb = PolyBuffers{};
+  c->dpobj = nullptr;
   c->dp = nullptr;
   c->configured = false;
 }
@@ -143,6 +148,7 @@ void build_params(bdem_ctx* c) {
   P.rotation = s.Rotation ? 1 : 0;
   P.mode2d = s.Mode2D ? 1 : 0;
   P.sphere_orient = s.sphere_orient ? 1 : 0;
+  P.poly = s.particle_type == 1 ? 1 : 0;
   P.law = c->has_ext ? c->ext.contact_law : BDEM_LAW_REFERENCE;
   P.per_particle = (c->has_ext && c->ext.per_particle_radius) ? 1 : 0;
   P.num_wobj = c->num_wobj_active;
@@ -191,6 +197,9 @@ void build_params(bdem_ctx* c) {
 int upload_params(bdem_ctx* c) {
   build_params(c);
   HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
+  if (c->dpobj)
+    HIPCHK(c, hipMemcpyAsync(c->dpobj, c->pobj.data(), c->pobj.size() * sizeof(bdem_POBJ),
+                             hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BDEM_OK;
 }
@@ -229,6 +238,30 @@ int check_dobj(bdem_ctx* c, const bdem_DOBJ* d, int nd) {
     for (int e = 0; e < D.num_edges; ++e)
       if (D.edge[e].point_vindex[0] >= (unsigned)D.num_vertex || D.edge[e].point_vindex[1] >= (unsigned)D.num_vertex)
         return fail(c, BDEM_EINVAL, "lifter edge vertex index out of range");
+  }
+  return BDEM_OK;
+}
+
+/* POBJ polyhedron tables (KSimulationObjects.h:272-300): the index ranges
+ * the polyhedra kernels read. */
+int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
+  for (int t = 0; t < np; ++t) {
+    const bdem_POBJ& P = p[t];
+    if (P.num_vertex < 1 || P.num_vertex > 32 || P.num_faces < 1 || P.num_faces > 32 || P.num_edges < 0 ||
+        P.num_edges > 32)
+      return fail(c, BDEM_EINVAL, "polyhedron vertex/face/edge count out of range (type " + std::to_string(t) + ")");
+    if (!(P.radius > 0.0f) || !(P.mass > 0.0f))
+      return fail(c, BDEM_EINVAL, "polyhedron radius and mass must be positive");
+    for (int f = 0; f < P.num_faces; ++f) {
+      const bdem_Surface& S = P.face[f];
+      if (S.num_vertex < 1 || S.num_vertex > 8) return fail(c, BDEM_EINVAL, "polyhedron face vertex count out of range");
+      for (int q = 0; q < S.num_vertex; ++q)
+        if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= P.num_vertex)
+          return fail(c, BDEM_EINVAL, "polyhedron face vertex index out of range");
+    }
+    for (int e = 0; e < P.num_edges; ++e)
+      if (P.edge[e].point_vindex[0] >= (unsigned)P.num_vertex || P.edge[e].point_vindex[1] >= (unsigned)P.num_vertex)
+        return fail(c, BDEM_EINVAL, "polyhedron edge vertex index out of range");
   }
   return BDEM_OK;
 }
@@ -393,7 +426,8 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
                              const bdem_DOBJ* dobj, const bdem_SimulationInfo* sim, int num_wobj,
                              int num_pobj, int num_dobj, const bdem_InitConfig* init) {
   if (!c || !sim || !pobj || (num_wobj > 0 && !wobj)) return c ? fail(c, BDEM_EINVAL, "null argument") : BDEM_EINVAL;
-  if (sim->particle_type != 0) return fail(c, BDEM_ENOTSUP, "polyhedra (particle_type 1) are not implemented yet");
+  if (sim->particle_type != 0 && sim->particle_type != 1)
+    return fail(c, BDEM_EINVAL, "particle_type must be 0 (spheres) or 1 (polyhedra)");
   if (sim->Integration_Type != 0) return fail(c, BDEM_ENOTSUP, "only symplectic Euler (Integration_Type 0) is live in the reference");
   if (sim->use_symmetry) return fail(c, BDEM_ENOTSUP, "the symmetric path is disabled in the reference (main.cpp:811)");
   if (num_wobj < 0 || num_wobj > MAX_WOBJ) return fail(c, BDEM_EINVAL, "num_wobj must be in [0, 8]");
@@ -407,6 +441,14 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   if (num_dobj > 0 && !dobj) return fail(c, BDEM_EINVAL, "null dynamic-object table");
   if (num_dobj > 0) {
     int rc0 = check_dobj(c, dobj, num_dobj);
+    if (rc0) return rc0;
+  }
+  if (sim->particle_type == 1) {
+    if (num_dobj > 0)
+      return fail(c, BDEM_ENOTSUP, "lifters with polyhedra (Collision_Detection_Polyhedra.cuh:716-890) are not implemented");
+    if (sim->use_hist)
+      return fail(c, BDEM_ENOTSUP, "polyhedra with contact history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) are not implemented");
+    int rc0 = check_poly(c, pobj, num_pobj);
     if (rc0) return rc0;
   }
   for (int k = 0; k < num_wobj; ++k) {
@@ -437,7 +479,8 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   c->num_wobj_active = num_wobj;
   c->has_ext = false;
   std::memset(&c->ext, 0, sizeof(c->ext));
-  c->orient = sim->sphere_orient;
+  c->poly = sim->particle_type == 1;
+  c->orient = sim->sphere_orient || c->poly;
   c->stepc = 0;
   c->hist_par = 0;
 
@@ -462,6 +505,23 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(b.wall_force, (size_t)n * 3);
+  if (c->poly) {
+    PolyBuffers& pb = c->pb;
+    int pv = 0, pf = 0;
+    for (int t = 0; t < num_pobj; ++t) {
+      pv = pobj[t].num_vertex > pv ? pobj[t].num_vertex : pv;
+      pf = pobj[t].num_faces > pf ? pobj[t].num_faces : pf;
+    }
+    pb.L = PolyLayout{pv, pf, pv + 2 * pf};
+    ALLOC(c->dpobj, (size_t)num_pobj);
+    ALLOC(pb.rot, (size_t)n * pb.L.stride);
+    ALLOC(pb.list, (size_t)n * LIST_STRIDE);
+    ALLOC(pb.nn, (size_t)n);
+    ALLOC(pb.queue, (size_t)n * LIST_STRIDE);
+    ALLOC(pb.qcount, 1);
+    ALLOC(pb.pf, (size_t)n * LIST_STRIDE);
+    ALLOC(pb.pt, (size_t)n * LIST_STRIDE);
+  }
 #undef ALLOC
   HIPCHK(c, hipMemset(b.status, 0, sizeof(uint32_t)));
   HIPCHK(c, hipMemset(b.kill_count, 0, sizeof(int)));
@@ -502,6 +562,8 @@ int bdem_set_ext_params(bdem_ctx* c, const bdem_ext_params* ext) {
   if (!ext) return fail(c, BDEM_EINVAL, "null ext");
   if (ext->contact_law < 0 || ext->contact_law > 2) return fail(c, BDEM_EINVAL, "unknown contact law");
   if (ext->num_cones < 0 || ext->num_cones > BDEM_MAX_CONES) return fail(c, BDEM_EINVAL, "num_cones out of range");
+  if (c->poly && (ext->contact_law != BDEM_LAW_REFERENCE || ext->per_particle_radius))
+    return fail(c, BDEM_ENOTSUP, "polyhedra use the reference pair law and their type's bounding radius");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->ext = *ext;
@@ -601,7 +663,16 @@ int bdem_get_system_state(bdem_ctx* c, bdem_float3* pos, bdem_Quaterion* ornt, b
   std::vector<float4> P, V, O, Q;
   if ((rc = download(c, P, V, O, Q)) != BDEM_OK) return rc;
   for (int i = 0; i < c->b.n; ++i) {
-    if (pos) pos[i] = bdem_float3{P[i].x, P[i].y, P[i].z};
+    const int packed_t = f_bits(O[i].w) & 7;
+    if (pos) {
+      pos[i] = bdem_float3{P[i].x, P[i].y, P[i].z};
+      if (c->poly && packed_t < int(c->pobj.size())) {
+        /* Get_SystemState / Get_Particle_PosArray return position - COM for
+         * polyhedra (GPU_Memory_Transfer.cuh:461, 504) */
+        const bdem_float3 cm = c->pobj[packed_t].COM;
+        pos[i] = bdem_float3{P[i].x - cm.x, P[i].y - cm.y, P[i].z - cm.z};
+      }
+    }
     if (vel) vel[i] = bdem_float3{V[i].x, V[i].y, V[i].z};
     if (avel) avel[i] = bdem_float3{O[i].x, O[i].y, O[i].z};
     if (acc) acc[i] = bdem_float3{0.f, 0.f, 0.f};
@@ -783,8 +854,12 @@ int bdem_update_sim(bdem_ctx* c) {
     if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
     else c->stepc += 1;
   }
-  launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
-              c->append_par, do_kill, c->record_wf, c->stream);
+  if (c->poly)
+    launch_poly_step(c->dp, c->dpobj, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
+                     c->record_wf, c->stream);
+  else
+    launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
+                c->append_par, do_kill, c->record_wf, c->stream);
   if (do_kill) c->full_next = true;
   if (ev) {
     (void)hipEventRecord(ev[2], c->stream);
@@ -918,8 +993,12 @@ int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw,
   const int n = c->b.n;
   float* d = nullptr;
   HIPCHK(c, hipMalloc((void**)&d, (size_t)n * 12 * sizeof(float)));
-  launch_forces(c->dp, c->b, c->hp.law, c->hist_par, d, d + 3 * (size_t)n, d + 6 * (size_t)n,
-                d + 9 * (size_t)n, c->stream);
+  if (c->poly)
+    launch_poly_forces(c->dp, c->dpobj, c->b, c->pb, d, d + 3 * (size_t)n, d + 6 * (size_t)n, d + 9 * (size_t)n,
+                       c->stream);
+  else
+    launch_forces(c->dp, c->b, c->hp.law, c->hist_par, d, d + 3 * (size_t)n, d + 6 * (size_t)n,
+                  d + 9 * (size_t)n, c->stream);
   hipError_t e = hipStreamSynchronize(c->stream);
   std::vector<float> h((size_t)n * 12);
   if (e == hipSuccess) e = hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost);

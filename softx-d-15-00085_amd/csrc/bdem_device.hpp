@@ -88,6 +88,7 @@ struct DevParams {
   bdem_float3 gravity;
   float roll_res;
   int rotation, mode2d, sphere_orient;
+  int poly;                   /* particle_type 1: polyhedra */
   int law;                    /* BDEM_LAW_* */
   int per_particle;           /* 1: inverse inertia from per-particle radius/mass */
   /* walls */
@@ -426,15 +427,12 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
 /* Kernel_ParticleInteraction_Spheres_NonSymmetry DEM_Kernels_Compute.cuh:656-712 */
 /* Collision_Response_Particle_Particle Collision_Response.cuh:374-494       */
 /* ------------------------------------------------------------------------ */
-__device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3 va, float ma,
-                                         f3 wa, int ta, f3 pb, float rb, f3 vb, float mb, f3 wb,
-                                         int tb, f3& Fc, f3& Tc) {
-  const f3 relPos = pa - pb;
+/* Collision_Response_Particle_Particle (Collision_Response.cuh:374-494) for
+ * normal N (towards A), depth dis and lever arms RA, RB from the centres. */
+__device__ __forceinline__ void pair_resp(const DevParams& P, f3 va, float ma, f3 wa, int ta, f3 vb,
+                                          float mb, f3 wb, int tb, f3 N, float dis, f3 RA, f3 RB,
+                                          f3& Fc, f3& Tc) {
   const f3 RVel = va - vb;
-  const f3 N = relPos / length(relPos);
-  const float dis = fabsf(length(relPos) - (ra + rb)) + 0.000010f;
-  const f3 RA = (-1.0f * N) * ra;
-  const f3 RB = N * rb;
   const float Kn = P.pp_kn[ta][tb], Cn = P.pp_cn[ta][tb], Fk = P.pp_fk[ta][tb];
   const f3 Vnormal = dot(RVel, N) * N;
   const float Fnormal = Kn * dis;
@@ -459,6 +457,17 @@ __device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3
   Ft = Ft + Tang;
   Fc = Ft;                 /* the kernel adds it: force += Force_PairA.trans (:709) */
   Tc = cross(RA, Ft);      /* torque = cross(R_Body_COMA, Force.trans) (:485) */
+}
+
+__device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3 va, float ma,
+                                         f3 wa, int ta, f3 pb, float rb, f3 vb, float mb, f3 wb,
+                                         int tb, f3& Fc, f3& Tc) {
+  const f3 relPos = pa - pb;
+  const f3 N = relPos / length(relPos);
+  const float dis = fabsf(length(relPos) - (ra + rb)) + 0.000010f;
+  const f3 RA = (-1.0f * N) * ra;
+  const f3 RB = N * rb;
+  pair_resp(P, va, ma, wa, ta, vb, mb, wb, tb, N, dis, RA, RB, Fc, Tc);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -903,11 +912,59 @@ __device__ __forceinline__ int lifter_forces(const DevParams& P, int t, f3 p, fl
   return res.type;
 }
 
+/* Get_Angular_Acc (Device/Utilities.cuh:486-608): IC = RQ I0, then
+ * IN[i][j] = sum_k IC[k][j] RQT[i][k] = (RQ^T RQ I0)[i][j], which is I0 up to
+ * rounding, applied to the torque; summation orders as written. */
+__device__ __forceinline__ f3 angular_acc(const float* __restrict__ I0, QuatF Q, f3 tau) {
+  float RQ[9];
+  RQ[0] = (Q.w * Q.w + Q.x * Q.x - Q.y * Q.y - Q.z * Q.z);
+  RQ[1] = (2.0f * Q.x * Q.y - 2.0f * Q.w * Q.z);
+  RQ[2] = (2.0f * Q.x * Q.z + 2.0f * Q.w * Q.y);
+  RQ[3] = (2.0f * Q.x * Q.y + 2.0f * Q.w * Q.z);
+  RQ[4] = (Q.w * Q.w - Q.x * Q.x + Q.y * Q.y - Q.z * Q.z);
+  RQ[5] = (2.0f * Q.y * Q.z - 2.0f * Q.w * Q.x);
+  RQ[6] = (2.0f * Q.x * Q.z - 2.0f * Q.w * Q.y);
+  RQ[7] = (2.0f * Q.y * Q.z + 2.0f * Q.w * Q.x);
+  RQ[8] = (Q.w * Q.w - Q.x * Q.x - Q.y * Q.y + Q.z * Q.z);
+  float IC[9], IN[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sum += (RQ[r * 3 + k] * I0[k * 3 + c]);
+      IC[r * 3 + c] = sum;
+    }
+  const float RQT[9] = {RQ[0], RQ[3], RQ[6], RQ[1], RQ[4], RQ[7], RQ[2], RQ[5], RQ[8]};
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) sum += ((IC[k * 3 + c]) * RQT[r * 3 + k]);
+      IN[r * 3 + c] = sum;
+    }
+  const float f[3] = {tau.x, tau.y, tau.z};
+  float out[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sum += (IN[r * 3 + k] * (f[k]));
+    out[r] = sum;
+  }
+  return mk3(out[0], out[1], out[2]);
+}
+
 /* Integrate_Euler_NonSymmetry (Integration.cuh:6-184); the lifter terms are
- * added only when lifters exist (Integration.cuh:41-44, 92-95). */
+ * added only when lifters exist (Integration.cuh:41-44, 92-95).  Polyhedra
+ * (I0 given) take the tensor product of Get_Angular_Acc and always carry
+ * their orientation. */
 __device__ __forceinline__ void integrate(const DevParams& P, float inv_inertia, float m, f3 Fpp,
                                           f3 Tpp, f3 Fw, f3 Tw, f3 Fl, f3 Tl, f3& p, f3& v, f3& w,
-                                          QuatF* q) {
+                                          QuatF* q, const float* I0 = nullptr) {
   const float dt = P.dt;
   f3 acc = Fw + Fpp;
   if (P.num_dobj > 0) acc = acc + Fl;
@@ -919,11 +976,11 @@ __device__ __forceinline__ void integrate(const DevParams& P, float inv_inertia,
   if (P.num_dobj > 0) fl = fl + Tl;
   if (P.rotation) {
     if (dot(fl, fl) > 0.0f) {
-      const f3 aa = inv_inertia * fl;
+      const f3 aa = I0 ? angular_acc(I0, *q, fl) : inv_inertia * fl;
       w = w + aa * dt;
       w = w - w * P.roll_res;
     }
-    if (P.sphere_orient && q) {
+    if ((P.sphere_orient || I0) && q) {
       float mo = dot(w, w);
       QuatF dQ{1.0f, 0.0f, 0.0f, 0.0f};
       if (mo > 0.000000f) {

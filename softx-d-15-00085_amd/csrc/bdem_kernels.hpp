@@ -39,6 +39,21 @@ struct StepBuffers {
   int *kill_count;
 };
 
+/* Polyhedra: world-frame shapes, one record of `stride` float4 per particle
+ * (pv vertex slots, pf face normals, pf face centroids), the bounding-sphere
+ * contact list, its pair queue and one force/torque slot per listed pair. */
+struct PolyLayout { int pv, pf, stride; };
+struct PolyBuffers {
+  PolyLayout L;
+  float4* rot;                         /* n * stride */
+  uint32_t* list;                      /* n * 32 partner slots */
+  uint32_t* nn;                        /* full contact count per particle */
+  uint32_t* queue;                     /* (particle * 32 + k) work items */
+  uint32_t* qcount;                    /* queue length */
+  float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
+};
+constexpr int POLY_PAIR_BLOCKS = 2048;
+
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
@@ -95,6 +110,14 @@ hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
                  bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force,
                  hipStream_t s);
+
+/* Polyhedra step (rotate -> list + queue -> pairs -> finish), srt -> cur,
+ * with the same outputs as launch_step; and the force stage alone. */
+void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+                      const uint32_t* sorted_keys, int append_par, bool do_kill, bool want_wall_force,
+                      hipStream_t s);
+void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+                        float* fpp, float* tpp, float* fw, float* tw, hipStream_t s);
 
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,

@@ -1,0 +1,740 @@
+/*
+ * bdem_poly.hip - polyhedra (particle_type 1) step for gfx950.
+ *
+ * Kernel_ParticleInteraction_Polyhedra_NonSymmetry (DEM_Kernels_Compute.cuh:
+ * 736-992) rotates both shapes for every listed pair and tests one pair per
+ * particle thread in a serial loop.  Here a step is four launches:
+ *
+ *   k_poly_rotate   one thread per (particle, vertex / face-normal /
+ *                   face-centroid slot): world-frame shapes into `rot`,
+ *                   written coalesced (RotatePolyhedra, Utilities.cuh:420-479)
+ *   k_poly_list     bounding-sphere contact list in the reference order (the
+ *                   sphere broad phase), plus a queue of (particle, slot)
+ *                   work items appended one atomic per wave
+ *   k_poly_pairs    one thread per queued pair, grid-stride over the queue:
+ *                   separating plane, vertex-face both ways, edge-edge, depth
+ *                   clamps, pair law; the force and torque go to the pair's
+ *                   own slot, so the queue order does not matter
+ *   k_poly_finish   one thread per particle: gravity, then the slots in list
+ *                   order (the reference's summation order), walls, the
+ *                   tensor integrator and quaternion, discharge, next key,
+ *                   changed flag and changed-list append
+ *
+ * Every float expression follows the restatement's operation order, so the
+ * results equal the oracle's bit for bit away from sin/cos in the quaternion.
+ */
+#include "bdem_kernels.hpp"
+
+namespace bdem {
+
+namespace {
+
+__device__ __forceinline__ f3 ld3(const float4* __restrict__ p) {
+  const float4 v = *p;
+  return mk3(v.x, v.y, v.z);
+}
+
+/* q v q* with (0, v), the reference's make_quaterionD / make_vectorD. */
+__device__ __forceinline__ f3 qrot(QuatF Q, QuatF QT, f3 v) {
+  const QuatF p{0.0f, v.x, v.y, v.z};
+  const QuatF r = qmul(qmul(Q, p), QT);
+  return mk3(r.x, r.y, r.z);
+}
+
+/* A rotated shape in `rot`: vertices, then face normals, then face centroids. */
+struct Shape {
+  const float4* V;
+  const float4* Fn;
+  const float4* Fc;
+  int nv, nf;
+};
+__device__ __forceinline__ Shape shape_of(const float4* __restrict__ rot, const PolyLayout& L, int i,
+                                          const bdem_POBJ& po) {
+  const float4* base = rot + (size_t)i * L.stride;
+  return Shape{base, base + L.pv, base + L.pv + L.pf, po.num_vertex, po.num_faces};
+}
+
+/* Face f; f = -1 (no face of B had A's centre in front) is a zero plane. */
+__device__ __forceinline__ f3 fnorm(const Shape& B, int f) { return f >= 0 ? ld3(B.Fn + f) : mk3(0.f, 0.f, 0.f); }
+__device__ __forceinline__ f3 fcent(const Shape& B, int f) { return f >= 0 ? ld3(B.Fc + f) : mk3(0.f, 0.f, 0.f); }
+
+enum { C_UNDEF = 0, C_VF = 1, C_EF = 2, C_FF = 3, C_EE = 4, C_SP = 5 };
+struct PColl { bool hit; int cls; float dis; f3 n; f3 cp; };
+
+/* is_PointOnSurface over a polyhedron's rotated vertices (Utilities.cuh:20-53) */
+__device__ __forceinline__ bool point_on_face(const bdem_Surface& f, const float4* __restrict__ V, f3 point,
+                                              float tol) {
+  float area = 0.000000f;
+  const int nv = f.num_vertex;
+  for (int q = 0; q < nv - 1; ++q) {
+    const f3 V1 = point - ld3(V + f.vertex_Order[q]);
+    const f3 V2 = point - ld3(V + f.vertex_Order[q + 1]);
+    area = area + 0.500000f * length(cross(V1, V2));
+  }
+  const f3 V1 = point - ld3(V + f.vertex_Order[nv - 1]);
+  const f3 V2 = point - ld3(V + f.vertex_Order[0]);
+  area = area + 0.500000f * length(cross(V1, V2));
+  return (fabsf(area - f.area) / f.area) * 100.0000f <= tol;
+}
+
+/* is_SP_Poly_Poly (Utilities.cuh:105-156) */
+__device__ __forceinline__ bool sp_poly_poly(const Shape& A, f3 PosB, const Shape& B) {
+  for (int f = 0; f < A.nf; ++f) {
+    const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
+    if (dot(n, PosB - c) > 0.0f) {
+      bool contact = false;
+      for (int v = 0; v < B.nv; ++v) {
+        if (dot(n, ld3(B.V + v) - c) <= 0.00000f) { contact = true; break; }
+      }
+      if (!contact) return true;
+    }
+  }
+  return false;
+}
+
+/* isPointinPoly (Utilities.cuh:63-98) */
+__device__ __forceinline__ bool point_in_poly(const Shape& A, f3 point) {
+  for (int f = 0; f < A.nf; ++f)
+    if (!(dot(ld3(A.Fn + f), point - ld3(A.Fc + f)) <= 0.000100f)) return false;
+  return true;
+}
+
+struct InPoly { int f0, f1, f2; float d0, d1, d2; };
+
+/* Collision_Detection_Vertex_InPoly_3Faces (Collision_Detection_Polyhedra.cuh:287-375):
+ * inside every face of B; the three closest faces that have PosP in front. */
+__device__ __forceinline__ bool vertex_in_poly(f3 PosP, f3 v, const Shape& B, InPoly& r) {
+  r = InPoly{-1, -1, -1, 10.0f, 10.0f, 10.0f};
+  for (int f = 0; f < B.nf; ++f) {
+    const f3 n = ld3(B.Fn + f), c = ld3(B.Fc + f);
+    const float vd = dot(n, v - c);
+    if (!(vd < -0.0000100f)) return false;
+    if (dot(n, PosP - c) > 0.0f) {
+      const float a = fabsf(vd);
+      if (a < r.d0) {
+        if (r.f0 > -1) {
+          if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
+          r.d1 = r.d0; r.f1 = r.f0;
+        }
+        r.d0 = a; r.f0 = f;
+      } else if (a < r.d1) {
+        if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
+        r.d1 = a; r.f1 = f;
+      } else if (a < r.d2) {
+        r.d2 = a; r.f2 = f;
+      }
+    }
+  }
+  return true;
+}
+
+/* The ray from PosA through `through` meets face f of B inside its polygon. */
+__device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& B, const bdem_POBJ& pb,
+                                              int f, float tol) {
+  if (f < 0) return false;
+  f3 Ray = through - PosA;
+  Ray = Ray / length(Ray);
+  const f3 n = ld3(B.Fn + f);
+  const float dir = dot(Ray, n);
+  if (!(fabsf(dir) > 0.0f)) return false;
+  const float d = dot((ld3(B.Fc + f) - PosA), n) / dir;
+  return point_on_face(pb.face[f], B.V, PosA + d * Ray, tol);
+}
+
+__device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ? s.f0 : (q == 1 ? s.f1 : s.f2); }
+
+/* Collision_Detection_Polyhedara_Polyhedra_VertexFaces
+ * (Collision_Detection_Polyhedra.cuh:1154-1630): vertices of A inside B; one
+ * -> vertex-face, two -> edge-face, more -> face-face (the first eight
+ * inside vertices: the reference's lists hold eight), with a ray test
+ * choosing among the three closest faces. */
+__device__ __forceinline__ PColl vertex_faces(f3 PosA, const Shape& A, const Shape& B, const bdem_POBJ& pb) {
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  int np = 0;
+  InPoly s0{}, t{};
+  int s1f1 = -1;
+  int v0 = -1, v1 = -1;
+  f3 sum = mk3(0.0f, 0.0f, 0.0f);
+  for (int v = 0; v < A.nv; ++v) {
+    const f3 pv = ld3(A.V + v);
+    if (!vertex_in_poly(PosA, pv, B, t)) continue;
+    if (np < 8) {
+      if (np == 0) { s0 = t; v0 = v; }
+      if (np == 1) { s1f1 = t.f1; v1 = v; }
+      sum = sum + pv;
+    }
+    ++np;
+  }
+  if (np == 0) return res;
+  np = np < 8 ? np : 8;
+  res.hit = true;
+  if (np == 1) {
+    res.cls = C_VF;
+    res.dis = s0.d0;
+    res.n = fnorm(B, s0.f0);
+    res.cp = ld3(A.V + v0);
+    if (s0.f1 == -1) return res;
+    if (ray_hits_face(PosA, res.cp, B, pb, s0.f0, 0.10f)) return res;
+    if (s0.f1 > -1 && ray_hits_face(PosA, res.cp, B, pb, s0.f1, 0.10f)) {
+      res.n = fnorm(B, s0.f1);
+      res.dis = s0.d1;
+      return res;
+    }
+    if (s0.f2 > -1 && ray_hits_face(PosA, res.cp, B, pb, s0.f2, 0.10f)) {
+      res.n = fnorm(B, s0.f2);
+      res.dis = s0.d2;
+      return res;
+    }
+    return res;
+  }
+  if (np == 2) {
+    res.cls = C_EF;
+    res.cp = (ld3(A.V + v0) + ld3(A.V + v1)) * 0.50f;
+    res.dis = fabsf(dot(fnorm(B, s0.f0), (res.cp - fcent(B, s0.f0))));
+    res.n = fnorm(B, s0.f0);
+    if (s0.f1 == -1 && s1f1 == -1) return res;
+    for (int q = 0; q < 3; ++q) {
+      const int f = face_at(s0, q);
+      if (q > 0 && !(f > -1)) continue;
+      if (ray_hits_face(PosA, res.cp, B, pb, f, 0.10f)) {
+        res.dis = fabsf(dot(fnorm(B, f), (res.cp - fcent(B, f))));
+        res.n = fnorm(B, f);
+        return res;
+      }
+    }
+    return res;
+  }
+  res.cls = C_FF;
+  res.cp = sum / (float)np;
+  for (int q = 0; q < 3; ++q) {
+    const int f = face_at(s0, q);
+    if (q > 0 && !(f > -1)) continue;
+    if (ray_hits_face(PosA, res.cp, B, pb, f, q == 1 ? 0.010f : 0.10f)) {
+      res.dis = fabsf(dot(fnorm(B, f), (res.cp - fcent(B, f))));
+      res.n = fnorm(B, f);
+      return res;
+    }
+  }
+  return res;   /* face logic fall-through: depth 0, zero normal */
+}
+
+/* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707) */
+__device__ __forceinline__ PColl edges(const Shape& A, const bdem_POBJ& pa, const Shape& B, const bdem_POBJ& pb,
+                                       f3 RelPos, f3 PosB) {
+  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
+  float min_edgedis = 5.0f;
+  f3 selA = mk3(0.f, 0.f, 0.f), selB = selA;
+  bool found = false;
+  const int ea = pa.num_edges, eb = pb.num_edges;
+  for (int i = 0; i < ea; ++i) {
+    const f3 PA = ld3(A.V + pa.edge[i].point_vindex[0]);
+    const f3 DA = ld3(A.V + pa.edge[i].point_vindex[1]) - PA;
+    const float DEAD = dot(DA, DA);
+    for (int j = 0; j < eb; ++j) {
+      const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
+      const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
+      const f3 dir = cross(DA, DB);
+      const float mag = dot(dir, dir);
+      if (!(mag > 0.00000f)) continue;
+      f3 DV = PA - PB;
+      const float DEBD = dot(DB, DB);
+      const float DEABD = dot(DA, DB);
+      const float DEADAB = dot(DA, DV);
+      const float DEBDAB = dot(DB, DV);
+      const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
+      const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+      if (!(dA >= -0.010f && dA <= 1.0100f)) continue;
+      const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+      if (!(dB >= -0.0100f && dB <= 1.010f)) continue;
+      const f3 pA = PA + DA * dA;
+      const f3 pB = PB + DB * dB;
+      DV = (pA - pB);
+      float dis = dot(DV, DV);
+      if (dis > 0.00000f) {
+        dis = __builtin_sqrtf(dis);
+        if (dis < min_edgedis) {
+          found = true;
+          min_edgedis = dis;
+          res.cp = pA;
+          res.n = dir;
+          selA = pA;
+          selB = pB;
+        }
+      }
+    }
+  }
+  if (!found) return res;
+  res.n = res.n * (1.0f / length(res.n));
+  if (length(PosB - selB) < length(PosB - selA)) {
+    if (!point_in_poly(B, selA)) return res;
+  }
+  res.hit = true;
+  res.cls = C_EE;
+  const float dn = dot(RelPos, res.n);
+  if (fabsf(dn) > 0.0f) {
+    res.n = (dot(RelPos, res.n) / fabsf(dot(RelPos, res.n))) * res.n;
+    res.dis = fabsf(dot(res.n, (selB - selA))) + 0.000010f;
+  } else {
+    res.n = RelPos / length(RelPos);
+    res.cp = PosB;
+    res.dis = fabsf(dot(res.n, (selB - selA))) + 0.000010f;
+  }
+  return res;
+}
+
+/* The contact of A with listed partner B after the depth clamps
+ * (DEM_Kernels_Compute.cuh:820-950). */
+__device__ __forceinline__ PColl poly_pair_contact(const bdem_POBJ& pa, const bdem_POBJ& pb, const Shape& A,
+                                                   const Shape& B, f3 PosA, f3 PosB, f3 RVel) {
+  const f3 RelPos = (PosA - PosB) / length(PosA - PosB);
+  bool sp = sp_poly_poly(A, PosB, B);
+  if (!sp) sp = sp_poly_poly(B, PosA, A);
+  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  if (!sp) {
+    res = vertex_faces(PosA, A, B, pb);
+    if (!res.hit) {
+      res = vertex_faces(PosB, B, A, pa);
+      res.n = res.n * -1.0f;   /* B into A: normal reversed */
+    }
+    if (!res.hit) {
+      res = edges(A, pa, B, pb, RelPos, PosB);
+      if (res.hit) {
+        res.cls = C_EE;
+        res.dis = fminf(0.025f * pa.radius, res.dis);
+      }
+    }
+  }
+  if (!res.hit) return res;
+  const float rA = pa.radius;
+  if (res.dis > 0.1500f * rA) {   /* contact error: fixed small overlap along the centres */
+    res.n = RelPos / length(RelPos);
+    res.dis = 0.025f * rA;
+    res.cp = PosA;
+    res.cls |= 8;
+  }
+  if (res.dis > 0.0500f * rA) {   /* large overlap: limited by speed, first branch wins as written */
+    res.n = RelPos / length(RelPos);
+    const float sp_rel = length(RVel);
+    if (sp_rel > 50.0f) res.dis = 0.010f * rA;
+    else if (sp_rel > 150.0f) res.dis = 0.015f * rA;
+    else if (sp_rel > 250.0f) res.dis = 0.020f * rA;
+    else res.dis = 0.0030f * rA;
+    res.cp = PosA;
+    res.cls |= 16;
+  }
+  return res;
+}
+
+/* ---------------------------------------------------------------- walls */
+/* Collision_Detection_Vertcies_Surface (Collision_Detection_Polyhedra.cuh:85-185) */
+__device__ __forceinline__ PColl vertices_surface(const Shape& A, const bdem_Surface& S, const bdem_float3* svtx) {
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
+  float dis_max = 0.0f;
+  int cnt = 0;
+  const f3 n = cvt(S.normal), c = cvt(S.centroid);
+  for (int v = 0; v < A.nv; ++v) {
+    const f3 pv = ld3(A.V + v);
+    const float vd = dot(n, (pv - c));
+    if (vd < 0.000000f) {
+      const f3 hit = pv + fabsf(vd) * n;
+      if (point_on_surface(S, svtx, hit)) {
+        ++cnt;
+        res.cp = res.cp + hit;
+        if (fabsf(vd) > dis_max) { res.hit = true; dis_max = fabsf(vd); }
+        if (cnt > 2) {
+          res.cls = C_FF; res.n = n; res.dis = dis_max;
+          res.cp = mk3(0.0f, 0.0f, 0.0f);
+          return res;
+        }
+      }
+    }
+  }
+  if (!res.hit) { res.cls = C_SP; return res; }
+  res.cls = cnt == 1 ? C_VF : C_EF;
+  res.n = n;
+  res.dis = dis_max;
+  res.cp = res.cp / (float)cnt;
+  return res;
+}
+
+/* Collision_Detection_Polyhedra_Surface (Collision_Detection_Polyhedra.cuh:489-529) */
+__device__ __forceinline__ PColl poly_surface(f3 p, float r, const Shape& A, const bdem_Surface& S,
+                                              const bdem_float3* svtx) {
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  const f3 n = cvt(S.normal), c = cvt(S.centroid);
+  if (dot(n, (p - c)) > 0.000000f) {
+    if (dot(n, (p - c)) <= r) {
+      res = vertices_surface(A, S, svtx);
+      if (res.cls == C_FF) res.cp = p;   /* no moment for face-face */
+    }
+  }
+  return res;
+}
+
+/* Collision_Detection_Vertcies_Plane (Collision_Detection_Polyhedra.cuh:6-84) */
+__device__ __forceinline__ PColl vertices_plane(const Shape& A, f3 n, f3 c) {
+  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
+  float dis_max = 0.0f;
+  int cnt = 0;
+  for (int v = 0; v < A.nv; ++v) {
+    const f3 pv = ld3(A.V + v);
+    const float vd = dot(n, (pv - c));
+    if (vd < 0.000000f) {
+      ++cnt;
+      res.cp = res.cp + pv;
+      if (fabsf(vd) > dis_max) dis_max = fabsf(vd);
+    }
+  }
+  if (cnt == 0) return res;
+  res.hit = true;
+  res.cls = cnt == 2 ? C_EF : C_FF;
+  res.cp = res.cp / (float)cnt;
+  res.n = n;
+  res.dis = dis_max;
+  return res;
+}
+
+/* Collision_Detection_Polyhedra_Cylinder (Collision_Detection_Polyhedra.cuh:2010-2132);
+ * the running depth is averaged in double, as written. */
+__device__ __forceinline__ PColl poly_cylinder(f3 p, float r, const Shape& A, const bdem_Macro_Cylinder& cyl) {
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  const f3 AX = cvt(cyl.AXIS), cb = cvt(cyl.center_bot_cap);
+  const f3 Pvec = p * AX - cb;
+  const float Pvec_dis = length(Pvec) + r;
+  if (!(Pvec_dis > cyl.radius)) return res;
+  float dis = 0.0f;
+  int cnt = 0;
+  f3 s_point = mk3(0.0f, 0.0f, 0.0f);
+  for (int v = 0; v < A.nv; ++v) {
+    const f3 pv = ld3(A.V + v);
+    const f3 vp = pv * AX - cb;
+    const float rc = length(vp);
+    if (rc > cyl.radius) {
+      const float vd = rc - cyl.radius;
+      ++cnt;
+      s_point = s_point + pv;
+      dis = dis + vd;
+      if (cnt > 2) {
+        dis = (float)((double)dis * (1.0 / (double)(float)cnt));
+        res.hit = true;
+        res.cls = C_FF;
+        res.n = -1.0f * Pvec;
+        res.n = res.n * (1.0f / length(res.n));
+        res.dis = fminf(0.00100f, fabsf(dis));
+        res.cp = p;
+        return res;
+      }
+    }
+  }
+  if (cnt == 0) return res;
+  dis = (float)((double)dis * (1.0 / (double)(float)cnt));
+  res.n = -1.0f * Pvec;
+  res.n = res.n * (1.0f / length(res.n));
+  res.hit = true;
+  if (cnt == 2) {
+    res.cls = C_EF;
+    res.cp = s_point / (float)cnt;
+    res.dis = fabsf(dis);
+    return res;
+  }
+  res.cp = s_point;
+  res.dis = dis;
+  res.cls = C_VF;
+  return res;
+}
+
+/* WorldObject_InteractionParticle, polyhedra branches (DEM_Kernels_Compute.cuh:
+ * 254-610).  Infinite planes have no polyhedra method: the previous result
+ * stands, as written. */
+__device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p, float r, f3 v, f3 w, float m,
+                                                 const Shape& A, f3& F, f3& T) {
+  F = mk3(0.f, 0.f, 0.f);
+  T = mk3(0.f, 0.f, 0.f);
+  const bool moving_planes = P.rotating_wobj && P.cyl_rotation;
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  for (int k = 0; k < P.num_wobj; ++k) {
+    const bdem_WOBJ& W = P.wobj[k];
+    if (W.surface_type == BDEM_SURFACE_PLANE) {
+      for (int s = 0; s < W.num_surfaces; ++s) {
+        const bdem_Surface& S = W.surfaces[s];
+        if (S.area != -1.0f) res = poly_surface(p, r, A, S, W.vertex);
+        if (!res.hit) continue;
+        const Coll c{true, res.dis, res.cp, res.n};
+        f3 ft = mk3(0.f, 0.f, 0.f), tq = ft;
+        if (moving_planes) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq);
+        else resp_static(P, t, p, v, w, m, c, ft, tq);
+        F = F + ft;
+        if (P.rotation && res.cls != C_FF) T = T + tq;
+      }
+    } else if (W.surface_type == BDEM_SURFACE_CYLINDER) {
+      const bdem_Macro_Cylinder& cyl = W.cyl_geo;
+      res = poly_cylinder(p, r, A, cyl);
+      if (res.hit) {
+        const Coll c{true, res.dis, res.cp, res.n};
+        f3 ft = mk3(0.f, 0.f, 0.f), tq = ft;
+        if (P.cyl_rotation) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq);
+        else resp_static(P, t, p, v, w, m, c, ft, tq);
+        F = ft;
+        if (P.rotation && res.cls != C_FF) T = T + tq;
+      }
+      if (cyl.has_top_cap) {
+        res = vertices_plane(A, cvt(cyl.normal_top_cap), cvt(cyl.center_top_cap));
+        if (res.hit) F = resp_endplate(P, t, v, Coll{true, res.dis, res.cp, res.n});
+      }
+      if (cyl.has_bot_cap) {
+        res = vertices_plane(A, cvt(cyl.normal_bot_cap), cvt(cyl.center_bot_cap));
+        if (res.hit) F = resp_endplate(P, t, v, Coll{true, res.dis, res.cp, res.n});
+      }
+    }
+  }
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ kernels */
+__global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict__ Pp,
+                                                     const bdem_POBJ* __restrict__ pobj, int n,
+                                                     PolyLayout L, const float4* __restrict__ pos,
+                                                     const float4* __restrict__ omg,
+                                                     const float4* __restrict__ quat,
+                                                     float4* __restrict__ rot, uint32_t* __restrict__ qcount) {
+  const DevParams& P = *Pp;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *qcount = 0u;   /* the pair queue of this step starts empty */
+  const size_t i = t / (size_t)L.stride;
+  const int e = (int)(t - i * (size_t)L.stride);
+  if (i >= (size_t)n) return;
+  const bdem_POBJ& po = pobj[packed_type(omg[i].w)];
+  const f3 c = xyz(pos[i]);
+  const float4 q4 = quat[i];
+  const QuatF Q{q4.x, q4.y, q4.z, q4.w};
+  const QuatF QT{Q.w, -Q.x, -Q.y, -Q.z};
+  f3 out = mk3(0.f, 0.f, 0.f);
+  if (e < L.pv) {
+    if (e < po.num_vertex) out = (P.rotation ? qrot(Q, QT, cvt(po.Local_Vertex[e])) : cvt(po.Local_Vertex[e])) + c;
+  } else if (e < L.pv + L.pf) {
+    const int f = e - L.pv;
+    if (f < po.num_faces) out = P.rotation ? qrot(Q, QT, cvt(po.face[f].normal)) : cvt(po.face[f].normal);
+  } else {
+    const int f = e - L.pv - L.pf;
+    if (f < po.num_faces) {
+      const f3 d = cvt(po.face[f].centroid) - cvt(po.COM);
+      out = (P.rotation ? qrot(Q, QT, d) : d) + c;
+    }
+  }
+  rot[t] = make_float4(out.x, out.y, out.z, 0.0f);
+}
+
+/* Bounding-sphere contact list (the sphere broad phase: Kernel_BroadCollision-
+ * Detection_NonSymmetry, 32 slots, full count) and the pair work queue. */
+__global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__ Pp, int n,
+                                                   const uint2* __restrict__ cells,
+                                                   const float4* __restrict__ pos_r,
+                                                   const float4* __restrict__ omg,
+                                                   uint32_t* __restrict__ list32, uint32_t* __restrict__ nn,
+                                                   uint32_t* __restrict__ queue, uint32_t* __restrict__ qcount,
+                                                   uint32_t* status) {
+  const DevParams& P = *Pp;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = 0;
+  if (i < n && packed_pid(omg[i].w) >= 0) {
+    const float4 pr = pos_r[i];
+    c = for_each_contact(P, cells, pos_r, i, xyz(pr), pr.w, [&](int j, float4, int k) {
+      if (k < LIST_STRIDE) list32[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
+    });
+    if (c > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
+  }
+  if (i < n) nn[i] = (uint32_t)c;
+  const int m = c < LIST_STRIDE ? c : LIST_STRIDE;
+  /* wave prefix of the per-lane pair counts, one atomic per wave */
+  const int lane = threadIdx.x & 63;
+  int incl = m;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  const int total = __shfl(incl, 63, 64);
+  uint32_t base = 0u;
+  if (lane == 63 && total > 0) base = atomicAdd(qcount, (uint32_t)total);
+  base = __shfl(base, 63, 64);
+  const uint32_t at = base + (uint32_t)(incl - m);
+  const uint32_t cap = (uint32_t)n * LIST_STRIDE;
+  for (int k = 0; k < m; ++k)
+    if (at + (uint32_t)k < cap) queue[at + k] = (uint32_t)i * LIST_STRIDE + (uint32_t)k;
+}
+
+/* One queued pair per thread; grid-stride so that every wave exits once the
+ * queue (at most n * 32 entries) is drained. */
+__global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict__ Pp,
+                                                    const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+                                                    const float4* __restrict__ rot,
+                                                    const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                    const float4* __restrict__ omg,
+                                                    const uint32_t* __restrict__ list32,
+                                                    const uint32_t* __restrict__ queue,
+                                                    const uint32_t* __restrict__ qcount,
+                                                    float4* __restrict__ pf, float4* __restrict__ pt) {
+  const DevParams& P = *Pp;
+  const uint32_t total = *qcount;
+  const uint32_t cap = (uint32_t)n * LIST_STRIDE;
+  const uint32_t cnt = total < cap ? total : cap;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
+    const uint32_t slot = queue[q];
+    const int a = (int)(slot / LIST_STRIDE);
+    const int b = (int)list32[slot];
+    const float4 oa = omg[a], ob = omg[b];
+    const int ta = packed_type(oa.w), tb = packed_type(ob.w);
+    const bdem_POBJ& pa = pobj[ta];
+    const bdem_POBJ& pb = pobj[tb];
+    const float4 va4 = vel[a], vb4 = vel[b];
+    const f3 PosA = xyz(pos[a]), PosB = xyz(pos[b]);
+    const f3 va = xyz(va4), vb = xyz(vb4);
+    const Shape A = shape_of(rot, L, a, pa);
+    const Shape B = shape_of(rot, L, b, pb);
+    const PColl c = poly_pair_contact(pa, pb, A, B, PosA, PosB, va - vb);
+    float4 F4 = make_float4(0.f, 0.f, 0.f, 0.f), T4 = F4;
+    if (c.hit) {
+      f3 F, T;
+      pair_resp(P, va, va4.w, xyz(oa), ta, vb, vb4.w, xyz(ob), tb, c.n, c.dis, c.cp - PosA, c.cp - PosB, F, T);
+      F4 = make_float4(F.x, F.y, F.z, 1.0f);
+      T4 = make_float4(T.x, T.y, T.z, 0.0f);
+    }
+    pf[slot] = F4;
+    pt[slot] = T4;
+  }
+}
+
+/* Per particle: gravity, the pair slots in list order, walls, integration,
+ * discharge, next key, changed flag and changed-list append (as k_step).
+ * FORCES_ONLY stops after the forces and writes them (debug stage). */
+template <bool FORCES_ONLY>
+__global__ __launch_bounds__(256) void k_poly_finish(
+    const DevParams* __restrict__ Pp, const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+    const float4* __restrict__ rot, const uint32_t* __restrict__ nn, const float4* __restrict__ pf,
+    const float4* __restrict__ pt, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+    const float4* __restrict__ omg_in, const float4* __restrict__ quat_in, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, float4* __restrict__ omg_out, float4* __restrict__ quat_out,
+    uint32_t* __restrict__ keys_out, const uint32_t* __restrict__ sorted_keys, uint8_t* __restrict__ changed,
+    unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ append, const uint32_t* __restrict__ hwc_in,
+    uint32_t* __restrict__ hwc_out, float* __restrict__ wall_force, int do_kill, uint32_t* status,
+    int* kill_count, float* __restrict__ fpp, float* __restrict__ tpp, float* __restrict__ fw,
+    float* __restrict__ tw) {
+  const DevParams& P = *Pp;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const bool in = i < n;
+  bool chg = false;
+  uint32_t new_key = 0u;
+  if (in) {
+    const float4 pr = pos_in[i], vm = vel_in[i], ot = omg_in[i], q4 = quat_in[i];
+    int pid = packed_pid(ot.w);
+    const int ta = packed_type(ot.w);
+    f3 p = xyz(pr), v = xyz(vm), w = xyz(ot);
+    const float r = pr.w, m = vm.w;
+    QuatF q{q4.x, q4.y, q4.z, q4.w};
+    if (pid >= 0) {
+      const bdem_POBJ& po = pobj[ta];
+      f3 Fpp = cvt(P.gravity) * m, Tpp = mk3(0.f, 0.f, 0.f);
+      const uint32_t c = nn[i];
+      const int cnt = c < (uint32_t)LIST_STRIDE ? (int)c : LIST_STRIDE;
+      for (int k = 0; k < cnt; ++k) {
+        const float4 f = pf[(size_t)i * LIST_STRIDE + k];
+        if (f.w != 0.0f) {
+          Fpp = Fpp + xyz(f);
+          Tpp = Tpp + xyz(pt[(size_t)i * LIST_STRIDE + k]);
+        }
+      }
+      if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
+      const Shape A = shape_of(rot, L, i, po);
+      f3 Fw, Tw;
+      wall_forces_poly(P, ta, p, r, v, w, m, A, Fw, Tw);
+      if (FORCES_ONLY) {
+        fpp[3 * i] = Fpp.x; fpp[3 * i + 1] = Fpp.y; fpp[3 * i + 2] = Fpp.z;
+        tpp[3 * i] = Tpp.x; tpp[3 * i + 1] = Tpp.y; tpp[3 * i + 2] = Tpp.z;
+        fw[3 * i] = Fw.x; fw[3 * i + 1] = Fw.y; fw[3 * i + 2] = Fw.z;
+        tw[3 * i] = Tw.x; tw[3 * i + 1] = Tw.y; tw[3 * i + 2] = Tw.z;
+        return;
+      }
+      if (wall_force) {
+        wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
+      }
+      const f3 zero = mk3(0.f, 0.f, 0.f);
+      integrate(P, 0.0f, m, Fpp, Tpp, Fw, Tw, zero, zero, p, v, w, &q, po.InertiaT);
+      /* Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
+      if (do_kill && p.y < P.kill_height) {
+        p = cvt(P.max_size);
+        pid = -1;
+        atomicAdd(kill_count, 1);
+      }
+      pos_out[i] = make_float4(p.x, p.y, p.z, r);
+      vel_out[i] = make_float4(v.x, v.y, v.z, m);
+      omg_out[i] = make_float4(w.x, w.y, w.z, __int_as_float(pack_id_type(pid, ta)));
+      quat_out[i] = make_float4(q.w, q.x, q.y, q.z);
+      if (hwc_out) hwc_out[i] = 0u;
+    } else {
+      if (FORCES_ONLY) {
+        for (int k = 0; k < 3; ++k) { fpp[3 * i + k] = 0.f; tpp[3 * i + k] = 0.f; fw[3 * i + k] = 0.f; tw[3 * i + k] = 0.f; }
+        return;
+      }
+      pos_out[i] = pr;
+      vel_out[i] = vm;
+      omg_out[i] = ot;
+      quat_out[i] = q4;
+      if (hwc_out) hwc_out[i] = hwc_in ? hwc_in[i] : 0u;
+      if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
+    }
+    new_key = key_of(P, p, pid, status);
+    keys_out[i] = new_key;
+    chg = new_key != sorted_keys[i];
+    changed[i] = chg ? 1 : 0;
+  }
+  if (FORCES_ONLY) return;
+  const uint64_t mask = __ballot(chg);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(append, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t at = base + (uint32_t)__popcll(mask & lt);
+    if (chg) {
+      if (at < (uint32_t)n) sbuf[at] = ((unsigned long long)new_key << 32) | (uint32_t)i;
+      else atomicOr(status, ST_SORT_ERROR);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ launchers */
+static void launch_poly_forces_common(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+                                      const PolyBuffers& pb, hipStream_t s) {
+  const size_t rot_threads = (size_t)b.n * pb.L.stride;
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dpobj,
+                     b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
+  hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
+                     b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
+  /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
+  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dpobj, b.n, pb.L, pb.rot,
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.pf, pb.pt);
+}
+
+void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+                      const uint32_t* sk, int ap, bool do_kill, bool want_wf, hipStream_t s) {
+  launch_poly_forces_common(dP, dpobj, b, pb, s);
+  hipLaunchKernelGGL(k_poly_finish<false>, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+                     pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur,
+                     b.vel_cur, b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf,
+                     b.meta + META_APPEND + ap, b.hwc_srt, b.hwc_cur, want_wf ? b.wall_force : nullptr,
+                     do_kill ? 1 : 0, b.status, b.kill_count, nullptr, nullptr, nullptr, nullptr);
+}
+
+void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+                        float* fpp, float* tpp, float* fw, float* tw, hipStream_t s) {
+  launch_poly_forces_common(dP, dpobj, b, pb, s);
+  hipLaunchKernelGGL(k_poly_finish<true>, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+                     pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                     b.status, b.kill_count, fpp, tpp, fw, tw);
+}
+
+}  // namespace bdem

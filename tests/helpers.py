@@ -11,7 +11,8 @@ def make_device(deck):
         dev.set_ext(deck["ext"])
     if deck.get("radius") is not None:
         dev.set_radii(deck["radius"], deck["mass"])
-    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"])
+    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"],
+                  quat=deck.get("quat"))
     return dev
 
 
@@ -30,3 +31,14 @@ def by_pid(state):
     """Reorder a state dict by P_ID (dead particles keep their slot order)."""
     order = np.argsort(state["pid"], kind="stable")
     return {k: v[order] for k, v in state.items()}
+
+
+def oracle_positions_out(deck, state):
+    """The positions the library returns for the oracle's state: for
+    polyhedra, position - COM of the type in float (Get_SystemState,
+    GPU_Memory_Transfer.cuh:504)."""
+    pos = np.asarray(state["pos"], np.float32)
+    if deck["sim"].particle_type != 1:
+        return pos.astype(np.float64)
+    com = np.array([[p.COM.x, p.COM.y, p.COM.z] for p in deck["pobj"]], np.float32)
+    return (pos - com[state["ptype"]]).astype(np.float64)
