@@ -74,6 +74,22 @@ int bdem_set_ext_params(bdem_ctx* ctx, const bdem_ext_params* ext);
  * dimensions are fixed at setup. */
 int bdem_update_simulation_info(bdem_ctx* ctx, const bdem_SimulationInfo* sim);
 
+/* ---- materials (host only, no context, no device) ------------------------- */
+/* Spring constants from restitution, as the deck reader derives them:
+ * Set_SpringConstantsParticleKN / Surface_KN / DSurfaceKN (mode
+ * BDEM_SPRING_KN, Input/KSimulationData.cpp:955-1071: the caller sets
+ * PP[i].Kn, PP[i].Fric_* of every type i, surface_Kn and Dsurface_Kn) and
+ * Set_SpringConstantsParticle_TC / Surface_TC / DSurface_TC (mode
+ * BDEM_SPRING_TC, :887-953: the caller sets PP[0].Fric_*; tc_* are contact
+ * times in seconds, the deck value times the time step as at :1119-1121).
+ * Fills the PP matrix row by row (:276-292) and the wall dampings; at most
+ * 6 types (POBJ::PP[6]). */
+#define BDEM_SPRING_TC 0
+#define BDEM_SPRING_KN 1
+int bdem_derive_spring_constants(bdem_POBJ* pobj, int num_pobj, int mode, const double* cor_pp,
+                                 const double* cor_surface, const double* cor_dsurface,
+                                 double tc_pp, double tc_ps, double tc_pd);
+
 /* ---- state in ----------------------------------------------------------- */
 /* Device_Set_Pos, Device/DeviceInterface.h:22-23 (ornt may be NULL for spheres). */
 int bdem_set_pos(bdem_ctx* ctx, const bdem_float3* pos, const bdem_Quaterion* ornt,

@@ -30,6 +30,7 @@ _qp = C.POINTER(T.Quaterion)
 _up = C.POINTER(C.c_uint)
 _ip = C.POINTER(C.c_int)
 _fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
 
 _PROTOS = {
     "bdem_create": (C.c_int, [C.POINTER(_P), C.c_int]),
@@ -76,7 +77,26 @@ _PROTOS = {
     "bdem_debug_cells": (C.c_int, [_P, _up, _up]),
     "bdem_debug_contacts": (C.c_int, [_P, _up, _up]),
     "bdem_debug_forces": (C.c_int, [_P, _fp, _fp, _fp, _fp, _ip]),
+    "bdem_derive_spring_constants": (C.c_int, [C.POINTER(T.POBJ), C.c_int, C.c_int, _dp, _dp, _dp,
+                                               C.c_double, C.c_double, C.c_double]),
 }
+
+SPRING_TC, SPRING_KN = 0, 1
+
+
+def derive_spring_constants(pobj, mode, cor_pp, cor_surface, cor_dsurface, tc=(0.0, 0.0, 0.0)):
+    """bdem_derive_spring_constants on a list of POBJ (modified in place;
+    host only, no device needed)."""
+    lib = load()
+    arr = (T.POBJ * len(pobj))(*pobj)
+    c = [np.ascontiguousarray(x, np.float64) for x in (cor_pp, cor_surface, cor_dsurface)]
+    rc = lib.bdem_derive_spring_constants(arr, len(pobj), int(mode), *[_ptr(x, C.c_double) for x in c],
+                                          float(tc[0]), float(tc[1]), float(tc[2]))
+    if rc != 0:
+        raise BdemError(f"bdem_derive_spring_constants: status {rc}")
+    for k in range(len(pobj)):
+        pobj[k] = arr[k]
+    return pobj
 
 EXPORTED = tuple(_PROTOS)
 
