@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--settle", type=int, default=24000, help="untimed settle steps, outlet closed")
     ap.add_argument("--settle-dt", type=float, default=2e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skin", type=float, default=None,
+                    help="skin masks of the step's stencil scan in cm (default: off, every candidate tested)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run the library's plain copy kernel (k_copy_u4, "
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
@@ -115,6 +117,8 @@ def main():
     sim = deck["sim"]
     dt = sim.InitalDelta_t
     dev = make_device(bdem, deck)
+    if args.skin is not None:
+        dev.set_skin(args.skin)
     ncells = dev.num_cells()
     log(f"[bench] silo n={args.n} cells={ncells} setup {time.time() - t0:.1f}s")
 

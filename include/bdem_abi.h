@@ -166,13 +166,28 @@ int bdem_step(bdem_ctx* ctx, int nsteps);
 /* Wait for the context's stream. */
 int bdem_sync(bdem_ctx* ctx);
 
+/* Skin masks of the sphere step, in cm.  No reference counterpart: the
+ * reference tests every candidate of the 27-cell stencil every step
+ * (Collision_Detection_BroadPhase.cuh:402-513).  With skin > 0 the step
+ * keeps, per particle and stencil row, which candidates are within
+ * ra + rb + skin and tests only those until a particle enters or leaves the
+ * row's cells or the displacement bound of the epoch reaches skin / 2; the
+ * contacts and their order are the reference's either way.  skin <= 0
+ * turns the masks off (the default: on the silo benchmark the unmasked step
+ * is faster).  Masks need Nx == Nz (an injective hash) and are not used for
+ * polyhedra. */
+int bdem_set_skin(bdem_ctx* ctx, float skin);
+/* The skin in use (0: masks off). */
+float bdem_get_skin(const bdem_ctx* ctx);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Record HIP events on the context's stream around each stage of the next
  * max_steps bdem_update_sim calls.  No reference counterpart (the reference
  * has no timers). */
 int bdem_timing_begin(bdem_ctx* ctx, int max_steps);
-/* Wait, then return the summed milliseconds of the 2 stages
- * {sort (hash if needed, resort or full sort, gather, cell table), fused step
+/* Wait, then return the summed milliseconds of the 3 stages
+ * {sort (hash if needed, resort or full sort, gather, cell table), lists
+ * (skin-list marking and rebuilding; 0 when the lists are off), fused step
  * kernel} and the number of steps recorded; releases the events. */
 int bdem_timing_end(bdem_ctx* ctx, double* stage_ms, int* nsteps);
 /* Wait, then return how many particles the last sort had to move: the size
@@ -199,6 +214,11 @@ int bdem_debug_cells(bdem_ctx* ctx, unsigned int* cell_start, unsigned int* cell
 int bdem_debug_contacts(bdem_ctx* ctx, unsigned int* list32, unsigned int* nn);
 /* Pair force (with gravity), pair torque, wall force, wall torque of the
  * sorted state, N x 3 floats each; sorted_pid receives the P_ID order. */
+/* One bdem_update_sim that also reports the contacts the step kernel used
+ * (the reference's Broad_List / NN_NUM format: 32 slots per particle in
+ * sorted order, full count), for parity checks of the skin lists.  No
+ * reference counterpart. */
+int bdem_debug_step_contacts(bdem_ctx* ctx, unsigned int* list32, unsigned int* nn);
 int bdem_debug_forces(bdem_ctx* ctx, float* fpp, float* tpp, float* fw, float* tw, int* sorted_pid);
 
 #ifdef __cplusplus

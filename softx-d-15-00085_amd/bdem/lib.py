@@ -77,6 +77,9 @@ _PROTOS = {
     "bdem_debug_cells": (C.c_int, [_P, _up, _up]),
     "bdem_debug_contacts": (C.c_int, [_P, _up, _up]),
     "bdem_debug_forces": (C.c_int, [_P, _fp, _fp, _fp, _fp, _ip]),
+    "bdem_debug_step_contacts": (C.c_int, [_P, _up, _up]),
+    "bdem_set_skin": (C.c_int, [_P, C.c_float]),
+    "bdem_get_skin": (C.c_float, [_P]),
     "bdem_derive_spring_constants": (C.c_int, [C.POINTER(T.POBJ), C.c_int, C.c_int, _dp, _dp, _dp,
                                                C.c_double, C.c_double, C.c_double]),
 }
@@ -256,7 +259,7 @@ class Device:
         self._check(self.lib.bdem_drum_rotation(self.h, 1 if on else 0))
 
     # -- measurement ------------------------------------------------------
-    STAGES = ("sort", "step")
+    STAGES = ("sort", "lists", "step")
 
     def timing_begin(self, max_steps):
         self._check(self.lib.bdem_timing_begin(self.h, int(max_steps)))
@@ -292,6 +295,20 @@ class Device:
         nn = np.zeros(self.n, np.uint32)
         self._check(self.lib.bdem_debug_contacts(self.h, _ptr(lst, C.c_uint), _ptr(nn, C.c_uint)))
         return lst.reshape(self.n, 32), nn
+
+    def debug_step_contacts(self):
+        """One step, returning the contacts the step kernel used (32 slots per
+        particle in sorted order, full counts)."""
+        lst = np.zeros(self.n * 32, np.uint32)
+        nn = np.zeros(self.n, np.uint32)
+        self._check(self.lib.bdem_debug_step_contacts(self.h, _ptr(lst, C.c_uint), _ptr(nn, C.c_uint)))
+        return lst.reshape(self.n, 32), nn
+
+    def set_skin(self, skin):
+        self._check(self.lib.bdem_set_skin(self.h, float(skin)))
+
+    def skin(self):
+        return float(self.lib.bdem_get_skin(self.h))
 
     def debug_forces(self):
         out = [np.zeros((self.n, 3), np.float32) for _ in range(4)]

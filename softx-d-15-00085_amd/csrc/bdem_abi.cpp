@@ -49,16 +49,22 @@ struct bdem_ctx {
   int sorted_par = 0;        /* keys_sorted[sorted_par] = latest sorted keys */
   int append_par = 0;        /* meta[META_APPEND + append_par] counts the changed list */
   bool full_next = false;    /* a discharge check ran: many keys changed at once */
+  bool lists_full = true;    /* every stencil row is tested in full at the next step */
+  int vpar = 0;              /* skin-mask epoch words of the last step */
+  float skin_req = -1.0f;    /* requested skin (cm); negative or 0: masks off */
+  uint32_t* dbg_list = nullptr;  /* debug: the contacts the step kernel used */
+  uint32_t* dbg_nn = nullptr;
   int stepc = 0;
   /* stage timing: one event per stage boundary per recorded step */
   std::vector<hipEvent_t> tev;
+  std::vector<char> t_lists;   /* per recorded step: the lists boundary was recorded */
   int t_cap = 0, t_used = 0;
 };
 
 /* Stage boundaries recorded per step when timing is on:
  * 0 start | sort (hash if needed, resort or full sort, gather, cell table)
- * | step (fused kernel) */
-static const int BDEM_NSTAGE = 2;
+ * | lists (skin-list marking and rebuilding) | step (fused kernel) */
+static const int BDEM_NSTAGE = 3;
 
 namespace {
 
@@ -266,6 +272,47 @@ int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
   return BDEM_OK;
 }
 
+/* Skin masks for the sphere step: allowed when the hash is injective
+ * (Nx == Nz, so cells can be recovered from keys); the buffers are
+ * allocated on first use. */
+int apply_skin(bdem_ctx* c) {
+  StepBuffers& b = c->b;
+  float skin = c->skin_req > 0.0f ? c->skin_req : 0.0f;   /* off unless asked for */
+  if (c->poly || c->sim.num_NNCells.x != c->sim.num_NNCells.z || !(skin > 0.0f)) skin = 0.0f;
+  if (skin > 0.0f && !b.vmeta) {
+    int rc;
+    const size_t n = (size_t)b.n;
+    if ((rc = dalloc(c, &b.mask_cur, n)) != BDEM_OK) return rc;
+    if ((rc = dalloc(c, &b.mask_srt, n)) != BDEM_OK) return rc;
+    if ((rc = dalloc(c, &b.inv_map, n)) != BDEM_OK) return rc;
+    if ((rc = dalloc(c, &b.rowflag, n)) != BDEM_OK) return rc;
+    if ((rc = dalloc(c, &b.vmeta, VM_WORDS)) != BDEM_OK) return rc;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(b.mask_cur), 2 * n, 0u, c->stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(b.mask_srt), 2 * n, 0u, c->stream);
+    launch_fill_u32(b.inv_map, n, 0u, c->stream);
+    launch_fill_u32(b.rowflag, n, 0u, c->stream);
+    launch_fill_u32(b.vmeta, VM_WORDS, 0u, c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  b.skin = skin;
+  c->lists_full = true;
+  return BDEM_OK;
+}
+
+SkinArgs skin_args(bdem_ctx* c) {
+  const StepBuffers& b = c->b;
+  SkinArgs a{};
+  a.on = b.skin > 0.0f ? 1 : 0;
+  if (!a.on) return a;
+  a.mask_in = b.mask_srt;
+  a.mask_out = b.mask_cur;
+  a.rowflag = b.rowflag;
+  a.vmeta = b.vmeta;
+  a.skin = b.skin;
+  a.par = c->vpar ^ 1;
+  return a;
+}
+
 int require(bdem_ctx* c) {
   if (!c) return BDEM_EINVAL;
   if (!c->configured) return fail(c, BDEM_ESTATE, "bdem_set_simulation_data has not been called");
@@ -320,6 +367,7 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   c->keys_valid = false;
+  c->lists_full = true;
   return BDEM_OK;
 }
 
@@ -376,12 +424,16 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   a.quat_srt = c->orient ? b.quat_srt : nullptr;
   a.hwc_cur = b.hwc_cur;
   a.hwc_srt = b.hwc_srt;
+  a.inv_map = b.skin > 0.0f ? b.inv_map : nullptr;
+  a.mask_cur = b.skin > 0.0f ? b.mask_cur : nullptr;
+  a.mask_srt = b.skin > 0.0f ? b.mask_srt : nullptr;
   hipError_t e = launch_sort(a, incremental, c->append_par, c->stream);
   if (ev) (void)hipEventRecord(ev[1], c->stream);
   if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("sort launch: ") + hipGetErrorString(e));
   c->sorted_par = next;
   c->append_par ^= 1;
   c->full_next = false;
+  if (!incremental) c->lists_full = true;   /* masks were not carried through: test every row */
   /* the step kernel must produce new keys and flags before the next sort */
   c->keys_valid = false;
   c->flags_valid = false;
@@ -531,6 +583,9 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   c->configured = true;
   if ((rc = upload_params(c)) != BDEM_OK) return rc;
   if ((rc = ensure_counts(c)) != BDEM_OK) return rc;
+  c->skin_req = -1.0f;
+  c->vpar = 0;
+  if ((rc = apply_skin(c)) != BDEM_OK) return rc;
 
   /* Set_Init_Config_Device (GPU_Memory_Transfer.cuh:205-271): velocity =
    * InitVel[0], angular velocity = 0.  InitVel[0] is init->velocity[0] for a
@@ -595,6 +650,8 @@ int bdem_update_simulation_info(bdem_ctx* c, const bdem_SimulationInfo* sim) {
                          std::memcmp(&sim->cellSize, &o.cellSize, sizeof(bdem_float3)) == 0;
   c->sim = *sim;
   if (!same_grid) c->keys_valid = false;
+  int rc2 = apply_skin(c);   /* the default skin follows the cell size; lists rebuilt */
+  if (rc2) return rc2;
   return upload_params(c);
 }
 
@@ -644,6 +701,7 @@ int bdem_set_radii(bdem_ctx* c, const float* radius, const float* mass) {
   }
   HIPCHK(c, hipMemcpy(c->b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  c->lists_full = true;   /* list membership depends on the radii */
   return BDEM_OK;
 }
 
@@ -854,15 +912,25 @@ int bdem_update_sim(bdem_ctx* c) {
     if (c->stepc + 1 >= interval) { do_kill = true; c->stepc = 0; }
     else c->stepc += 1;
   }
+  const SkinArgs sa = skin_args(c);
+  if (sa.on) {
+    launch_skin(c->dp, c->b, c->b.keys_sorted[c->sorted_par ^ 1], sa.par, c->lists_full, c->stream);
+    c->lists_full = false;
+  }
+  /* the lists boundary only when there was a lists stage (an event between
+   * two dependent launches costs a few microseconds) */
+  if (ev && sa.on) (void)hipEventRecord(ev[2], c->stream);
+  if (ev) c->t_lists[(size_t)c->t_used] = sa.on ? 1 : 0;
   if (c->poly)
     launch_poly_step(c->dp, c->dpobj, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
                      c->record_wf, c->stream);
   else
     launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
-                c->append_par, do_kill, c->record_wf, c->stream);
+                c->append_par, do_kill, c->record_wf, sa, c->dbg_list, c->dbg_nn, c->stream);
+  if (sa.on) c->vpar = sa.par;
   if (do_kill) c->full_next = true;
   if (ev) {
-    (void)hipEventRecord(ev[2], c->stream);
+    (void)hipEventRecord(ev[3], c->stream);
     c->t_used += 1;
   }
   HIPCHK(c, hipGetLastError());
@@ -896,6 +964,7 @@ int bdem_timing_begin(bdem_ctx* c, int max_steps) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   c->tev.assign((size_t)max_steps * (BDEM_NSTAGE + 1), nullptr);
+  c->t_lists.assign((size_t)max_steps, 0);
   for (auto& e : c->tev) HIPCHK(c, hipEventCreate(&e));
   c->t_cap = max_steps;
   c->t_used = 0;
@@ -910,11 +979,17 @@ int bdem_timing_end(bdem_ctx* c, double* stage_ms, int* nsteps) {
   double acc[BDEM_NSTAGE] = {};
   for (int s = 0; s < c->t_used; ++s) {
     hipEvent_t* ev = &c->tev[(size_t)s * (BDEM_NSTAGE + 1)];
-    for (int k = 0; k < BDEM_NSTAGE; ++k) {
-      float ms = 0.f;
-      HIPCHK(c, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
-      acc[k] += ms;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    acc[0] += ms;
+    if (c->t_lists[(size_t)s]) {
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[1], ev[2]));
+      acc[1] += ms;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[2], ev[3]));
+    } else {
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[1], ev[3]));
     }
+    acc[2] += ms;
   }
   if (stage_ms) for (int k = 0; k < BDEM_NSTAGE; ++k) stage_ms[k] = acc[k];
   if (nsteps) *nsteps = c->t_used;
@@ -1014,6 +1089,49 @@ int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw,
     for (int i = 0; i < n; ++i) sorted_pid[i] = f_bits(O[i].w) >> 3;
   }
   c->keys_valid = false;
+  return BDEM_OK;
+}
+
+int bdem_set_skin(bdem_ctx* c, float skin) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!(skin == skin)) return fail(c, BDEM_EINVAL, "skin must be a number");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->skin_req = skin;
+  return apply_skin(c);
+}
+
+float bdem_get_skin(const bdem_ctx* c) { return (c && c->configured) ? c->b.skin : 0.0f; }
+
+int bdem_debug_step_contacts(bdem_ctx* c, unsigned int* list32, unsigned int* nn) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!list32 || !nn) return fail(c, BDEM_EINVAL, "null output");
+  if (c->poly) return fail(c, BDEM_ENOTSUP, "the polyhedra step keeps its list in its own buffers (bdem_debug_contacts)");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int n = c->b.n;
+  uint32_t *dl = nullptr, *dn = nullptr;
+  HIPCHK(c, hipMalloc((void**)&dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t)));
+  if (hipMalloc((void**)&dn, (size_t)n * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipFree(dl);
+    return fail(c, BDEM_ENOMEM, "debug buffer");
+  }
+  launch_fill_u32(dl, (size_t)n * LIST_STRIDE, 0u, c->stream);
+  launch_fill_u32(dn, (size_t)n, 0u, c->stream);
+  c->dbg_list = dl;
+  c->dbg_nn = dn;
+  rc = bdem_update_sim(c);
+  c->dbg_list = nullptr;
+  c->dbg_nn = nullptr;
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(list32, dl, (size_t)n * LIST_STRIDE * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(nn, dn, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(dl);
+  (void)hipFree(dn);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("debug_step_contacts: ") + hipGetErrorString(e));
   return BDEM_OK;
 }
 

@@ -175,10 +175,29 @@ __device__ __forceinline__ bool overlaps(f3 pa, float ra, float4 pb) {
   return hit;
 }
 
-template <class Visit>
+/* The contact predicate of the reference (exact, above) and the membership
+ * predicate of the skin lists: |pa - pb| < ra + rb + skin, decided on the
+ * safe side (every pair that close is included, a few slightly farther may
+ * be: their contact is still tested exactly each step).  With
+ * d2 = fl(dot(d,d)) <= D^2 (1 + 2^-22) and s = fl(fl(ra+rb) + skin) >=
+ * S (1 - 2^-23), comparing against fl(s*s) (1 + 2^-19) includes every D < S. */
+struct OverlapTest {
+  __device__ __forceinline__ bool operator()(f3 pa, float ra, float4 pb) const { return overlaps(pa, ra, pb); }
+};
+struct SkinTest {
+  float skin;
+  __device__ __forceinline__ bool operator()(f3 pa, float ra, float4 pb) const {
+    const f3 d = pa - xyz(pb);
+    const float d2 = dot(d, d);
+    const float s = (ra + pb.w) + skin;
+    return d2 < (s * s) * 1.0000019073486328125f;   /* 1 + 2^-19 */
+  }
+};
+
+template <class Visit, class Pred = OverlapTest>
 __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2* __restrict__ cells,
                                                 const float4* __restrict__ pos_r, int i, f3 pa,
-                                                float ra, Visit&& visit) {
+                                                float ra, Visit&& visit, Pred pred = Pred()) {
   int g[3];
   grid_pos(P, pa, g);
   const int xs = max(g[0] - 1, 0), xe = min(g[0] + 1, P.nx - 1);
@@ -203,7 +222,7 @@ __device__ __forceinline__ int for_each_contact(const DevParams& P, const uint2*
       for (uint32_t j = lo; j < hi; ++j) {
         if ((int)j == i) continue;
         const float4 pb = pos_r[j];
-        if (overlaps(pa, ra, pb)) {
+        if (pred(pa, ra, pb)) {
           visit((int)j, pb, count);
           ++count;
         }
@@ -279,23 +298,96 @@ __device__ __forceinline__ RowStage row_stage(uint2 e0, uint2 e1, uint32_t n) {
 }
 
 /* Candidates [lo, hi) of one lane, read from `src` (LDS stage or global). */
-template <class Visit>
+template <class Visit, class Pred>
 __device__ __forceinline__ void scan_candidates(const float4* src, uint32_t lo, uint32_t hi, uint32_t shift,
-                                                int i, f3 pa, float ra, int& count, Visit& visit) {
+                                                int i, f3 pa, float ra, int& count, Visit& visit,
+                                                const Pred& pred) {
   for (uint32_t j = lo; j < hi; ++j) {
     const float4 pb = src[j - shift];
-    if ((int)j != i && overlaps(pa, ra, pb)) {
+    if ((int)j != i && pred(pa, ra, pb)) {
       visit((int)j, pb, count);
       ++count;
     }
   }
 }
 
-template <class Visit>
-__device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const uint2* __restrict__ cells,
-                                                     const float4* __restrict__ pos_r, uint32_t n, int i,
-                                                     f3 pa, float ra, bool active, WaveStage& st,
-                                                     Visit&& visit) {
+/* How one lane scans the candidate run [lo, hi) of stencil row r
+ * (r = 3 (dy + 1) + (dz + 1)): every candidate with a predicate. */
+template <class Pred>
+struct PlainRows {
+  Pred pred;
+  template <class Visit>
+  __device__ __forceinline__ void row(const float4* src, uint32_t lo, uint32_t hi, uint32_t shift, int, int i,
+                                      f3 pa, float ra, int& count, Visit& visit) {
+    scan_candidates(src, lo, hi, shift, i, pa, ra, count, visit, pred);
+  }
+  __device__ __forceinline__ Pred lane_pred() { return pred; }
+};
+
+/* Skin masks.  Bits 7r .. 7r+6 of `mask` mark which of the first seven
+ * candidates lo .. lo+6 of row r were within ra + rb + skin when the row was
+ * last tested in full; candidates beyond the seventh are always tested.
+ * While no particle enters or leaves the row's three cells the row's run
+ * holds the same particles in the same order (the re-sort is stable), so
+ * the bits keep their meaning however the run shifts in the sorted arrays;
+ * and while the epoch's displacement bound holds (twice it below the skin)
+ * no unmarked candidate can have come into contact.  Rows in `full_rows`
+ * are tested in full and their bits rebuilt.  Contacts are visited in the
+ * run's order either way. */
+struct MaskedRows {
+  uint64_t mask;
+  uint32_t full_rows;
+  float skin;
+  template <class Visit>
+  __device__ __forceinline__ void row(const float4* src, uint32_t lo, uint32_t hi, uint32_t shift, int r, int i,
+                                      f3 pa, float ra, int& count, Visit& visit) {
+    const uint32_t sh = 7u * (uint32_t)r;
+    if ((full_rows >> r) & 1u) {
+      const SkinTest near{skin};
+      uint64_t nb = 0ull;
+      for (uint32_t j = lo; j < hi; ++j) {
+        const float4 pb = src[j - shift];
+        if ((int)j == i) continue;
+        if (j - lo < 7u && near(pa, ra, pb)) nb |= 1ull << (j - lo);
+        if (overlaps(pa, ra, pb)) {
+          visit((int)j, pb, count);
+          ++count;
+        }
+      }
+      mask |= nb << sh;   /* the row's field was cleared when it was marked full */
+    } else {
+      uint32_t bits = (uint32_t)(mask >> sh) & 0x7Fu;
+      while (bits) {
+        const uint32_t j = lo + (uint32_t)__builtin_ctz(bits);
+        bits &= bits - 1u;
+        if (j >= hi || (int)j == i) continue;
+        const float4 pb = src[j - shift];
+        if (overlaps(pa, ra, pb)) {
+          visit((int)j, pb, count);
+          ++count;
+        }
+      }
+      for (uint32_t j = lo + 7u; j < hi; ++j) {
+        const float4 pb = src[j - shift];
+        if ((int)j != i && overlaps(pa, ra, pb)) {
+          visit((int)j, pb, count);
+          ++count;
+        }
+      }
+    }
+  }
+  /* a lane scanned on its own (outside the grid): no mask */
+  __device__ __forceinline__ OverlapTest lane_pred() {
+    mask = 0ull;
+    return OverlapTest();
+  }
+};
+
+template <class Visit, class Rows>
+__device__ __forceinline__ int for_each_contact_wave_rows(const DevParams& P, const uint2* __restrict__ cells,
+                                                          const float4* __restrict__ pos_r, uint32_t n, int i,
+                                                          f3 pa, float ra, bool active, WaveStage& st,
+                                                          Visit& visit, Rows& rows) {
   const int lane = threadIdx.x & 63;
   int g[3];
   grid_pos(P, pa, g);
@@ -369,9 +461,9 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
           /* a consistent cell table puts every lane's run inside the stage;
            * checked, so LDS is never indexed outside it whatever the table holds */
           if (sc.staged && lo >= sc.pmin && hi <= sc.pmax)
-            scan_candidates(st.pos, lo, hi, sc.pmin, i, pa, ra, count, visit);
+            rows.row(st.pos, lo, hi, sc.pmin, r, i, pa, ra, count, visit);
           else
-            scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+            rows.row(pos_r, lo, hi, 0u, r, i, pa, ra, count, visit);
         }
         __builtin_amdgcn_wave_barrier();
         ce0 = ne0; ce1 = ne1; ne0 = nn0; ne1 = nn1;
@@ -406,10 +498,10 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
           if (pmax - pmin <= (uint32_t)STAGE_POS) {
             for (uint32_t c = lane; c < pmax - pmin; c += 64) st.pos[c] = pos_r[pmin + c];
             __builtin_amdgcn_wave_barrier();
-            if (lo >= pmin && hi <= pmax) scan_candidates(st.pos, lo, hi, pmin, i, pa, ra, count, visit);
-            else scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+            if (lo >= pmin && hi <= pmax) rows.row(st.pos, lo, hi, pmin, (dy + 1) * 3 + (dz + 1), i, pa, ra, count, visit);
+            else rows.row(pos_r, lo, hi, 0u, (dy + 1) * 3 + (dz + 1), i, pa, ra, count, visit);
           } else {
-            scan_candidates(pos_r, lo, hi, 0u, i, pa, ra, count, visit);
+            rows.row(pos_r, lo, hi, 0u, (dy + 1) * 3 + (dz + 1), i, pa, ra, count, visit);
           }
           __builtin_amdgcn_wave_barrier();
         }
@@ -417,8 +509,17 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
     }
   }
   /* a live particle outside the grid (key C): the per-lane traversal */
-  if (active && !ingrid) count = for_each_contact(P, cells, pos_r, i, pa, ra, visit);
+  if (active && !ingrid) count = for_each_contact(P, cells, pos_r, i, pa, ra, visit, rows.lane_pred());
   return count;
+}
+
+template <class Visit, class Pred = OverlapTest>
+__device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const uint2* __restrict__ cells,
+                                                     const float4* __restrict__ pos_r, uint32_t n, int i,
+                                                     f3 pa, float ra, bool active, WaveStage& st,
+                                                     Visit&& visit, Pred pred = Pred()) {
+  PlainRows<Pred> rows{pred};
+  return for_each_contact_wave_rows(P, cells, pos_r, n, i, pa, ra, active, st, visit, rows);
 }
 
 

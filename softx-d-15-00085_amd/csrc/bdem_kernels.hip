@@ -226,8 +226,9 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
 /* Fused step: contacts -> pair forces (+gravity) -> walls -> integrate ->    */
 /* discharge check -> next key.  Reads the sorted scratch, writes the state.  */
 /* ------------------------------------------------------------------------ */
-template <int LAW, bool ORIENT>
+template <int LAW, bool ORIENT, bool SKIN>
 __global__ __launch_bounds__(256) void k_step(
+    SkinArgs sk, uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
     const DevParams* __restrict__ Pp, int n, const uint2* __restrict__ cells,
     const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
     const float4* __restrict__ omg_in, const float4* __restrict__ quat_in,
@@ -259,11 +260,34 @@ __global__ __launch_bounds__(256) void k_step(
   const float r = pr.w, m = vm.w;
 
   /* phase 1: contact partners in the reference order, into this thread's list */
-  const int nc = for_each_contact_wave(
-      P, cells, pos_in, (uint32_t)n, i, p, r, alive, *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]),
-      [&](int j, float4, int k) {
-        if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
-      });
+  auto record = [&](int j, float4, int k) {
+    if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
+    if (dbg_list && k < LIST_STRIDE) dbg_list[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
+  };
+  int nc = 0;
+  uint64_t new_mask = 0ull;
+  if (SKIN) {
+    /* skin masks: rows flagged (or all, at an epoch start) are tested in
+     * full and re-masked, the others only at their masked candidates */
+    const bool full = sk.vmeta[VM_FULL] != 0u;
+    uint32_t flags = 0x1FFu;
+    if (in && !full) {
+      flags = sk.rowflag[i];
+      if (flags) sk.rowflag[i] = 0u;
+    }
+    uint64_t m0 = in ? sk.mask_in[i] : 0ull;
+#pragma unroll
+    for (int rr = 0; rr < 9; ++rr)
+      if ((flags >> rr) & 1u) m0 &= ~(0x7Full << (7 * rr));
+    MaskedRows rows{m0, flags, sk.skin};
+    nc = for_each_contact_wave_rows(P, cells, pos_in, (uint32_t)n, i, p, r, alive,
+                                    *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]), record, rows);
+    new_mask = rows.mask;
+  } else {
+    nc = for_each_contact_wave(P, cells, pos_in, (uint32_t)n, i, p, r, alive,
+                               *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]), record);
+  }
+  if (dbg_nn && in) dbg_nn[i] = (uint32_t)nc;
   __builtin_amdgcn_wave_barrier();
   /* phase 2: gravity first (DEM_Kernels_Compute.cuh:654), then the contacts */
   f3 Fpp = alive ? cvt(P.gravity) * m : mk3(0.f, 0.f, 0.f);
@@ -273,6 +297,7 @@ __global__ __launch_bounds__(256) void k_step(
                         *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp);
 
   bool chg = false;
+  float dmove = 0.0f;
   if (alive) {
     const bool hist_slot = LAW != BDEM_LAW_REFERENCE && pid < hist_cap;
     if (nc > CLIST) {
@@ -284,6 +309,7 @@ __global__ __launch_bounds__(256) void k_step(
       }
       for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
         if (k >= CLIST) particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc);
+        if (dbg_list && k >= CLIST && k < LIST_STRIDE) dbg_list[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
       });
       atomicOr(status, ST_LIST_OVERFLOW);
     }
@@ -320,6 +346,11 @@ __global__ __launch_bounds__(256) void k_step(
     QuatF q{1.f, 0.f, 0.f, 0.f};
     if (ORIENT) { const float4 q4 = quat_in[i]; q = QuatF{q4.x, q4.y, q4.z, q4.w}; }
     integrate(P, inv_inertia, m, Fpp, Tpp, Fw, Tw, Fl, Tl, p, v, w, ORIENT ? &q : nullptr);
+    if (SKIN) {
+      /* |p_new - p_old| <= sum |v dt| (1 + 2^-20) + sum |p| 2^-22 (rounding of p += v dt) */
+      dmove = ((fabsf(v.x) + fabsf(v.y) + fabsf(v.z)) * P.dt) * 1.00000095367431640625f +
+              (fabsf(p.x) + fabsf(p.y) + fabsf(p.z)) * 2.384185791015625e-07f;
+    }
 
     /* a11: Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
     if (do_kill && p.y < P.kill_height) {
@@ -331,12 +362,14 @@ __global__ __launch_bounds__(256) void k_step(
     vel_out[i] = make_float4(v.x, v.y, v.z, m);
     omg_out[i] = make_float4(w.x, w.y, w.z, __int_as_float(pack_id_type(pid, ta)));
     if (ORIENT) quat_out[i] = make_float4(q.w, q.x, q.y, q.z);
+    if (SKIN) sk.mask_out[i] = new_mask;
   } else if (in) {
     /* dead: inert, copied through */
     pos_out[i] = pr;
     vel_out[i] = vm;
     omg_out[i] = ot;
     if (ORIENT) quat_out[i] = quat_in[i];
+    if (SKIN) sk.mask_out[i] = 0ull;
     if (hwc_out) hwc_out[i] = hwc;
     if (wall_force) { wall_force[3 * i] = 0.f; wall_force[3 * i + 1] = 0.f; wall_force[3 * i + 2] = 0.f; }
   }
@@ -346,6 +379,12 @@ __global__ __launch_bounds__(256) void k_step(
     keys_out[i] = new_key;
     chg = new_key != sorted_keys[i];
     changed[i] = chg ? 1 : 0;
+  }
+  if (SKIN) {
+    /* the step's largest displacement bound, for the next step's epoch test */
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dmove = fmaxf(dmove, __shfl_xor(dmove, o, 64));
+    if ((threadIdx.x & 63) == 0 && dmove > 0.0f) atomicMax(&sk.vmeta[VM_DMAX + sk.par], __float_as_uint(dmove));
   }
   /* append changed slots for the next incremental sort (one atomic per wave) */
   const uint64_t mask = __ballot(chg);
@@ -364,27 +403,37 @@ __global__ __launch_bounds__(256) void k_step(
   }
 }
 
-template <int LAW, bool ORIENT>
+template <int LAW, bool ORIENT, bool SKIN>
 static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
-                          bool do_kill, bool want_wf, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<LAW, ORIENT>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n,
+                          bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dbg_list, uint32_t* dbg_nn,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((k_step<LAW, ORIENT, SKIN>), dim3((b.n + 255) / 256), dim3(256), 0, s, sa, dbg_list, dbg_nn,
+                     dP, b.n,
                      b.cells, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
                      b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta + META_APPEND + ap,
                      b.hwc_srt, b.hwc_cur, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp], b.wrec[hp ^ 1],
                      b.hist_cap, want_wf ? b.wall_force : nullptr, do_kill ? 1 : 0, b.status, b.kill_count);
 }
 
+template <int LAW, bool ORIENT>
+static void launch_step_s(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
+                          bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dl, uint32_t* dn, hipStream_t s) {
+  if (sa.on) launch_step_t<LAW, ORIENT, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+  else launch_step_t<LAW, ORIENT, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+}
+
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int law, bool orient,
-                 int hp, int ap, bool do_kill, bool want_wf, hipStream_t s) {
+                 int hp, int ap, bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dl, uint32_t* dn,
+                 hipStream_t s) {
   if (law == BDEM_LAW_REFERENCE) {
-    if (orient) launch_step_t<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    if (orient) launch_step_s<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
   } else if (law == BDEM_LAW_LSD_COULOMB) {
-    if (orient) launch_step_t<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    if (orient) launch_step_s<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
   } else {
-    if (orient) launch_step_t<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, s);
-    else launch_step_t<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, s);
+    if (orient) launch_step_s<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
   }
 }
 

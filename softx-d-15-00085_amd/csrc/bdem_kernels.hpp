@@ -37,6 +37,26 @@ struct StepBuffers {
   float *wall_force;                   /* optional n*3 output of the wall force */
   uint32_t *status;                    /* ST_* bits */
   int *kill_count;
+  /* skin masks (spheres): which near candidates of each stencil row are
+   * within ra + rb + skin; they travel with the state like the spring counts */
+  uint64_t *mask_cur, *mask_srt;
+  uint32_t *inv_map;                   /* previous sorted position -> new (written by the resort) */
+  uint32_t *rowflag;                   /* per sorted position: 9 bits, rows to test in full this step */
+  uint32_t *vmeta;                     /* VM_* words */
+  float skin;                          /* 0: masks off (every candidate tested every step) */
+};
+
+enum { VM_DMAX = 1, VM_U = 3, VM_FULL = 5, VM_WORDS = 8 };
+
+/* What the step kernel needs of the skin masks; par alternates per step. */
+struct SkinArgs {
+  const uint64_t* mask_in;             /* this step's masks, sorted order */
+  uint64_t* mask_out;                  /* written with the state, for the next step */
+  uint32_t* rowflag;
+  uint32_t* vmeta;
+  float skin;
+  int par;
+  int on;
 };
 
 /* Polyhedra: world-frame shapes, one record of `stride` float4 per particle
@@ -94,6 +114,9 @@ struct SortArgs {
   float4 *pos_srt, *vel_srt, *omg_srt, *quat_srt;
   const uint32_t* hwc_cur;             /* history counts, gathered with the state when set */
   uint32_t* hwc_srt;
+  uint32_t* inv_map;                   /* optional: previous position -> new position */
+  const uint64_t* mask_cur;            /* skin masks, gathered with the state when set */
+  uint64_t* mask_srt;
 };
 
 /* Individual stages (all asynchronous on `s`). */
@@ -109,6 +132,11 @@ hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t
  * meta[META_APPEND + append_par]. */
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
                  bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force,
+                 const SkinArgs& sk, uint32_t* dbg_list, uint32_t* dbg_nn, hipStream_t s);
+/* Skin masks before a step: decide a new epoch (every row tested in full)
+ * from the displacement bound, else flag the stencil rows whose cells a
+ * particle entered or left.  Uses this step's cell table; par alternates. */
+void launch_skin(const DevParams* dP, const StepBuffers& b, const uint32_t* old_keys, int par, bool full,
                  hipStream_t s);
 
 /* Polyhedra step (rotate -> list + queue -> pairs -> finish), srt -> cur,

@@ -316,11 +316,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_resort(SortArgs a, int par) {
     if (pos >= (uint32_t)n) { atomicOr(a.status, ST_SORT_ERROR); continue; }
     a.keys_sorted[pos] = ok[it];
     a.idx_sorted[pos] = s;
+    if (a.inv_map) a.inv_map[s] = pos;
     a.pos_srt[pos] = a.pos_cur[s];
     a.vel_srt[pos] = a.vel_cur[s];
     a.omg_srt[pos] = a.omg_cur[s];
     if (a.quat_cur) a.quat_srt[pos] = a.quat_cur[s];
     if (a.hwc_cur) a.hwc_srt[pos] = a.hwc_cur[s];
+    if (a.mask_cur) a.mask_srt[pos] = a.mask_cur[s];
   }
   for (uint32_t r = tid; r < nl; r += RS_THREADS) {
     const unsigned long long c = big ? gl[r] : Ls[r];
@@ -335,11 +337,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_resort(SortArgs a, int par) {
     if (pos >= (uint32_t)n || s >= (uint32_t)n) { atomicOr(a.status, ST_SORT_ERROR); continue; }
     a.keys_sorted[pos] = k;
     a.idx_sorted[pos] = (int)s;
+    if (a.inv_map) a.inv_map[s] = pos;
     a.pos_srt[pos] = a.pos_cur[s];
     a.vel_srt[pos] = a.vel_cur[s];
     a.omg_srt[pos] = a.omg_cur[s];
     if (a.quat_cur) a.quat_srt[pos] = a.quat_cur[s];
     if (a.hwc_cur) a.hwc_srt[pos] = a.hwc_cur[s];
+    if (a.mask_cur) a.mask_srt[pos] = a.mask_cur[s];
   }
   __syncthreads();   /* the block's keys_sorted writes are visible to the block */
 

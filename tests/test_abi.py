@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "bdem_abi.h")
 
 def header_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*|long long)\s+(bdem_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|float|const char\*|long long)\s+(bdem_\w+)\s*\(", text, re.M)))
 
 
 def test_struct_sizes_match_reference_layout():
