@@ -97,6 +97,8 @@ def main():
                     help="after the timed steps, run the library's plain copy kernel (k_copy_u4, "
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
+    ap.add_argument("--workload", choices=("silo", "poly"), default="silo",
+                    help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box)")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of settling "
                          "(tangential springs restart at zero)")
@@ -107,6 +109,8 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "poly":
+        return run_poly(args)
 
     import numpy as np
     import bdem
@@ -235,6 +239,84 @@ def main():
                    "changed_per_step": round(changed)},
         "roofline": roof,
         "step_roofline": step_roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_poly(args):
+    """Polyhedra throughput: 200,000 seeded convex hulls of 12 points
+    (20 triangular faces, 30 edges, circumradius ~0.48 cm) on a jittered
+    lattice in a closed box, random orientations, reference pair law; the
+    four polyhedra kernels (rotate, list + queue, pairs, finish) plus the
+    sort, timed per step."""
+    import numpy as np
+    import bdem
+    from bdem import scenarios as S
+
+    n = args.n if args.n != 1048576 else 200000
+    t0 = time.time()
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    spacing = 0.85
+    L = side * spacing + 2.5
+    deck = S.poly_deck(n, seed=71, L=L, shape="hull", size=1.0, spacing=spacing, vmax=10.0, wmax=5.0)
+    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"])
+    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"],
+                  quat=deck["quat"])
+    ncells = dev.num_cells()
+    _, nn = dev.debug_contacts()
+    k = float(nn.mean())
+    log(f"[bench] poly n={n} box {L:.1f} cm cells={ncells} candidates/particle {k:.2f} "
+        f"setup {time.time() - t0:.1f}s")
+    dev.step(args.warmup)
+    dev.sync()
+    dev.timing_begin(args.steps)
+    ta = time.perf_counter()
+    dev.step(args.steps)
+    dev.sync()
+    tb = time.perf_counter()
+    stages, nrec = dev.timing_end()
+    elapsed = tb - ta
+    ms_per_step = elapsed * 1e3 / args.steps
+    st = dev.get_state()
+    n_live = int(np.count_nonzero(st["pid"] >= 0))
+    finite = bool(np.all(np.isfinite(st["pos"])))
+    status = dev.status()
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step; stage ms/step "
+        + ", ".join(f"{s_}={v / nrec:.4f}" for s_, v in stages.items())
+        + f"; status {status}; finite {finite}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_py import Oracle
+        ora = Oracle(deck, bits=32)
+        ora.time_steps(1)
+        steps = 3
+        el = ora.time_steps(steps)
+        ora.close()
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = dict(value=n * steps / el, unit="particle-steps/s", cores=cores, kind="port",
+                   sample=f"{steps} steps (after 1 untimed) of the same {n}-hull deck from its initial "
+                          f"state, FP32 OpenMP restatement")
+        log(f"[bench] CPU baseline {cpu['value']:.3e} particle-steps/s on {cores} threads")
+    dev.close()
+    out = {
+        "metric": "particle-steps/s, convex polyhedra (reference law)",
+        "value": round(n_live * args.steps / elapsed, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n} convex hulls of 12 seeded points (20 faces, 30 edges), jittered "
+                               f"lattice {spacing} cm in a closed {L:.1f} cm box, dt 1e-5 s",
+                   "particles": n_live, "cells": ncells, "candidates_per_particle": round(k, 3),
+                   "stage_ms_per_step": {s_: round(v / nrec, 5) for s_, v in stages.items()}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

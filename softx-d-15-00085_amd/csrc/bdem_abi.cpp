@@ -570,7 +570,8 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     ALLOC(pb.list, (size_t)n * LIST_STRIDE);
     ALLOC(pb.nn, (size_t)n);
     ALLOC(pb.queue, (size_t)n * LIST_STRIDE);
-    ALLOC(pb.qcount, 1);
+    ALLOC(pb.equeue, (size_t)n * LIST_STRIDE);
+    ALLOC(pb.qcount, 2);
     ALLOC(pb.pf, (size_t)n * LIST_STRIDE);
     ALLOC(pb.pt, (size_t)n * LIST_STRIDE);
   }

@@ -69,7 +69,8 @@ struct PolyBuffers {
   uint32_t* list;                      /* n * 32 partner slots */
   uint32_t* nn;                        /* full contact count per particle */
   uint32_t* queue;                     /* (particle * 32 + k) work items */
-  uint32_t* qcount;                    /* queue length */
+  uint32_t* equeue;                    /* the items that need the edge-edge search */
+  uint32_t* qcount;                    /* [0] queue length, [1] edge queue length */
   float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
 };
 constexpr int POLY_PAIR_BLOCKS = 2048;

@@ -3,7 +3,7 @@
  *
  * Kernel_ParticleInteraction_Polyhedra_NonSymmetry (DEM_Kernels_Compute.cuh:
  * 736-992) rotates both shapes for every listed pair and tests one pair per
- * particle thread in a serial loop.  Here a step is four launches:
+ * particle thread in a serial loop.  Here a step is five launches:
  *
  *   k_poly_rotate   one thread per (particle, vertex / face-normal /
  *                   face-centroid slot): world-frame shapes into `rot`,
@@ -12,9 +12,12 @@
  *                   sphere broad phase), plus a queue of (particle, slot)
  *                   work items appended one atomic per wave
  *   k_poly_pairs    one thread per queued pair, grid-stride over the queue:
- *                   separating plane, vertex-face both ways, edge-edge, depth
- *                   clamps, pair law; the force and torque go to the pair's
- *                   own slot, so the queue order does not matter
+ *                   the separating-plane test; the pairs it does not
+ *                   separate go to a second queue
+ *   k_poly_narrow   one wave per pair of that queue: vertex-face both ways
+ *                   (a lane per vertex), edge-edge (lanes over edge pairs),
+ *                   depth clamps, pair law.  Results go to the pair's own
+ *                   slot, so the queue orders do not matter
  *   k_poly_finish   one thread per particle: gravity, then the slots in list
  *                   order (the reference's summation order), walls, the
  *                   tensor integrator and quaternion, discharge, next key,
@@ -147,26 +150,29 @@ __device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ?
  * (Collision_Detection_Polyhedra.cuh:1154-1630): vertices of A inside B; one
  * -> vertex-face, two -> edge-face, more -> face-face (the first eight
  * inside vertices: the reference's lists hold eight), with a ray test
- * choosing among the three closest faces. */
-__device__ __forceinline__ PColl vertex_faces(f3 PosA, const Shape& A, const Shape& B, const bdem_POBJ& pb) {
+ * choosing among the three closest faces.  A whole wave works on one pair:
+ * lane v < A.nv tests vertex v
+ * (Collision_Detection_Vertex_InPoly_3Faces); the inside vertices are then
+ * taken in ascending order, as the serial loop appends them, so the first
+ * two vertices, their closest faces and the ordered sum of the first eight
+ * are the serial ones, and the selection runs on wave-uniform values. */
+__device__ __forceinline__ PColl vertex_faces_wave(f3 PosA, const Shape& A, const Shape& B, const bdem_POBJ& pb) {
+  const int lane = threadIdx.x & 63;
   PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
-  int np = 0;
-  InPoly s0{}, t{};
-  int s1f1 = -1;
-  int v0 = -1, v1 = -1;
-  f3 sum = mk3(0.0f, 0.0f, 0.0f);
-  for (int v = 0; v < A.nv; ++v) {
-    const f3 pv = ld3(A.V + v);
-    if (!vertex_in_poly(PosA, pv, B, t)) continue;
-    if (np < 8) {
-      if (np == 0) { s0 = t; v0 = v; }
-      if (np == 1) { s1f1 = t.f1; v1 = v; }
-      sum = sum + pv;
-    }
-    ++np;
-  }
-  if (np == 0) return res;
-  np = np < 8 ? np : 8;
+  InPoly t{-1, -1, -1, 10.0f, 10.0f, 10.0f};
+  bool inside = false;
+  if (lane < A.nv) inside = vertex_in_poly(PosA, ld3(A.V + lane), B, t);
+  uint64_t mask = __ballot(inside);
+  const int np_all = __popcll(mask);
+  if (np_all == 0) return res;
+  const int v0 = __ffsll((unsigned long long)mask) - 1;
+  const uint64_t rest = mask & (mask - 1ull);
+  const int v1 = rest ? __ffsll((unsigned long long)rest) - 1 : v0;
+  InPoly s0;
+  s0.f0 = __shfl(t.f0, v0, 64); s0.f1 = __shfl(t.f1, v0, 64); s0.f2 = __shfl(t.f2, v0, 64);
+  s0.d0 = __shfl(t.d0, v0, 64); s0.d1 = __shfl(t.d1, v0, 64); s0.d2 = __shfl(t.d2, v0, 64);
+  const int s1f1 = __shfl(t.f1, v1, 64);
+  const int np = np_all < 8 ? np_all : 8;
   res.hit = true;
   if (np == 1) {
     res.cls = C_VF;
@@ -205,6 +211,13 @@ __device__ __forceinline__ PColl vertex_faces(f3 PosA, const Shape& A, const Sha
     return res;
   }
   res.cls = C_FF;
+  f3 sum = mk3(0.0f, 0.0f, 0.0f);
+  uint64_t m = mask;
+  for (int k = 0; k < np; ++k) {
+    const int v = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1ull;
+    sum = sum + ld3(A.V + v);
+  }
   res.cp = sum / (float)np;
   for (int q = 0; q < 3; ++q) {
     const int f = face_at(s0, q);
@@ -218,52 +231,78 @@ __device__ __forceinline__ PColl vertex_faces(f3 PosA, const Shape& A, const Sha
   return res;   /* face logic fall-through: depth 0, zero normal */
 }
 
-/* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707) */
-__device__ __forceinline__ PColl edges(const Shape& A, const bdem_POBJ& pa, const Shape& B, const bdem_POBJ& pb,
-                                       f3 RelPos, f3 PosB) {
-  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
-  float min_edgedis = 5.0f;
-  f3 selA = mk3(0.f, 0.f, 0.f), selB = selA;
-  bool found = false;
+/* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707),
+ * with a whole wave on one pair: lane l takes the
+ * edge pairs t = l, l + 64, ... of the sequence t = i * E_B + j that the
+ * serial loop walks.  The serial loop keeps the first strictly smaller
+ * distance below 5, i.e. the smallest distance and among equal distances
+ * the smallest t: a wave minimum over (distance, t) selects the same pair,
+ * whose points are then recomputed by every lane with the same arithmetic.
+ * The result is wave-uniform. */
+__device__ __forceinline__ PColl edges_wave(const Shape& A, const bdem_POBJ& pa, const Shape& B, const bdem_POBJ& pb,
+                                            f3 RelPos, f3 PosB) {
+  const int lane = threadIdx.x & 63;
   const int ea = pa.num_edges, eb = pb.num_edges;
-  for (int i = 0; i < ea; ++i) {
+  const int total = ea * eb;
+  float best = 5.0f;
+  int bt = 0x7FFFFFFF;
+  for (int t = lane; t < total; t += 64) {
+    const int i = t / eb, j = t - (t / eb) * eb;
     const f3 PA = ld3(A.V + pa.edge[i].point_vindex[0]);
     const f3 DA = ld3(A.V + pa.edge[i].point_vindex[1]) - PA;
     const float DEAD = dot(DA, DA);
-    for (int j = 0; j < eb; ++j) {
-      const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
-      const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
-      const f3 dir = cross(DA, DB);
-      const float mag = dot(dir, dir);
-      if (!(mag > 0.00000f)) continue;
-      f3 DV = PA - PB;
-      const float DEBD = dot(DB, DB);
-      const float DEABD = dot(DA, DB);
-      const float DEADAB = dot(DA, DV);
-      const float DEBDAB = dot(DB, DV);
-      const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
-      const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
-      if (!(dA >= -0.010f && dA <= 1.0100f)) continue;
-      const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
-      if (!(dB >= -0.0100f && dB <= 1.010f)) continue;
-      const f3 pA = PA + DA * dA;
-      const f3 pB = PB + DB * dB;
-      DV = (pA - pB);
-      float dis = dot(DV, DV);
-      if (dis > 0.00000f) {
-        dis = __builtin_sqrtf(dis);
-        if (dis < min_edgedis) {
-          found = true;
-          min_edgedis = dis;
-          res.cp = pA;
-          res.n = dir;
-          selA = pA;
-          selB = pB;
-        }
-      }
+    const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
+    const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
+    const f3 dir = cross(DA, DB);
+    const float mag = dot(dir, dir);
+    if (!(mag > 0.00000f)) continue;
+    const f3 DV = PA - PB;
+    const float DEBD = dot(DB, DB);
+    const float DEABD = dot(DA, DB);
+    const float DEADAB = dot(DA, DV);
+    const float DEBDAB = dot(DB, DV);
+    const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
+    const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+    if (!(dA >= -0.010f && dA <= 1.0100f)) continue;
+    const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+    if (!(dB >= -0.0100f && dB <= 1.010f)) continue;
+    const f3 pA = PA + DA * dA;
+    const f3 pB = PB + DB * dB;
+    const f3 D = (pA - pB);
+    float dis = dot(D, D);
+    if (dis > 0.00000f) {
+      dis = __builtin_sqrtf(dis);
+      if (dis < best) { best = dis; bt = t; }   /* t grows along the lane's walk: first wins */
     }
   }
-  if (!found) return res;
+  /* wave minimum of (distance, t) */
+  float m = best;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+  int tm = (best == m) ? bt : 0x7FFFFFFF;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tm = min(tm, __shfl_xor(tm, o, 64));
+  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
+  if (!(m < 5.0f) || tm == 0x7FFFFFFF) return res;
+  /* the selected pair, recomputed by every lane */
+  const int i = tm / eb, j = tm - (tm / eb) * eb;
+  const f3 PA = ld3(A.V + pa.edge[i].point_vindex[0]);
+  const f3 DA = ld3(A.V + pa.edge[i].point_vindex[1]) - PA;
+  const float DEAD = dot(DA, DA);
+  const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
+  const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
+  const f3 DV = PA - PB;
+  const float DEBD = dot(DB, DB);
+  const float DEABD = dot(DA, DB);
+  const float DEADAB = dot(DA, DV);
+  const float DEBDAB = dot(DB, DV);
+  const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
+  const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+  const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+  const f3 selA = PA + DA * dA;
+  const f3 selB = PB + DB * dB;
+  res.cp = selA;
+  res.n = cross(DA, DB);
   res.n = res.n * (1.0f / length(res.n));
   if (length(PosB - selB) < length(PosB - selA)) {
     if (!point_in_poly(B, selA)) return res;
@@ -282,30 +321,18 @@ __device__ __forceinline__ PColl edges(const Shape& A, const bdem_POBJ& pa, cons
   return res;
 }
 
-/* The contact of A with listed partner B after the depth clamps
- * (DEM_Kernels_Compute.cuh:820-950). */
-__device__ __forceinline__ PColl poly_pair_contact(const bdem_POBJ& pa, const bdem_POBJ& pb, const Shape& A,
-                                                   const Shape& B, f3 PosA, f3 PosB, f3 RVel) {
-  const f3 RelPos = (PosA - PosB) / length(PosA - PosB);
+/* The contact of A with listed partner B (DEM_Kernels_Compute.cuh:820-905)
+ * starts with the separating-plane test both ways (is_SP_Poly_Poly):
+ * true when a face of one body separates them. */
+__device__ __forceinline__ bool pair_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB) {
   bool sp = sp_poly_poly(A, PosB, B);
   if (!sp) sp = sp_poly_poly(B, PosA, A);
-  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
-  if (!sp) {
-    res = vertex_faces(PosA, A, B, pb);
-    if (!res.hit) {
-      res = vertex_faces(PosB, B, A, pa);
-      res.n = res.n * -1.0f;   /* B into A: normal reversed */
-    }
-    if (!res.hit) {
-      res = edges(A, pa, B, pb, RelPos, PosB);
-      if (res.hit) {
-        res.cls = C_EE;
-        res.dis = fminf(0.025f * pa.radius, res.dis);
-      }
-    }
-  }
-  if (!res.hit) return res;
-  const float rA = pa.radius;
+  return sp;
+}
+
+/* The depth clamps (DEM_Kernels_Compute.cuh:918-950). */
+__device__ __forceinline__ void depth_clamps(PColl& res, float rA, f3 PosA, f3 PosB, f3 RVel) {
+  const f3 RelPos = (PosA - PosB) / length(PosA - PosB);
   if (res.dis > 0.1500f * rA) {   /* contact error: fixed small overlap along the centres */
     res.n = RelPos / length(RelPos);
     res.dis = 0.025f * rA;
@@ -322,7 +349,45 @@ __device__ __forceinline__ PColl poly_pair_contact(const bdem_POBJ& pa, const bd
     res.cp = PosA;
     res.cls |= 16;
   }
-  return res;
+}
+
+/* A queued pair's operands. */
+struct PairOps {
+  int a, b, ta, tb;
+  f3 PosA, PosB, va, vb, wa, wb;
+  float ma, mb;
+};
+__device__ __forceinline__ PairOps pair_ops(uint32_t slot, const uint32_t* __restrict__ list32,
+                                            const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                            const float4* __restrict__ omg) {
+  PairOps o;
+  o.a = (int)(slot / LIST_STRIDE);
+  o.b = (int)list32[slot];
+  const float4 oa = omg[o.a], ob = omg[o.b];
+  o.ta = packed_type(oa.w);
+  o.tb = packed_type(ob.w);
+  const float4 va4 = vel[o.a], vb4 = vel[o.b];
+  o.PosA = xyz(pos[o.a]);
+  o.PosB = xyz(pos[o.b]);
+  o.va = xyz(va4); o.vb = xyz(vb4);
+  o.ma = va4.w; o.mb = vb4.w;
+  o.wa = xyz(oa); o.wb = xyz(ob);
+  return o;
+}
+
+/* Clamps, then the reference pair law into the pair's slot. */
+__device__ __forceinline__ void pair_store(const DevParams& P, const bdem_POBJ& pa, const PairOps& o, PColl c,
+                                           uint32_t slot, float4* __restrict__ pf, float4* __restrict__ pt) {
+  float4 F4 = make_float4(0.f, 0.f, 0.f, 0.f), T4 = F4;
+  if (c.hit) {
+    depth_clamps(c, pa.radius, o.PosA, o.PosB, o.va - o.vb);
+    f3 F, T;
+    pair_resp(P, o.va, o.ma, o.wa, o.ta, o.vb, o.mb, o.wb, o.tb, c.n, c.dis, c.cp - o.PosA, c.cp - o.PosB, F, T);
+    F4 = make_float4(F.x, F.y, F.z, 1.0f);
+    T4 = make_float4(T.x, T.y, T.z, 0.0f);
+  }
+  pf[slot] = F4;
+  pt[slot] = T4;
 }
 
 /* ---------------------------------------------------------------- walls */
@@ -500,7 +565,7 @@ __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict
                                                      float4* __restrict__ rot, uint32_t* __restrict__ qcount) {
   const DevParams& P = *Pp;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) *qcount = 0u;   /* the pair queue of this step starts empty */
+  if (t == 0) { qcount[0] = 0u; qcount[1] = 0u; }   /* this step's queues start empty */
   const size_t i = t / (size_t)L.stride;
   const int e = (int)(t - i * (size_t)L.stride);
   if (i >= (size_t)n) return;
@@ -564,8 +629,11 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
     if (at + (uint32_t)k < cap) queue[at + k] = (uint32_t)i * LIST_STRIDE + (uint32_t)k;
 }
 
-/* One queued pair per thread; grid-stride so that every wave exits once the
- * queue (at most n * 32 entries) is drained. */
+/* One queued pair per thread, grid-stride so that every wave exits once the
+ * queue (at most n * 32 entries) is drained: the separating-plane test.
+ * The few pairs it does not separate (about 1 in 60 in a dense pack) go to
+ * a second queue, where a wave per pair runs the long vertex-face and
+ * edge-edge searches. */
 __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict__ Pp,
                                                     const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
                                                     const float4* __restrict__ rot,
@@ -573,35 +641,79 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
                                                     const float4* __restrict__ omg,
                                                     const uint32_t* __restrict__ list32,
                                                     const uint32_t* __restrict__ queue,
-                                                    const uint32_t* __restrict__ qcount,
+                                                    uint32_t* __restrict__ qcount, uint32_t* __restrict__ equeue,
                                                     float4* __restrict__ pf, float4* __restrict__ pt) {
   const DevParams& P = *Pp;
-  const uint32_t total = *qcount;
+  const uint32_t total = qcount[0];
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
   const uint32_t cnt = total < cap ? total : cap;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x) {
-    const uint32_t slot = queue[q];
-    const int a = (int)(slot / LIST_STRIDE);
-    const int b = (int)list32[slot];
-    const float4 oa = omg[a], ob = omg[b];
-    const int ta = packed_type(oa.w), tb = packed_type(ob.w);
-    const bdem_POBJ& pa = pobj[ta];
-    const bdem_POBJ& pb = pobj[tb];
-    const float4 va4 = vel[a], vb4 = vel[b];
-    const f3 PosA = xyz(pos[a]), PosB = xyz(pos[b]);
-    const f3 va = xyz(va4), vb = xyz(vb4);
-    const Shape A = shape_of(rot, L, a, pa);
-    const Shape B = shape_of(rot, L, b, pb);
-    const PColl c = poly_pair_contact(pa, pb, A, B, PosA, PosB, va - vb);
-    float4 F4 = make_float4(0.f, 0.f, 0.f, 0.f), T4 = F4;
-    if (c.hit) {
-      f3 F, T;
-      pair_resp(P, va, va4.w, xyz(oa), ta, vb, vb4.w, xyz(ob), tb, c.n, c.dis, c.cp - PosA, c.cp - PosB, F, T);
-      F4 = make_float4(F.x, F.y, F.z, 1.0f);
-      T4 = make_float4(T.x, T.y, T.z, 0.0f);
+  const int lane = threadIdx.x & 63;
+  /* every lane of a wave runs the same number of iterations (the bound is
+   * rounded up to whole waves) so that the append below sees the whole wave */
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t cnt_w = (cnt + 63u) & ~63u;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt_w; q += stride) {
+    bool need = false;
+    uint32_t slot = 0u;
+    if (q < cnt) {
+      slot = queue[q];
+      const PairOps o = pair_ops(slot, list32, pos, vel, omg);
+      const bdem_POBJ& pa = pobj[o.ta];
+      const bdem_POBJ& pb = pobj[o.tb];
+      const Shape A = shape_of(rot, L, o.a, pa);
+      const Shape B = shape_of(rot, L, o.b, pb);
+      need = !pair_separated(A, B, o.PosA, o.PosB);
+      if (!need) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
     }
-    pf[slot] = F4;
-    pt[slot] = T4;
+    const uint64_t mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t base = 0u;
+      if (lane == leader) base = atomicAdd(&qcount[1], (uint32_t)__popcll(mask));
+      base = __shfl(base, leader, 64);
+      const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+      const uint32_t at = base + (uint32_t)__popcll(mask & lt);
+      if (need && at < cap) equeue[at] = slot;
+    }
+  }
+}
+
+/* The pairs the first pass could not separate, one wave per pair,
+ * grid-stride over the queue by waves: vertices of A in B, vertices of B in
+ * A (normal reversed), then edge-edge with its depth limited to 0.025 rA
+ * (DEM_Kernels_Compute.cuh:852-904). */
+__global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict__ Pp,
+                                                     const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+                                                     const float4* __restrict__ rot,
+                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                     const float4* __restrict__ omg,
+                                                     const uint32_t* __restrict__ list32,
+                                                     const uint32_t* __restrict__ qcount,
+                                                     const uint32_t* __restrict__ equeue,
+                                                     float4* __restrict__ pf, float4* __restrict__ pt) {
+  const DevParams& P = *Pp;
+  const uint32_t cap = (uint32_t)n * LIST_STRIDE;
+  const uint32_t cnt = qcount[1] < cap ? qcount[1] : cap;
+  const uint32_t waves = gridDim.x * (blockDim.x / 64u);
+  for (uint32_t q = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); q < cnt; q += waves) {
+    const uint32_t slot = equeue[q];
+    if (slot >= cap) continue;   /* wave-uniform */
+    const PairOps o = pair_ops(slot, list32, pos, vel, omg);
+    const bdem_POBJ& pa = pobj[o.ta];
+    const bdem_POBJ& pb = pobj[o.tb];
+    const Shape A = shape_of(rot, L, o.a, pa);
+    const Shape B = shape_of(rot, L, o.b, pb);
+    PColl c = vertex_faces_wave(o.PosA, A, B, pb);
+    if (!c.hit) {
+      c = vertex_faces_wave(o.PosB, B, A, pa);
+      c.n = c.n * -1.0f;   /* B into A: normal reversed */
+    }
+    if (!c.hit) {
+      const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
+      c = edges_wave(A, pa, B, pb, RelPos, o.PosB);
+      if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
+    }
+    if ((threadIdx.x & 63) == 0) pair_store(P, pa, o, c, slot, pf, pt);
   }
 }
 
@@ -715,7 +827,9 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_POBJ* dpob
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
   hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dpobj, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.pf, pb.pt);
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt);
+  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dpobj, b.n, pb.L, pb.rot,
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.qcount, pb.equeue, pb.pf, pb.pt);
 }
 
 void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
