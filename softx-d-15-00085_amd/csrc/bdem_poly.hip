@@ -321,10 +321,37 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const bdem_POBJ& pa,
   return res;
 }
 
+/* Face f of A separates B: B's centre and every vertex of B strictly in
+ * front (the per-face test of is_SP_Poly_Poly). */
+__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B) {
+  const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
+  if (!(dot(n, PosB - c) > 0.0f)) return false;
+  for (int v = 0; v < B.nv; ++v)
+    if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
+  return true;
+}
+
+/* The face of A whose normal points most towards B's centre. */
+__device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
+  int best = 0;
+  float bv = -3.0e38f;
+  for (int f = 0; f < A.nf; ++f) {
+    const float d = dot(ld3(A.Fn + f), dirAB);
+    if (d > bv) { bv = d; best = f; }
+  }
+  return best;
+}
+
 /* The contact of A with listed partner B (DEM_Kernels_Compute.cuh:820-905)
- * starts with the separating-plane test both ways (is_SP_Poly_Poly):
- * true when a face of one body separates them. */
+ * starts with the separating-plane test both ways (is_SP_Poly_Poly): true
+ * when some face of one body separates them.  Which face is found does not
+ * matter, only whether one exists, so each body's face turned most towards
+ * the other is tried first: it separates most separated pairs, and the
+ * full loops run only when it does not. */
 __device__ __forceinline__ bool pair_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB) {
+  const f3 dAB = PosB - PosA;
+  if (face_separates(A, facing_face(A, dAB), PosB, B)) return true;
+  if (face_separates(B, facing_face(B, -1.0f * dAB), PosA, A)) return true;
   bool sp = sp_poly_poly(A, PosB, B);
   if (!sp) sp = sp_poly_poly(B, PosA, A);
   return sp;
@@ -600,15 +627,17 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
                                                    uint32_t* __restrict__ queue, uint32_t* __restrict__ qcount,
                                                    uint32_t* status) {
   const DevParams& P = *Pp;
+  __shared__ __align__(16) unsigned char wlds[4][sizeof(WaveStage)];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int c = 0;
-  if (i < n && packed_pid(omg[i].w) >= 0) {
-    const float4 pr = pos_r[i];
-    c = for_each_contact(P, cells, pos_r, i, xyz(pr), pr.w, [&](int j, float4, int k) {
-      if (k < LIST_STRIDE) list32[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
-    });
-    if (c > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
-  }
+  const bool live = i < n && packed_pid(omg[i].w) >= 0;
+  const float4 pr = live ? pos_r[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  /* the staged stencil scan of a wave of consecutive particles */
+  int c = for_each_contact_wave(P, cells, pos_r, (uint32_t)n, i, xyz(pr), pr.w, live,
+                                *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]), [&](int j, float4, int k) {
+                                  if (k < LIST_STRIDE) list32[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
+                                });
+  if (!live) c = 0;
+  if (c > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
   if (i < n) nn[i] = (uint32_t)c;
   const int m = c < LIST_STRIDE ? c : LIST_STRIDE;
   /* wave prefix of the per-lane pair counts, one atomic per wave */
