@@ -125,7 +125,8 @@ __device__ __forceinline__ void particle_contact(const DevParams& P, int j, cons
  * body then runs max(contacts) times instead of max(candidates) times.
  * Particles with more than CLIST contacts (beyond the reference's 32-entry
  * list) finish the tail with a second scan, in the same order. */
-constexpr int CLIST = 16;   /* LDS list entries per thread: 16 KiB + wave staging keeps 5 blocks per CU */
+constexpr int CLIST = 12;   /* LDS list entries per thread: 12 KiB + wave staging fits 6 blocks per CU */
+static_assert(CLIST <= HMAX, "springs of the listed contacts are stored first");
 constexpr int STEP_THREADS = 256;
 
 template <int LAW>
@@ -227,7 +228,7 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
 /* discharge check -> next key.  Reads the sorted scratch, writes the state.  */
 /* ------------------------------------------------------------------------ */
 template <int LAW, bool ORIENT, bool SKIN>
-__global__ __launch_bounds__(256) void k_step(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_step(
     SkinArgs sk, uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
     const DevParams* __restrict__ Pp, int n, const uint2* __restrict__ cells,
     const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
@@ -301,17 +302,21 @@ __global__ __launch_bounds__(256) void k_step(
   if (alive) {
     const bool hist_slot = LAW != BDEM_LAW_REFERENCE && pid < hist_cap;
     if (nc > CLIST) {
-      /* beyond the list: rescan, same order (rare) */
-      PairHistCtx hc{nullptr, 0, 0, nullptr, HMAX};   /* springs looked up, not stored */
+      /* beyond the list: rescan, same order (rare).  Springs of contacts
+       * CLIST .. HMAX-1 are stored after the list's, as in the restatement */
+      PairHistCtx hc{nullptr, 0, 0, nullptr, HMAX};   /* no slot: looked up, not stored */
       if (hist_slot) {
         hc.in = hr_in + (size_t)pid * HMAX;
         hc.in_cnt = min((int)(hwc & 0xFFFFu), HMAX);
+        hc.out = hr_out + (size_t)pid * HMAX;
+        hc.out_cnt = CLIST;
       }
       for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
         if (k >= CLIST) particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc);
         if (dbg_list && k >= CLIST && k < LIST_STRIDE) dbg_list[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
       });
-      atomicOr(status, ST_LIST_OVERFLOW);
+      /* the reference's 32-entry list overflows (Collision_Detection_BroadPhase.cuh:481) */
+      if (nc > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
     }
     if (LAW != BDEM_LAW_REFERENCE && nc > HMAX) atomicOr(status, ST_HIST_OVERFLOW);
     if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
