@@ -63,8 +63,9 @@ constexpr unsigned long long COMP_INF = ~0ull;
 /* Incremental path                                                          */
 /* ------------------------------------------------------------------------ */
 constexpr int RS_ITEMS = RS_TILE / RS_THREADS;
+constexpr int RS_ROWS = RS_ITEMS * (RS_THREADS / 64);   /* (item row, wave) counts per block */
 constexpr int RS_UNROLL = 4;
-static_assert(RS_ITEMS * (RS_THREADS / 64) == 64, "one wave scans the per-(item row, wave) counts");
+static_assert(RS_ROWS <= 64 && RS_THREADS >= 256, "one wave (the fourth) scans the per-(item row, wave) counts");
 
 /* Ascending sort of v[0, cnt) by a bitonic network whose first stage of
  * every merge compares i with i ^ (k - 1): every comparator puts the
@@ -183,16 +184,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_resort(SortArgs a, int par) {
   __syncthreads();
 
   if (wave == 3) {
-    /* exclusive scan of the 64 (item row, wave) counts, in slot order */
-    const uint32_t v = wcnt[lane];
+    /* exclusive scan of the RS_ROWS (item row, wave) counts, in slot order */
+    const uint32_t v = lane < RS_ROWS ? wcnt[lane] : 0u;
     uint32_t incl = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
       if (lane >= o) incl += u;
     }
-    wpre[lane] = incl - v;
-    if (lane == 63) ucnt[RS_TILE] = (uint16_t)incl;
+    if (lane < RS_ROWS) wpre[lane] = incl - v;
+    if (lane == RS_ROWS - 1) ucnt[RS_TILE] = (uint16_t)incl;
   } else if (wave == 0) {
     /* window ends and the last unchanged element before the block */
     unsigned long long wlo = COMP_INF;
