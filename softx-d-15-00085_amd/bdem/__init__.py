@@ -3,6 +3,7 @@
 types      ctypes mirrors of the reference's boundary structs
 lib        the C ABI and a KDevice-style Device wrapper
 scenarios  seeded synthetic decks (box fill, silo, dense random packs)
+deck       the reference project decks and state files
 """
-from . import types, lib, scenarios  # noqa: F401
+from . import types, lib, scenarios, deck  # noqa: F401
 from .lib import Device, load  # noqa: F401

@@ -82,6 +82,19 @@ _PROTOS = {
     "bdem_get_skin": (C.c_float, [_P]),
     "bdem_derive_spring_constants": (C.c_int, [C.POINTER(T.POBJ), C.c_int, C.c_int, _dp, _dp, _dp,
                                                C.c_double, C.c_double, C.c_double]),
+    # include/bdem_deck.h
+    "bdem_ksim_read": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _ip, _ip, _ip]),
+    "bdem_deck_load": (C.c_int, [C.POINTER(_P), C.c_char_p, C.c_char_p, C.c_int, C.c_char_p, C.c_int]),
+    "bdem_deck_free": (None, [_P]),
+    "bdem_deck_get": (C.c_int, [_P, C.POINTER(C.POINTER(T.DeckInfo)), C.POINTER(C.POINTER(T.SimulationInfo)),
+                                C.POINTER(C.POINTER(T.InitConfig)), C.POINTER(C.POINTER(T.WOBJ)),
+                                C.POINTER(C.POINTER(T.POBJ)), C.POINTER(C.POINTER(T.DOBJ))]),
+    "bdem_deck_setup": (C.c_int, [_P, _P]),
+    "bdem_state_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(T.StateHeader),
+                                  _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip, _ip]),
+    "bdem_state_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(T.StateHeader),
+                                   _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip, C.POINTER(T.POBJ), C.c_int]),
+    "bdem_deck_load_state": (C.c_int, [_P, _P, C.c_char_p, C.POINTER(T.StateHeader)]),
 }
 
 SPRING_TC, SPRING_KN = 0, 1
@@ -232,6 +245,17 @@ class Device:
     def calibration_copy(self, nbytes, repeats=3):
         """Plain copy kernel over nbytes, `repeats` times (counter calibration)."""
         self._check(self.lib.bdem_calibration_copy(self.h, int(nbytes), int(repeats)))
+
+    def set_tallys(self, tallys):
+        """Device_Set_Tallys: the three dissipation tallies (unscaled)."""
+        t = np.ascontiguousarray(tallys, dtype=np.float32)
+        self._check(self.lib.bdem_set_tallys(self.h, _ptr(t, C.c_float)))
+
+    def energy(self):
+        """Device_Get_Energy: {particle, lifter, surface} tallies times 1E-4."""
+        e = np.zeros(3, np.float32)
+        self._check(self.lib.bdem_get_energy(self.h, _ptr(e, C.c_float)))
+        return e
 
     def update_dobj(self, dobj):
         """Upload moved lifters (Device_Update_DObjectPostions)."""

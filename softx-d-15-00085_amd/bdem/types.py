@@ -205,6 +205,54 @@ class ExtParams(C.Structure):
 SURFACE_PLANE = 0
 SURFACE_CYLINDER = 1
 
+SIM_GENERAL, SIM_BALLMILL, SIM_SILO = 0, 1, 2
+
+
+class DeckInfo(C.Structure):
+    """bdem_deck_info (include/bdem_deck.h)."""
+    _fields_ = [
+        ("title", C.c_char * 128),
+        ("force_spec", C.c_int), ("sim_type", C.c_int), ("particle_type", C.c_int),
+        ("dt_in_setup", C.c_int),
+        ("num_wobj", C.c_int), ("num_pobj", C.c_int), ("num_dobj", C.c_int),
+        ("dobj_types", C.c_int), ("dobj_repeat", C.c_int),
+        ("num_per_pobj", C.c_int * 8),
+        ("read_file", C.c_int),
+        ("start_file", C.c_char * 256),
+        ("render", C.c_int), ("color_type", C.c_int),
+        ("color_bin", C.c_float),
+        ("ortho", C.c_int), ("fps", C.c_int), ("gl_debug", C.c_int), ("write_steps", C.c_int),
+        ("output_format", C.c_int),
+        ("total_time", C.c_float),
+        ("is_run_limit", C.c_int),
+        ("run_time", C.c_float),
+        ("energy_steps", C.c_int), ("wall_steps", C.c_int),
+        ("power_revs", C.c_int), ("total_revs", C.c_int),
+        ("mill_rpm", C.c_float),
+        ("steps_per_rev", C.c_int),
+        ("rad_per_step", C.c_double),
+        ("vel_mill", C.c_float),
+        ("rotate_about", Float3),
+        ("manual_hatch", C.c_int), ("measure_flow", C.c_int), ("flow_steps", C.c_int),
+        ("kill_particle", C.c_int),
+        ("hatch_height", C.c_float),
+        ("rotating_world", C.c_int),
+    ]
+
+
+class StateHeader(C.Structure):
+    """bdem_state_header (include/bdem_deck.h)."""
+    _fields_ = [
+        ("spec_type", C.c_int),
+        ("time", C.c_double),
+        ("rev_num", C.c_int),
+        ("rev_complete", C.c_int),
+        ("rot_angle", C.c_double),
+        ("energy", C.c_float * 5),
+        ("num_rem", C.c_int),
+        ("rot_steps", C.c_int),
+    ]
+
 EXPECTED_SIZES = {
     "SimulationInfo": 176, "InitConfig": 720, "Surface": 68, "POBJ": 3396, "DOBJ": 392,
     "MacroCylinder": 84, "WOBJ": 1380,

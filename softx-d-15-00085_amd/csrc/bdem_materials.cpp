@@ -76,37 +76,54 @@ float wall_cn_kn(double kn, double m, double COR) {
 
 }  // namespace
 
-extern "C" int bdem_derive_spring_constants(bdem_POBJ* pobj, int num_pobj, int mode, const double* cor_pp,
-                                            const double* cor_surface, const double* cor_dsurface,
-                                            double tc_pp, double tc_ps, double tc_pd) {
+namespace bdem {
+
+/* The deck reader's order: per type as each POBJ is read (:1102-1168, with
+ * that type's own surface and lifter contact times), then the pair matrix
+ * row by row (:276-292, with one pair contact time: the reader keeps only
+ * the last type's). */
+int derive_springs(bdem_POBJ* pobj, int num_pobj, int mode, const double* cor_pp, const double* cor_surface,
+                   const double* cor_dsurface, double tc_pp, const double* tc_ps, const double* tc_pd) {
   if (!pobj || !cor_pp || !cor_surface || !cor_dsurface) return BDEM_EINVAL;
   if (num_pobj < 1 || num_pobj > PP_SLOTS) return BDEM_EINVAL;
   if (mode != BDEM_SPRING_TC && mode != BDEM_SPRING_KN) return BDEM_EINVAL;
   for (int i = 0; i < num_pobj; ++i)
     if (!(pobj[i].mass > 0.0f)) return BDEM_EINVAL;
-  if (mode == BDEM_SPRING_TC && !(tc_pp > 0.0 && tc_ps > 0.0 && tc_pd > 0.0)) return BDEM_EINVAL;
-  /* per type, as each POBJ is read (:1102-1168) */
+  if (mode == BDEM_SPRING_TC) {
+    if (!tc_ps || !tc_pd || !(tc_pp > 0.0)) return BDEM_EINVAL;
+    for (int i = 0; i < num_pobj; ++i)
+      if (!(tc_ps[i] > 0.0 && tc_pd[i] > 0.0)) return BDEM_EINVAL;
+  }
   for (int i = 0; i < num_pobj; ++i) {
     bdem_POBJ& p = pobj[i];
     const double m = p.mass;
     if (mode == BDEM_SPRING_TC) {
       const double Pi = 2.0 * std::asin(1.0);
       double cs = shifted(cor_surface[i]);
-      p.surface_Cn = 2.0 * m * (-std::log(cs) / (tc_ps));
-      p.surface_Kn = (m / (tc_ps * tc_ps)) * (std::log(cs) * std::log(cs) + Pi * Pi);
+      p.surface_Cn = 2.0 * m * (-std::log(cs) / (tc_ps[i]));
+      p.surface_Kn = (m / (tc_ps[i] * tc_ps[i])) * (std::log(cs) * std::log(cs) + Pi * Pi);
       double cd = shifted(cor_dsurface[i]);
-      p.Dsurface_Cn = 2.0 * m * (-std::log(cd) / (tc_pd));
-      p.Dsurface_Kn = (m / (tc_pd * tc_pd)) * (std::log(cd) * std::log(cd) + Pi * Pi);
+      p.Dsurface_Cn = 2.0 * m * (-std::log(cd) / (tc_pd[i]));
+      p.Dsurface_Kn = (m / (tc_pd[i] * tc_pd[i])) * (std::log(cd) * std::log(cd) + Pi * Pi);
     } else {
       p.surface_Cn = wall_cn_kn(p.surface_Kn, m, cor_surface[i]);
       p.Dsurface_Cn = wall_cn_kn(p.Dsurface_Kn, m, cor_dsurface[i]);
     }
   }
-  /* pair matrix, row by row (:276-292) */
   for (int i = 0; i < num_pobj; ++i)
     for (int j = 0; j < num_pobj; ++j) {
       if (mode == BDEM_SPRING_TC) particle_tc(pobj, i, j, cor_pp[i], cor_pp[j], tc_pp);
       else particle_kn(pobj, i, j, cor_pp[i], cor_pp[j]);
     }
   return BDEM_OK;
+}
+
+}  // namespace bdem
+
+extern "C" int bdem_derive_spring_constants(bdem_POBJ* pobj, int num_pobj, int mode, const double* cor_pp,
+                                            const double* cor_surface, const double* cor_dsurface,
+                                            double tc_pp, double tc_ps, double tc_pd) {
+  double ps[PP_SLOTS], pd[PP_SLOTS];
+  for (int i = 0; i < PP_SLOTS; ++i) { ps[i] = tc_ps; pd[i] = tc_pd; }
+  return bdem::derive_springs(pobj, num_pobj, mode, cor_pp, cor_surface, cor_dsurface, tc_pp, ps, pd);
 }

@@ -13,12 +13,12 @@ from bdem import scenarios as S
 from bdem import types as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "bdem_abi.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("bdem_abi.h", "bdem_deck.h")]
 
 
 def header_functions():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|float|const char\*|long long)\s+(bdem_\w+)\s*\(", text, re.M)))
+    text = "".join(open(h).read() for h in HEADERS)
+    return sorted(set(re.findall(r"^\s*(?:int|void|float|const char\*|long long)\s+(bdem_\w+)\s*\(", text, re.M)))
 
 
 def test_struct_sizes_match_reference_layout():
