@@ -126,8 +126,11 @@ int bdem_get_velocity(bdem_ctx* ctx, bdem_float3* vel, bdem_float3* avel, int fl
  * contact points are not recorded and are returned as zero. */
 int bdem_get_wall_forces(bdem_ctx* ctx, bdem_float3* wall_forces, bdem_float3* contact_points);
 int bdem_record_wall_forces(bdem_ctx* ctx, int on);
-/* Device_Get_Energy, Device/DeviceInterface.h:63 (three tallies, scaled 1E-4
- * like GetEnergy, GPU_Memory_Transfer.cuh:603-619). */
+/* Device_Get_Energy, Device/DeviceInterface.h:63: the three dissipation
+ * tallies {particle-particle, particle-lifter, particle-surface} scaled 1E-4
+ * like GetEnergy (GPU_Memory_Transfer.cuh:603-619), i.e. in J.  The step
+ * adds to them while SimulationInfo::EnergyCalc is set (spheres, reference
+ * law), one float atomic per contact as in Collision_Response.cuh. */
 int bdem_get_energy(bdem_ctx* ctx, float* energy);
 /* Kernel warnings accumulated since the last call (BDEM_WARN_* bits); clears them. */
 int bdem_get_status(bdem_ctx* ctx, unsigned int* warnings);
@@ -148,6 +151,17 @@ int bdem_open_hatch(bdem_ctx* ctx, const bdem_WOBJ* wobj, int num_wobj);
 int bdem_reset_energy(bdem_ctx* ctx, int num_values);
 /* drum_rotation, Device/DeviceInterface.h:80 */
 int bdem_drum_rotation(bdem_ctx* ctx, int cylinder_rotation);
+/* Turn every lifter attached to the drum by rad_per_step about the point
+ * (ax, ay, az) in the xy plane before each step, on the device: the
+ * reference's UpdateSim does it on the host (Rotate_VolObject_Inc,
+ * Input/Processing_Mills.h:7-60) and re-uploads the table every step
+ * (Device_Update_DObjectPostions, Device/DeviceInterface.cu:2826-2831).
+ * Same arithmetic (double angle, float coordinates, faces re-derived).
+ * rad_per_step 0 turns it off.  No counterpart as an entry point. */
+int bdem_set_lifter_rotation(bdem_ctx* ctx, double rad_per_step, float ax, float ay, float az);
+/* The lifter table as it stands (synchronous); out may be NULL to ask for
+ * the count.  No reference counterpart (the host owns the table there). */
+int bdem_get_dobjects(bdem_ctx* ctx, bdem_DOBJ* out, int* num);
 /* Start_Bin, Device/DeviceInterface.h:81 (a host flag in the reference) */
 int bdem_start_bin(bdem_ctx* ctx);
 /* Device_AddForce / SubForce / ZeroForce, Device/DeviceInterface.h:83-85:

@@ -175,6 +175,47 @@ int bo_set_dobj(void* h, const bdem_DOBJ* d, int nd) {
   return 0;
 }
 
+/* The lifter table as it stands (after any turning). */
+int bo_get_dobj(void* h, bdem_DOBJ* out) {
+  WITH(h, for (size_t k = 0; k < o.dobj.size(); ++k) out[k] = o.dobj[k]);
+  return 0;
+}
+
+/* Rotate_VolObject_Inc once (Input/Processing_Mills.h:7-60). */
+int bo_turn_lifters(void* h, double rad, float ax, float ay, float az) {
+  WITH(h, o.turn_lifters(rad, bdem_float3{ax, ay, az}));
+  return 0;
+}
+
+/* The mill loop of UpdateSim (main.cpp:114-121, 202-204): turn the lifters
+ * by rad before each of nsteps steps (rad = 0: no turning). */
+int bo_run(void* h, int nsteps, double rad, float ax, float ay, float az) {
+  WITH(h, {
+    for (int s = 0; s < nsteps; ++s) {
+      if (rad != 0.0) o.turn_lifters(rad, bdem_float3{ax, ay, az});
+      o.step();
+    }
+  });
+  return 0;
+}
+
+/* Tallies {particle, lifter, surface}: scaled 1E-4 as GetEnergy does
+ * (GPU_Memory_Transfer.cuh:603-619) into e (float), and the same
+ * increments summed in double, unscaled, into exact (may be NULL). */
+int bo_energy(void* h, float* e, double* exact) {
+  WITH(h, for (int k = 0; k < 3; ++k) {
+    e[k] = float(double(o.tally[k]) * 1E-4);
+    if (exact) exact[k] = o.tally_exact[k];
+  });
+  return 0;
+}
+
+/* Device_Set_Tallys / Device_Reset_Energy for the three tallies. */
+int bo_set_tallys(void* h, const float* t) {
+  WITH(h, for (int k = 0; k < 3; ++k) { o.tally[k] = t[k]; o.tally_exact[k] = double(t[k]); });
+  return 0;
+}
+
 /* Bit 0: live particle outside the grid; bit 1: history overflow. */
 int bo_warnings(void* h) {
   WITH(h, return (o.warn_out_of_grid ? 1 : 0) | (o.warn_hist_overflow ? 2 : 0));

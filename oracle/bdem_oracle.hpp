@@ -121,6 +121,15 @@ class Oracle {
   std::vector<V3<R>> f_pp, t_pp, f_w, t_w, f_l, t_l;
   int stepc = 0;
   long long steps_done = 0;
+  /* energy-dissipation tallies {particle, lifter, surface}
+   * (Tally_EnergyDis_PP / DisLift / DisSurf, GPU_Memory_Transfer.cuh:97-99):
+   * every contact's increment is added on its own to a float total, as the
+   * reference's atomicAdd does, here in particle order; `exact` sums the
+   * same increments in double */
+  enum { T_PP = 0, T_LIFT = 1, T_SURF = 2 };
+  float tally[3] = {0.f, 0.f, 0.f};
+  double tally_exact[3] = {0.0, 0.0, 0.0};
+  mutable std::vector<std::vector<R>> incs[3];   /* this step's increments, per particle */
   int warn_out_of_grid = 0;
   int warn_list_overflow = 0;
   int warn_hist_overflow = 0;
@@ -369,6 +378,11 @@ class Oracle {
     pair_response(a, b, N3, dis, RA, RB, F, T);
   }
 
+  /* spheres only: the library does not tally the polyhedra path */
+  void record(int k, int i, R e) const {
+    if (sim.EnergyCalc && !poly()) incs[k][i].push_back(e);
+  }
+
   /* Collision_Response_Particle_Particle (Collision_Response.cuh:374-494):
    * normal N3 (towards a), depth dis, lever arms RA, RB from the centres. */
   void pair_response(int a, int b, V3<R> N3, R dis, V3<R> RA, V3<R> RB, V3<R>& F, V3<R>& T) const {
@@ -395,6 +409,14 @@ class Oracle {
       R tan_f = (((R(0.50f) * (mass[a] + mass[b])) / R(sim.InitalDelta_t)) * R(0.050f)) *
                 R(pp.Fric_kinetic) * (mag / Velf);
       Tang = (R(-1.0f) * tan_f) * (Vt / mag);
+    }
+    if (sim.EnergyCalc) {
+      /* half the normal dissipation (each pair is visited from both sides),
+       * the whole tangential one (:401-475) */
+      R e = dot(Vnormal, Vnormal) > R(0) ? ((R(0.50f) * length(Fdiss)) * length(Vnormal)) * R(sim.InitalDelta_t)
+                                         : R(0);
+      if (mag > R(0)) e = e + ((length(Tang) * mag) * R(sim.InitalDelta_t));
+      record(T_PP, a, e);
     }
     Ft = Ft + Tang;
     F = F + Ft;
@@ -568,6 +590,16 @@ class Oracle {
     return c;
   }
 
+  /* The wall and lifter tallies (Collision_Response.cuh:25-89, 127-214,
+   * 266-344): the damping work |Fdiss| |Vn| dt unless the force was clipped
+   * as attractive, plus the tangential work |Ft| |Vt| dt. */
+  void wall_tally(int k, int i, bool attractive, V3<R> Fdiss, V3<R> Vn, V3<R> Tang, R mag) const {
+    if (!sim.EnergyCalc) return;
+    R e = (!attractive && dot(Vn, Vn) > R(0)) ? (length(Fdiss) * length(Vn)) * R(sim.InitalDelta_t) : R(0);
+    if (mag > R(0)) e = e + ((length(Tang) * mag) * R(sim.InitalDelta_t));
+    record(k, i, e);
+  }
+
   /* Collision_Response_Particle_StaticSurface (Collision_Response.cuh:4-109) */
   void resp_static(int i, const Coll& c, V3<R>& F, V3<R>& T) const {
     const bdem_POBJ& P = pobj[ptype[i]];
@@ -578,7 +610,8 @@ class Oracle {
     V3<R> Fdiss = R(P.surface_Cn) * Vnormal;
     V3<R> Ftr = Fnormal * c.n - Fdiss;
     V3<R> Fdir = Ftr / length(Ftr);
-    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    const bool attractive = dot(Fdir, c.n) < R(0);
+    if (attractive) Ftr = Fnormal * c.n;
     V3<R> Vt = PVel - Vnormal;
     if (sim.Rotation) Vt = Vt + cross(avel[i], Rc);
     R mag = dot(Vt, Vt);
@@ -591,6 +624,7 @@ class Oracle {
                 (R(P.surface_Fric_static) * (mag / Velf));
       Tang = (R(-1.0f) * tan_f) * (Vt / mag);
     }
+    wall_tally(T_SURF, i, attractive, Fdiss, Vnormal, Tang, mag);
     Ftr = Ftr + Tang;
     F = F + Ftr;
     if (sim.Rotation) T = T + cross(Rc, Ftr);
@@ -606,7 +640,8 @@ class Oracle {
     V3<R> Fdiss = R(P.surface_Cn) * Vn;
     V3<R> Ftr = Fnormal * c.n - Fdiss;
     V3<R> Fdir = Ftr / length(Ftr);
-    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    const bool attractive = dot(Fdir, c.n) < R(0);
+    if (attractive) Ftr = Fnormal * c.n;
     V3<R> V_CPA = vel[i] + cross(avel[i], Rc);
     V3<R> V_CPNA = dot(V_CPA, c.n) * c.n;
     V3<R> Vt_A = V_CPA - V_CPNA;
@@ -621,6 +656,7 @@ class Oracle {
       R tan_f = R(P.surface_Fric_kinetic) * (mag / Velf);
       Tang = ((R(-1.0f) * tan_f) * (Vt / mag)) * (mass[i] / R(sim.InitalDelta_t)) * R(0.050f);
     }
+    wall_tally(T_SURF, i, attractive, Fdiss, Vn, Tang, mag);
     Ftr = Ftr + Tang;
     Tout = sim.Rotation ? cross(Rc, Ftr) : mk<R>(0, 0, 0);
     return Ftr;
@@ -1315,7 +1351,8 @@ class Oracle {
     const V3<R> Fdiss = R(P.Dsurface_Cn) * Vnormal;
     Ftr = Fnormal * c.n - Fdiss;
     const V3<R> Fdir = Ftr / length(Ftr);
-    if (dot(Fdir, c.n) < R(0)) Ftr = Fnormal * c.n;
+    const bool attractive = dot(Fdir, c.n) < R(0);
+    if (attractive) Ftr = Fnormal * c.n;
     const V3<R> V_CPA = vel[i] + cross(avel[i], Rc);
     const V3<R> V_CPB = Vl + cross(zero, zero);
     const V3<R> V_CPNA = dot(V_CPA, c.n) * c.n;
@@ -1333,6 +1370,7 @@ class Oracle {
                       (R(P.Dsurface_Fric_kinetic) * (mag / Velf));
       Tang = (R(-1.0f) * tan_f) * (Vt_Rel / mag);
     }
+    wall_tally(T_LIFT, i, attractive, Fdiss, Vnormal, Tang, mag);
     Ftr = Ftr + Tang;
     Tq = sim.Rotation ? cross(Rc, Ftr) : zero;
   }
@@ -1512,10 +1550,54 @@ class Oracle {
     bin_cells();
   }
 
+  /* Rotate_VolObject_Inc (Input/Processing_Mills.h:7-60): every lifter
+   * attached to the drum turned by `rad` about `about` in the xy plane --
+   * cos/sin of the double angle, products and sum in double, stored in
+   * float -- and its faces re-derived in float (quadrilateral form). */
+  void turn_lifters(double rad, bdem_float3 about) {
+    const double c = std::cos(rad), s = std::sin(rad);
+    auto turn = [&](bdem_float3 v) {   /* x, y replaced; z kept; then += about */
+      const float px = v.x - about.x, py = v.y - about.y;
+      bdem_float3 o;
+      o.x = float(c * double(px) + s * double(py)) + about.x;
+      o.y = float(-s * double(px) + c * double(py)) + about.y;
+      o.z = v.z + about.z;
+      return o;
+    };
+    for (bdem_DOBJ& d : dobj) {
+      if (!d.is_attached_DWorld) continue;
+      for (int k = 0; k < d.num_vertex; ++k) d.vertex[k] = turn(d.vertex[k]);
+      for (int j = 0; j < d.num_faces; ++j) {
+        bdem_Surface& f = d.faces[j];
+        V3<float> sum = mk<float>(0.f, 0.f, 0.f);
+        for (int q = 0; q < f.num_vertex; ++q) sum = sum + cvt<float>(d.vertex[f.vertex_Order[q]]);
+        const V3<float> cen = (1.0f / float(f.num_vertex)) * sum;
+        const V3<float> v0 = cvt<float>(d.vertex[f.vertex_Order[0]]), v1 = cvt<float>(d.vertex[f.vertex_Order[1]]);
+        const V3<float> v2 = cvt<float>(d.vertex[f.vertex_Order[2]]), v3 = cvt<float>(d.vertex[f.vertex_Order[3]]);
+        const V3<float> V1 = v1 - v0, V2 = v2 - v1, V3_ = v3 - v2, V4 = v0 - v3;
+        const V3<float> n = cross(V2, V1);
+        const V3<float> nn = n / length(n);
+        f.centroid = bdem_float3{cen.x, cen.y, cen.z};
+        f.normal = bdem_float3{nn.x, nn.y, nn.z};
+        f.area = 0.500000f * length(n) + 0.500000f * length(cross(V3_, V4));
+      }
+      d.COM = turn(d.COM);
+    }
+  }
+
   /* One Device_DEM_UpdateSim. */
   void step() {
     prepare();
+    if (sim.EnergyCalc)
+      for (int k = 0; k < 3; ++k) incs[k].assign(size_t(N), std::vector<R>());
     compute_forces();
+    if (sim.EnergyCalc)
+      for (int i = 0; i < N; ++i)
+        for (int k = 0; k < 3; ++k)
+          for (R e : incs[k][size_t(i)]) {
+            tally[k] = float(tally[k] + float(e));
+            tally_exact[k] += double(e);
+          }
     integrate();
     if (sim.Kill_Particles) {
       ++stepc;

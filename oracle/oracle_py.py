@@ -51,6 +51,11 @@ def load():
     lib.bo_poly_pair.argtypes = [P, C.c_int, C.c_int, dp]
     lib.bo_angular_acc.argtypes = [P, C.c_int, dp, dp]
     lib.bo_poly_walls.argtypes = [P, C.c_int, dp]
+    lib.bo_get_dobj.argtypes = [P, C.c_void_p]
+    lib.bo_turn_lifters.argtypes = [P, C.c_double, C.c_float, C.c_float, C.c_float]
+    lib.bo_run.argtypes = [P, C.c_int, C.c_double, C.c_float, C.c_float, C.c_float]
+    lib.bo_energy.argtypes = [P, fp, dp]
+    lib.bo_set_tallys.argtypes = [P, fp]
     _lib = lib
     return lib
 
@@ -149,6 +154,33 @@ class Oracle:
         arr = (dobj[0].__class__ * len(dobj))(*dobj) if dobj else None
         self._d = arr
         self.lib.bo_set_dobj(self.h, arr, len(dobj) if dobj else 0)
+
+    def get_dobj(self):
+        """The lifter table as it stands (after any turning)."""
+        d = self.deck["dobj"]
+        arr = (d[0].__class__ * len(d))()
+        self.lib.bo_get_dobj(self.h, arr)
+        return list(arr)
+
+    def turn_lifters(self, rad, about):
+        """Rotate_VolObject_Inc once."""
+        self.lib.bo_turn_lifters(self.h, float(rad), *[float(a) for a in about])
+
+    def run(self, nsteps, rad=0.0, about=(0.0, 0.0, 0.0)):
+        """nsteps steps, each after turning the lifters by rad (the mill loop of UpdateSim)."""
+        self.lib.bo_run(self.h, int(nsteps), float(rad), *[float(a) for a in about])
+
+    def energy(self):
+        """({particle, lifter, surface} tallies x 1E-4 as GetEnergy returns them,
+        the same increments summed in double, unscaled)."""
+        e = np.zeros(3, np.float32)
+        x = np.zeros(3, np.float64)
+        self.lib.bo_energy(self.h, _p(e, C.c_float), _p(x, C.c_double))
+        return e, x
+
+    def set_tallys(self, t):
+        t = np.ascontiguousarray(t, np.float32)
+        self.lib.bo_set_tallys(self.h, _p(t, C.c_float))
 
     def lifter_types(self):
         t = np.zeros(self.n, np.int32)

@@ -78,6 +78,8 @@ _PROTOS = {
     "bdem_debug_contacts": (C.c_int, [_P, _up, _up]),
     "bdem_debug_forces": (C.c_int, [_P, _fp, _fp, _fp, _fp, _ip]),
     "bdem_debug_step_contacts": (C.c_int, [_P, _up, _up]),
+    "bdem_set_lifter_rotation": (C.c_int, [_P, C.c_double, C.c_float, C.c_float, C.c_float]),
+    "bdem_get_dobjects": (C.c_int, [_P, C.POINTER(T.DOBJ), _ip]),
     "bdem_set_skin": (C.c_int, [_P, C.c_float]),
     "bdem_get_skin": (C.c_float, [_P]),
     "bdem_derive_spring_constants": (C.c_int, [C.POINTER(T.POBJ), C.c_int, C.c_int, _dp, _dp, _dp,
@@ -256,6 +258,22 @@ class Device:
         e = np.zeros(3, np.float32)
         self._check(self.lib.bdem_get_energy(self.h, _ptr(e, C.c_float)))
         return e
+
+    def set_lifter_rotation(self, rad_per_step, about=(0.0, 0.0, 0.0)):
+        """Turn the drum's lifters by rad_per_step before every step, on the device."""
+        self._check(self.lib.bdem_set_lifter_rotation(self.h, float(rad_per_step), *[float(a) for a in about]))
+
+    def get_dobj(self):
+        """The lifter table as it stands (after any turning on the device)."""
+        k = C.c_int(0)
+        self._check(self.lib.bdem_get_dobjects(self.h, None, C.byref(k)))
+        arr = (T.DOBJ * max(k.value, 1))()
+        self._check(self.lib.bdem_get_dobjects(self.h, arr, C.byref(k)))
+        return list(arr)[:k.value]
+
+    def reset_energy(self):
+        """Device_Reset_Energy(0): zero the three tallies."""
+        self._check(self.lib.bdem_reset_energy(self.h, 0))
 
     def update_dobj(self, dobj):
         """Upload moved lifters (Device_Update_DObjectPostions)."""

@@ -8,6 +8,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -34,6 +35,12 @@ struct bdem_ctx {
   bool record_wf = false;
   bool start_bin = false;
   float tallys[6] = {0, 0, 0, 0, 0, 0};
+  float* d_tally = nullptr;  /* device tallies {particle, lifter, surface} the step kernel adds to */
+  /* lifters turned on the device before every step (Rotate_VolObject_Inc) */
+  bool lift_on = false;
+  double lift_c = 1.0, lift_s = 0.0;
+  float lift_about[3] = {0.f, 0.f, 0.f};
+  bool dobj_dirty = false;   /* the device lifter table is ahead of c->dobj */
 
   DevParams hp{};
   DevParams* dp = nullptr;
@@ -191,6 +198,7 @@ void build_params(bdem_ctx* c) {
   P.num_dobj = int(c->dobj.size() < size_t(MAX_DOBJ) ? c->dobj.size() : size_t(MAX_DOBJ));
   for (int k = 0; k < P.num_dobj; ++k) P.dobj[k] = c->dobj[k];
   P.mill_sim = s.isMillSim ? 1 : 0;
+  P.tally = c->d_tally;
   P.mill_cent = s.cent;
   P.dobj_bound_r = s.DObject_CylinderBoundR;
   for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
@@ -200,7 +208,21 @@ void build_params(bdem_ctx* c) {
   }
 }
 
+/* The lifters the device has turned since the last upload, back into the
+ * host table (so a later upload does not undo the turning). */
+int pull_dobj(bdem_ctx* c) {
+  if (!c->dobj_dirty || c->dobj.empty() || !c->dp) return BDEM_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t nd = c->dobj.size() < size_t(MAX_DOBJ) ? c->dobj.size() : size_t(MAX_DOBJ);
+  HIPCHK(c, hipMemcpy(c->dobj.data(), reinterpret_cast<char*>(c->dp) + offsetof(DevParams, dobj),
+                      nd * sizeof(bdem_DOBJ), hipMemcpyDeviceToHost));
+  c->dobj_dirty = false;
+  return BDEM_OK;
+}
+
 int upload_params(bdem_ctx* c) {
+  int rc0 = pull_dobj(c);
+  if (rc0 != BDEM_OK) return rc0;
   build_params(c);
   HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
   if (c->dpobj)
@@ -535,6 +557,9 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   c->orient = sim->sphere_orient || c->poly;
   c->stepc = 0;
   c->hist_par = 0;
+  c->lift_on = false;
+  c->dobj_dirty = false;
+  std::memset(c->tallys, 0, sizeof(c->tallys));
 
   const int n = sim->Num_Particles;
   StepBuffers& b = c->b;
@@ -556,6 +581,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.part, 1024);
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
+  ALLOC(c->d_tally, 8);
   ALLOC(b.wall_force, (size_t)n * 3);
   if (c->poly) {
     PolyBuffers& pb = c->pb;
@@ -578,6 +604,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
 #undef ALLOC
   HIPCHK(c, hipMemset(b.status, 0, sizeof(uint32_t)));
   HIPCHK(c, hipMemset(b.kill_count, 0, sizeof(int)));
+  HIPCHK(c, hipMemset(c->d_tally, 0, 8 * sizeof(float)));   /* Set_Tallys with zeros (:1644-1660) */
   HIPCHK(c, hipMemset(b.meta, 0, META_WORDS * sizeof(uint32_t)));
   HIPCHK(c, hipMemset(b.wall_force, 0, (size_t)n * 3 * sizeof(float)));
   HIPCHK(c, hipMemset(b.pos_cur, 0, (size_t)n * sizeof(float4)));
@@ -711,6 +738,9 @@ int bdem_set_tallys(bdem_ctx* c, const float* t) {
   if (rc) return rc;
   if (!t) return fail(c, BDEM_EINVAL, "null argument");
   for (int i = 0; i < 3; ++i) c->tallys[i] = t[i];
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->d_tally, c->tallys, 3 * sizeof(float), hipMemcpyHostToDevice));
   return BDEM_OK;
 }
 
@@ -781,6 +811,9 @@ int bdem_get_energy(bdem_ctx* c, float* e) {
   int rc = require(c);
   if (rc) return rc;
   if (!e) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->tallys, c->d_tally, 3 * sizeof(float), hipMemcpyDeviceToHost));
   for (int i = 0; i < 3; ++i) e[i] = float(double(c->tallys[i]) * 1E-4);
   return BDEM_OK;
 }
@@ -846,6 +879,7 @@ int bdem_update_dobject_positions(bdem_ctx* c, const bdem_DOBJ* d) {
   if ((rc = check_dobj(c, d, int(c->dobj.size()))) != BDEM_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dobj_dirty = false;   /* the caller's table replaces the device's turned one */
   c->dobj.assign(d, d + c->dobj.size());
   return upload_params(c);
 }
@@ -876,8 +910,14 @@ int bdem_open_hatch(bdem_ctx* c, const bdem_WOBJ* w, int nw) {
 int bdem_reset_energy(bdem_ctx* c, int num) {
   int rc = require(c);
   if (rc) return rc;
-  if (num <= 3) { c->tallys[0] = c->tallys[1] = c->tallys[2] = 0.f; }
-  else { c->tallys[3] = c->tallys[4] = 0.f; }
+  if (num <= 3) {
+    c->tallys[0] = c->tallys[1] = c->tallys[2] = 0.f;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_tally, c->tallys, 3 * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    c->tallys[3] = c->tallys[4] = 0.f;
+  }
   return BDEM_OK;
 }
 
@@ -888,6 +928,32 @@ int bdem_drum_rotation(bdem_ctx* c, int on) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->cyl_rotation = on != 0;
   return upload_params(c);
+}
+
+int bdem_set_lifter_rotation(bdem_ctx* c, double rad_per_step, float ax, float ay, float az) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!std::isfinite(rad_per_step) || !std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az))
+    return fail(c, BDEM_EINVAL, "rotation angle and axis point must be finite");
+  if (c->poly && rad_per_step != 0.0) return fail(c, BDEM_ENOTSUP, "lifters are not supported with polyhedra");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->lift_on = rad_per_step != 0.0;
+  c->lift_c = std::cos(rad_per_step);
+  c->lift_s = std::sin(rad_per_step);
+  c->lift_about[0] = ax; c->lift_about[1] = ay; c->lift_about[2] = az;
+  return BDEM_OK;
+}
+
+int bdem_get_dobjects(bdem_ctx* c, bdem_DOBJ* out, int* num) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!num) return fail(c, BDEM_EINVAL, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = pull_dobj(c)) != BDEM_OK) return rc;
+  *num = (int)c->dobj.size();
+  if (out) for (size_t k = 0; k < c->dobj.size(); ++k) out[k] = c->dobj[k];
+  return BDEM_OK;
 }
 
 int bdem_start_bin(bdem_ctx* c) {
@@ -906,6 +972,12 @@ int bdem_update_sim(bdem_ctx* c) {
   if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   hipEvent_t* ev = timing_slot(c);
+  if (c->lift_on && !c->dobj.empty()) {
+    /* UpdateSim turns the lifters before the step (main.cpp:114-121) */
+    launch_turn_lifters(c->dp, c->lift_c, c->lift_s, c->lift_about[0], c->lift_about[1], c->lift_about[2],
+                        c->stream);
+    c->dobj_dirty = true;
+  }
   if ((rc = prepare_sorted(c, ev)) != BDEM_OK) return rc;
   bool do_kill = false;
   if (c->sim.Kill_Particles) {
@@ -927,7 +999,7 @@ int bdem_update_sim(bdem_ctx* c) {
                      c->record_wf, c->stream);
   else
     launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
-                c->append_par, do_kill, c->record_wf, sa, c->dbg_list, c->dbg_nn, c->stream);
+                c->append_par, do_kill, c->record_wf, c->sim.EnergyCalc, sa, c->dbg_list, c->dbg_nn, c->stream);
   if (sa.on) c->vpar = sa.par;
   if (do_kill) c->full_next = true;
   if (ev) {

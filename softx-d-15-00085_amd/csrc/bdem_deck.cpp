@@ -491,7 +491,8 @@ bool read_dobj(Reader& r, bdem_DOBJ& d, Err& e) {
  * the sum in double, rounded once to float), as the host rotations do. */
 bdem_float3 turn(bdem_float3 v, bdem_float3 about, double c, double s) {
   bdem_float3 p = sub(v, about);
-  bdem_float3 q = mk((float)(c * p.x + s * p.y), (float)(-s * p.x + c * p.y), p.z);
+  /* x and y replaced, z kept, then the whole point moved back by `about` */
+  bdem_float3 q = mk((float)(c * p.x + s * p.y), (float)(-s * p.x + c * p.y), v.z);
   return add(q, about);
 }
 
@@ -503,14 +504,14 @@ void make_new_lifter(std::vector<bdem_DOBJ>& L, bdem_float3 about, int index, in
   const float c = std::cos(angle), s = std::sin(angle);
   for (int k = 0; k < d.num_vertex; ++k) {
     bdem_float3 p = sub(d.vertex[k], about);
-    d.vertex[k] = add(mk(c * p.x + s * p.y, -s * p.x + c * p.y, p.z), about);
+    d.vertex[k] = add(mk(c * p.x + s * p.y, -s * p.x + c * p.y, d.vertex[k].z), about);
   }
   for (int j = 0; j < d.num_faces; ++j) {
     dobj_centroid(d.faces[j], d.vertex);
     dobj_quad_face(d.faces[j], d.vertex);
   }
   bdem_float3 p = sub(d.COM, about);
-  d.COM = add(mk(c * p.x + s * p.y, -s * p.x + c * p.y, p.z), about);
+  d.COM = add(mk(c * p.x + s * p.y, -s * p.x + c * p.y, d.COM.z), about);
 }
 
 /* Repeat_Lifters (:1962-2028) */

@@ -114,6 +114,7 @@ struct DevParams {
   int mill_sim;               /* SimulationInfo::isMillSim */
   bdem_float3 mill_cent;      /* SimulationInfo::cent */
   float dobj_bound_r;         /* SimulationInfo::DObject_CylinderBoundR */
+  float* tally;               /* {particle, lifter, surface} dissipation tallies (device) */
   float d_kn[MAX_TYPES], d_cn[MAX_TYPES], d_fk[MAX_TYPES];   /* POBJ Dsurface_* */
   bdem_DOBJ dobj[MAX_DOBJ];
 };
@@ -530,9 +531,26 @@ __device__ __forceinline__ int for_each_contact_wave(const DevParams& P, const u
 /* ------------------------------------------------------------------------ */
 /* Collision_Response_Particle_Particle (Collision_Response.cuh:374-494) for
  * normal N (towards A), depth dis and lever arms RA, RB from the centres. */
+/* Energy-dissipation tallies {particle, lifter, surface}
+ * (Tally_EnergyDis_PP / DisLift / DisSurf, GPU_Memory_Transfer.cuh:97-99):
+ * each contact adds its increment on its own with a float atomic, as the
+ * reference does.  tally == nullptr: not counted. */
+enum { TALLY_PP = 0, TALLY_LIFT = 1, TALLY_SURF = 2 };
+
+/* The wall and lifter increment (Collision_Response.cuh:25-89, 127-214,
+ * 266-344): |Fdiss| |Vn| dt unless the force was clipped as attractive,
+ * plus |Ft| |Vt| dt when there is tangential slip. */
+__device__ __forceinline__ void wall_tally(const DevParams& P, float* tally, int k, bool attractive, f3 Fdiss,
+                                           f3 Vn, f3 Tang, float mag) {
+  if (!tally) return;
+  float e = (!attractive && dot(Vn, Vn) > 0.0f) ? (length(Fdiss) * length(Vn)) * P.dt : 0.0f;
+  if (mag > 0.0f) e = e + ((length(Tang) * mag) * P.dt);
+  atomicAdd(tally + k, e);
+}
+
 __device__ __forceinline__ void pair_resp(const DevParams& P, f3 va, float ma, f3 wa, int ta, f3 vb,
                                           float mb, f3 wb, int tb, f3 N, float dis, f3 RA, f3 RB,
-                                          f3& Fc, f3& Tc) {
+                                          f3& Fc, f3& Tc, float* tally = nullptr) {
   const f3 RVel = va - vb;
   const float Kn = P.pp_kn[ta][tb], Cn = P.pp_cn[ta][tb], Fk = P.pp_fk[ta][tb];
   const f3 Vnormal = dot(RVel, N) * N;
@@ -555,6 +573,13 @@ __device__ __forceinline__ void pair_resp(const DevParams& P, f3 va, float ma, f
     const float tan_f = (((0.50f * (ma + mb)) / P.dt) * 0.050f) * Fk * (mag / Velf);
     Tang = (-1.0f * tan_f) * (Vt / mag);
   }
+  if (tally) {
+    /* half the normal work (each pair is visited from both sides), the whole
+     * tangential work (:401-475) */
+    float e = dot(Vnormal, Vnormal) > 0.0f ? ((0.50f * length(Fdiss)) * length(Vnormal)) * P.dt : 0.0f;
+    if (mag > 0.0f) e = e + ((length(Tang) * mag) * P.dt);
+    atomicAdd(tally + TALLY_PP, e);
+  }
   Ft = Ft + Tang;
   Fc = Ft;                 /* the kernel adds it: force += Force_PairA.trans (:709) */
   Tc = cross(RA, Ft);      /* torque = cross(R_Body_COMA, Force.trans) (:485) */
@@ -562,13 +587,13 @@ __device__ __forceinline__ void pair_resp(const DevParams& P, f3 va, float ma, f
 
 __device__ __forceinline__ void pair_ref(const DevParams& P, f3 pa, float ra, f3 va, float ma,
                                          f3 wa, int ta, f3 pb, float rb, f3 vb, float mb, f3 wb,
-                                         int tb, f3& Fc, f3& Tc) {
+                                         int tb, f3& Fc, f3& Tc, float* tally = nullptr) {
   const f3 relPos = pa - pb;
   const f3 N = relPos / length(relPos);
   const float dis = fabsf(length(relPos) - (ra + rb)) + 0.000010f;
   const f3 RA = (-1.0f * N) * ra;
   const f3 RB = N * rb;
-  pair_resp(P, va, ma, wa, ta, vb, mb, wb, tb, N, dis, RA, RB, Fc, Tc);
+  pair_resp(P, va, ma, wa, ta, vb, mb, wb, tb, N, dis, RA, RB, Fc, Tc, tally);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -698,14 +723,15 @@ __device__ __forceinline__ Coll sphere_cone(f3 p, float r, const bdem_cone& cn) 
 
 /* Collision_Response_Particle_StaticSurface, Collision_Response.cuh:4-109 */
 __device__ __forceinline__ void resp_static(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
-                                            const Coll& c, f3& F, f3& T) {
+                                            const Coll& c, f3& F, f3& T, float* tally = nullptr) {
   const f3 Rc = c.cp - p;
   const f3 Vnormal = dot(v, c.n) * c.n;
   const float Fnormal = P.s_kn[t] * c.dis;
   const f3 Fdiss = P.s_cn[t] * Vnormal;
   f3 Ftr = Fnormal * c.n - Fdiss;
   const f3 Fdir = Ftr / length(Ftr);
-  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  const bool attractive = dot(Fdir, c.n) < 0.0f;
+  if (attractive) Ftr = Fnormal * c.n;
   f3 Vt = v - Vnormal;
   if (P.rotation) Vt = Vt + cross(w, Rc);
   float mag = dot(Vt, Vt);
@@ -717,6 +743,7 @@ __device__ __forceinline__ void resp_static(const DevParams& P, int t, f3 p, f3 
     const float tan_f = (0.050f * (m / P.dt)) * (P.s_fs[t] * (mag / Velf));
     Tang = (-1.0f * tan_f) * (Vt / mag);
   }
+  wall_tally(P, tally, TALLY_SURF, attractive, Fdiss, Vnormal, Tang, mag);
   Ftr = Ftr + Tang;
   F = F + Ftr;
   if (P.rotation) T = T + cross(Rc, Ftr);
@@ -724,7 +751,7 @@ __device__ __forceinline__ void resp_static(const DevParams& P, int t, f3 p, f3 
 
 /* Collision_Response_Particle_MovingSurface, Collision_Response.cuh:116-233 */
 __device__ __forceinline__ f3 resp_moving(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
-                                          const Coll& c, f3 Vs, f3& Tout) {
+                                          const Coll& c, f3 Vs, f3& Tout, float* tally = nullptr) {
   const f3 Rc = c.cp - p;
   const f3 RVel = v - Vs;
   const f3 Vn = dot(RVel, c.n) * c.n;
@@ -732,7 +759,8 @@ __device__ __forceinline__ f3 resp_moving(const DevParams& P, int t, f3 p, f3 v,
   const f3 Fdiss = P.s_cn[t] * Vn;
   f3 Ftr = Fnormal * c.n - Fdiss;
   const f3 Fdir = Ftr / length(Ftr);
-  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  const bool attractive = dot(Fdir, c.n) < 0.0f;
+  if (attractive) Ftr = Fnormal * c.n;
   const f3 V_CPA = v + cross(w, Rc);
   const f3 V_CPNA = dot(V_CPA, c.n) * c.n;
   const f3 Vt_A = V_CPA - V_CPNA;
@@ -747,6 +775,7 @@ __device__ __forceinline__ f3 resp_moving(const DevParams& P, int t, f3 p, f3 v,
     const float tan_f = P.s_fk[t] * (mag / Velf);
     Tang = ((-1.0f * tan_f) * (Vt / mag)) * (m / P.dt) * 0.050f;
   }
+  wall_tally(P, tally, TALLY_SURF, attractive, Fdiss, Vn, Tang, mag);
   Ftr = Ftr + Tang;
   Tout = P.rotation ? cross(Rc, Ftr) : mk3(0.f, 0.f, 0.f);
   return Ftr;
@@ -810,7 +839,7 @@ __device__ __forceinline__ void wall_contact_hist(const DevParams& P, int t, f3 
 }
 
 __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, float r, f3 v, f3 w,
-                                            float m, f3& F, f3& T, WallHist* wh) {
+                                            float m, f3& F, f3& T, WallHist* wh, float* tally = nullptr) {
   F = mk3(0.f, 0.f, 0.f);
   T = mk3(0.f, 0.f, 0.f);
   const bool hist = P.law != BDEM_LAW_REFERENCE;
@@ -828,10 +857,10 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
           wall_contact_hist(P, t, p, r, v, w, m, c, vw, k * 16 + s, F, T, *wh);
         } else if (moving_planes) {
           f3 tq;
-          F = F + resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq);
+          F = F + resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq, tally);
           T = T + tq;
         } else {
-          resp_static(P, t, p, v, w, m, c, F, T);
+          resp_static(P, t, p, v, w, m, c, F, T, tally);
         }
       }
     } else if (W.surface_type == BDEM_SURFACE_CYLINDER) {
@@ -843,11 +872,11 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
           wall_contact_hist(P, t, p, r, v, w, m, c, vw, 128 + k, F, T, *wh);
         } else if (P.cyl_rotation) {
           f3 tq;
-          F = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq);
+          F = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq, tally);
           T = T + tq;
         } else {
           f3 f = mk3(0.f, 0.f, 0.f), tq = mk3(0.f, 0.f, 0.f);
-          resp_static(P, t, p, v, w, m, c, f, tq);
+          resp_static(P, t, p, v, w, m, c, f, tq, tally);
           F = f;
           T = T + tq;
         }
@@ -872,7 +901,7 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
     const Coll c = sphere_cone(p, r, P.cone[k]);
     if (!c.hit) continue;
     if (hist) wall_contact_hist(P, t, p, r, v, w, m, c, mk3(0.f, 0.f, 0.f), 176 + k, F, T, *wh);
-    else resp_static(P, t, p, v, w, m, c, F, T);
+    else resp_static(P, t, p, v, w, m, c, F, T, tally);
   }
 }
 
@@ -950,7 +979,7 @@ __device__ __forceinline__ DColl sphere_dobject(const bdem_DOBJ& D, f3 p, float 
 /* Collision_Response_Particle_VolumeObject, Collision_Response.cuh:241-363,
  * for a lifter without spin (R_Body_COMVOBJ = VelVOBJ_Ang = 0). */
 __device__ __forceinline__ void resp_dobject(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
-                                             const DColl& c, f3 Vl, f3& Ftr, f3& Tq) {
+                                             const DColl& c, f3 Vl, f3& Ftr, f3& Tq, float* tally = nullptr) {
   const f3 Rc = c.cp - p;
   const f3 zero = mk3(0.f, 0.f, 0.f);
   const f3 RVel = v - Vl;
@@ -959,7 +988,8 @@ __device__ __forceinline__ void resp_dobject(const DevParams& P, int t, f3 p, f3
   const f3 Fdiss = P.d_cn[t] * Vnormal;
   Ftr = Fnormal * c.n - Fdiss;
   const f3 Fdir = Ftr / length(Ftr);
-  if (dot(Fdir, c.n) < 0.0f) Ftr = Fnormal * c.n;
+  const bool attractive = dot(Fdir, c.n) < 0.0f;
+  if (attractive) Ftr = Fnormal * c.n;
   const f3 V_CPA = v + cross(w, Rc);
   const f3 V_CPB = Vl + cross(zero, zero);
   const f3 V_CPNA = dot(V_CPA, c.n) * c.n;
@@ -976,6 +1006,7 @@ __device__ __forceinline__ void resp_dobject(const DevParams& P, int t, f3 p, f3
     const float tan_f = (0.050f * (m / P.dt)) * (P.d_fk[t] * (mag / Velf));
     Tang = (-1.0f * tan_f) * (Vt_Rel / mag);
   }
+  wall_tally(P, tally, TALLY_LIFT, attractive, Fdiss, Vnormal, Tang, mag);
   Ftr = Ftr + Tang;
   Tq = P.rotation ? cross(Rc, Ftr) : zero;
 }
@@ -984,7 +1015,7 @@ __device__ __forceinline__ void resp_dobject(const DevParams& P, int t, f3 p, f3
  * for the next step (0 none, 1 face, 2 edge).  Only the first lifter in
  * contact acts (`break`, DEM_Kernels_Compute.cuh:224). */
 __device__ __forceinline__ int lifter_forces(const DevParams& P, int t, f3 p, float r, f3 v, f3 w,
-                                             float m, int prev_type, f3& F, f3& T) {
+                                             float m, int prev_type, f3& F, f3& T, float* tally = nullptr) {
   F = mk3(0.f, 0.f, 0.f);
   T = F;
   DColl res{false, 0, 0.0f, F, F};
@@ -1002,7 +1033,7 @@ __device__ __forceinline__ int lifter_forces(const DevParams& P, int t, f3 p, fl
           if (D.is_attached_DWorld) Vl = drum_velocity(P, res.cp);
           else if (D.is_translating) Vl = cvt(D.velocity);
           f3 Ftr, Tq;
-          resp_dobject(P, t, p, v, w, m, res, Vl, Ftr, Tq);
+          resp_dobject(P, t, p, v, w, m, res, Vl, Ftr, Tq, tally);
           F = F + Ftr;
           if (P.rotation) T = T + Tq;
           break;

@@ -80,12 +80,14 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
                                               const float4* __restrict__ vel_m,
                                               const float4* __restrict__ omg_t, f3 pa, float ra, f3 va,
                                               float ma, f3 wa, int ta, const HistRec* hin, int hin_cnt,
-                                              int cursor, HistRec* hrec, f3& Fc, f3& Tc) {
+                                              int cursor, HistRec* hrec, f3& Fc, f3& Tc,
+                                              float* tally = nullptr) {
   const float4 pb = pos_r[j];
   const float4 vb = vel_m[j];
   const float4 ob = omg_t[j];
   if (LAW == BDEM_LAW_REFERENCE) {
-    pair_ref(P, pa, ra, va, ma, wa, ta, xyz(pb), pb.w, xyz(vb), vb.w, xyz(ob), packed_type(ob.w), Fc, Tc);
+    pair_ref(P, pa, ra, va, ma, wa, ta, xyz(pb), pb.w, xyz(vb), vb.w, xyz(ob), packed_type(ob.w), Fc, Tc,
+             tally);
   } else {
     const f3 rel = pa - xyz(pb);
     const float d = length(rel);
@@ -109,11 +111,12 @@ template <int LAW>
 __device__ __forceinline__ void particle_contact(const DevParams& P, int j, const float4* __restrict__ pos_r,
                                                  const float4* __restrict__ vel_m,
                                                  const float4* __restrict__ omg_t, f3 pa, float ra, f3 va,
-                                                 float ma, f3 wa, int ta, f3& F, f3& T, PairHistCtx* hc) {
+                                                 float ma, f3 wa, int ta, f3& F, f3& T, PairHistCtx* hc,
+                                                 float* tally = nullptr) {
   f3 Fc, Tc;
   HistRec* rec = (LAW != BDEM_LAW_REFERENCE && hc->out_cnt < HMAX) ? hc->out + hc->out_cnt : nullptr;
   contact_force<LAW>(P, j, pos_r, vel_m, omg_t, pa, ra, va, ma, wa, ta, hc->in, hc->in_cnt,
-                     hc->out_cnt, rec, Fc, Tc);
+                     hc->out_cnt, rec, Fc, Tc, tally);
   if (LAW != BDEM_LAW_REFERENCE && hc->out_cnt < HMAX) ++hc->out_cnt;
   F = F + Fc;
   if (P.rotation) T = T + Tc;
@@ -175,7 +178,8 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
                                                  const uint32_t* clist, int nl, int base_i,
                                                  const uint32_t* __restrict__ hwc_in,
                                                  const HistRec* __restrict__ hr_in, HistRec* hr_out,
-                                                 int hist_cap, WaveScratch& ws, f3& F, f3& T) {
+                                                 int hist_cap, WaveScratch& ws, f3& F, f3& T,
+                                                 float* tally = nullptr) {
   const int lane = threadIdx.x & 63;
   const int wbase_tid = threadIdx.x & ~63;
   int incl = nl;
@@ -208,7 +212,7 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
         if (k < HMAX) rec = hr_out + (size_t)opid * HMAX + k;
       }
       contact_force<LAW>(P, j, pos_r, vel_m, omg_t, xyz(po), po.w, xyz(vo), vo.w, xyz(oo),
-                         packed_type(oo.w), hin, hin_cnt, k, rec, Fc, Tc);
+                         packed_type(oo.w), hin, hin_cnt, k, rec, Fc, Tc, tally);
     }
     ws.f[0][lane] = Fc.x; ws.f[1][lane] = Fc.y; ws.f[2][lane] = Fc.z;
     ws.f[3][lane] = Tc.x; ws.f[4][lane] = Tc.y; ws.f[5][lane] = Tc.z;
@@ -227,7 +231,7 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
 /* Fused step: contacts -> pair forces (+gravity) -> walls -> integrate ->    */
 /* discharge check -> next key.  Reads the sorted scratch, writes the state.  */
 /* ------------------------------------------------------------------------ */
-template <int LAW, bool ORIENT, bool SKIN>
+template <int LAW, bool ORIENT, bool SKIN, bool ENERGY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_step(
     SkinArgs sk, uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
     const DevParams* __restrict__ Pp, int n, const uint2* __restrict__ cells,
@@ -242,6 +246,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out, int hist_cap,
     float* __restrict__ wall_force, int do_kill, uint32_t* status, int* kill_count) {
   const DevParams& P = *Pp;
+  float* const tally = ENERGY ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
   __shared__ uint32_t clist[CLIST * STEP_THREADS];
   /* per wave: candidate staging in phase 1, pair results in phase 2 */
   __shared__ __align__(16) unsigned char wlds[STEP_THREADS / 64][WAVE_LDS_BYTES];
@@ -295,7 +300,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   f3 Tpp = mk3(0.f, 0.f, 0.f);
   const int nl = nc < CLIST ? nc : CLIST;
   wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, base_i, hwc_in, hr_in, hr_out, hist_cap,
-                        *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp);
+                        *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp, tally);
 
   bool chg = false;
   float dmove = 0.0f;
@@ -312,7 +317,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         hc.out_cnt = CLIST;
       }
       for_each_contact(P, cells, pos_in, i, p, r, [&](int j, float4, int k) {
-        if (k >= CLIST) particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc);
+        if (k >= CLIST)
+          particle_contact<LAW>(P, j, pos_in, vel_in, omg_in, p, r, v, m, w, ta, Fpp, Tpp, &hc, tally);
         if (dbg_list && k >= CLIST && k < LIST_STRIDE) dbg_list[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
       });
       /* the reference's 32-entry list overflows (Collision_Detection_BroadPhase.cuh:481) */
@@ -332,14 +338,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       }
     }
     f3 Fw, Tw;
-    wall_forces(P, ta, p, r, v, w, m, Fw, Tw, &wh);
+    wall_forces(P, ta, p, r, v, w, m, Fw, Tw, &wh, tally);
     if (wall_force) {
       wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
     }
     /* a9: lifters; the contact type of the last check is kept for the next step */
     f3 Fl = mk3(0.f, 0.f, 0.f), Tl = Fl;
     uint32_t ltype = 0u;
-    if (P.num_dobj > 0) ltype = (uint32_t)lifter_forces(P, ta, p, r, v, w, m, (int)((hwc >> 24) & 3u), Fl, Tl);
+    if (P.num_dobj > 0)
+      ltype = (uint32_t)lifter_forces(P, ta, p, r, v, w, m, (int)((hwc >> 24) & 3u), Fl, Tl, tally);
     if (hwc_out) {
       uint32_t h = ltype << 24;
       if (LAW != BDEM_LAW_REFERENCE && hist_slot)
@@ -408,11 +415,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   }
 }
 
-template <int LAW, bool ORIENT, bool SKIN>
+template <int LAW, bool ORIENT, bool SKIN, bool ENERGY>
 static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
                           bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dbg_list, uint32_t* dbg_nn,
                           hipStream_t s) {
-  hipLaunchKernelGGL((k_step<LAW, ORIENT, SKIN>), dim3((b.n + 255) / 256), dim3(256), 0, s, sa, dbg_list, dbg_nn,
+  hipLaunchKernelGGL((k_step<LAW, ORIENT, SKIN, ENERGY>), dim3((b.n + 255) / 256), dim3(256), 0, s, sa, dbg_list, dbg_nn,
                      dP, b.n,
                      b.cells, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
                      b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta + META_APPEND + ap,
@@ -422,23 +429,32 @@ static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint3
 
 template <int LAW, bool ORIENT>
 static void launch_step_s(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
-                          bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dl, uint32_t* dn, hipStream_t s) {
-  if (sa.on) launch_step_t<LAW, ORIENT, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-  else launch_step_t<LAW, ORIENT, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+                          bool do_kill, bool want_wf, bool energy, const SkinArgs& sa, uint32_t* dl, uint32_t* dn,
+                          hipStream_t s) {
+  if constexpr (LAW == BDEM_LAW_REFERENCE) {
+    /* the dissipation tallies are defined for the reference law only */
+    if (energy) {
+      if (sa.on) launch_step_t<LAW, ORIENT, true, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+      else launch_step_t<LAW, ORIENT, false, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+      return;
+    }
+  }
+  if (sa.on) launch_step_t<LAW, ORIENT, true, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+  else launch_step_t<LAW, ORIENT, false, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
 }
 
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int law, bool orient,
-                 int hp, int ap, bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dl, uint32_t* dn,
-                 hipStream_t s) {
+                 int hp, int ap, bool do_kill, bool want_wf, bool energy, const SkinArgs& sa, uint32_t* dl,
+                 uint32_t* dn, hipStream_t s) {
   if (law == BDEM_LAW_REFERENCE) {
-    if (orient) launch_step_s<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-    else launch_step_s<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    if (orient) launch_step_s<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
   } else if (law == BDEM_LAW_LSD_COULOMB) {
-    if (orient) launch_step_s<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-    else launch_step_s<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    if (orient) launch_step_s<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
   } else {
-    if (orient) launch_step_s<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-    else launch_step_s<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    if (orient) launch_step_s<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
   }
 }
 

@@ -132,8 +132,12 @@ hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t
  * sorted keys) and appends changed slots to sbuf, counted in
  * meta[META_APPEND + append_par]. */
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
-                 bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force,
+                 bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force, bool energy,
                  const SkinArgs& sk, uint32_t* dbg_list, uint32_t* dbg_nn, hipStream_t s);
+/* Rotate_VolObject_Inc on the device: every lifter attached to the drum in
+ * the parameter block turned by the angle whose cosine and sine are c, s
+ * about (ax, ay, az), its faces re-derived.  One small launch. */
+void launch_turn_lifters(DevParams* dP, double c, double s, float ax, float ay, float az, hipStream_t st);
 /* Skin masks before a step: decide a new epoch (every row tested in full)
  * from the displacement bound, else flag the stencil rows whose cells a
  * particle entered or left.  Uses this step's cell table; par alternates. */
