@@ -97,7 +97,8 @@ def main():
                     help="after the timed steps, run the library's plain copy kernel (k_copy_u4, "
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
-    ap.add_argument("--workload", choices=("silo", "poly"), default="silo",
+    ap.add_argument("--revs", type=float, default=1.0, help="mill workload: revolutions to run")
+    ap.add_argument("--workload", choices=("silo", "poly", "mill"), default="silo",
                     help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box)")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of settling "
@@ -111,6 +112,8 @@ def main():
         sys.exit(2)
     if args.workload == "poly":
         return run_poly(args)
+    if args.workload == "mill":
+        return run_mill(args)
 
     import numpy as np
     import bdem
@@ -318,6 +321,60 @@ def run_poly(args):
                    "particles": n_live, "cells": ncells, "candidates_per_particle": round(k, 3),
                    "stage_ms_per_step": {s_: round(v / nrec, 5) for s_, v in stages.items()}},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_mill(args):
+    """The reference's ball-mill deck (Projects/3.BallMill, World Mill_SPH30:
+    242 steel balls of 5 cm in a 90 cm drum with 8 lifters, 22 RPM, dt 1e-5 s)
+    from its start file, --revs revolutions (272,727 steps each) with the lifters
+    turned on the device and the dissipation tallies on; the rate is
+    particle-steps/s and the mill power after the revolution is reported
+    beside it (after one revolution: the reference's known answer)."""
+    import numpy as np
+    from bdem.deck import Deck
+
+    d = Deck(os.path.join(ROOT, "tests", "golden", "decks", "3.BallMill"), "Mill_SPH30")
+    dev = d.device()
+    d.load_state(dev)
+    dev.reset_energy()
+    dev.drum_rotation(True)
+    a = d.info.rotate_about
+    dev.set_lifter_rotation(d.info.rad_per_step, (a.x, a.y, a.z))
+    steps = max(1, int(round(args.revs * d.info.steps_per_rev)))
+    n = d.sim.Num_Particles
+    dev.sync()
+    ta = time.perf_counter()
+    dev.step(steps)
+    dev.sync()
+    tb = time.perf_counter()
+    elapsed = tb - ta
+    power = dev.energy().astype(np.float64) / (steps * d.sim.InitalDelta_t)
+    log(f"[bench] mill {steps} steps in {elapsed:.2f} s; power particle {power[0]:.2f} lifter {power[1]:.2f} "
+        f"drum {power[2]:.2f} total {power.sum():.2f} W; status {dev.status()}")
+    dev.close()
+    out = {
+        "metric": "particle-steps/s, ball-mill deck Mill_SPH30",
+        "value": round(n * steps / elapsed, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(elapsed * 1e3 / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "reference project deck",
+        "config": {"workload": "Mill_SPH30: 242 balls of 5 cm, 90 cm drum, 8 lifters, 22 RPM, dt 1e-5 s, "
+                               "from StartMill_SPH30, lifters turned on the device, tallies on",
+                   "revolutions": round(steps / d.info.steps_per_rev, 4),
+                   "power_W": {"particle": round(power[0], 3), "lifter": round(power[1], 3),
+                               "drum": round(power[2], 3), "total": round(power.sum(), 3)},
+                   "published_power_W": {"particle": 319.724, "lifter": 156.378, "drum": 149.878,
+                                         "total": 625.98}},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
