@@ -84,7 +84,13 @@ template <class R> struct PairHist { int id; V3<R> xi; };
 template <class R> struct LawPair {
   R kn, kt, beta, mu;     /* LSD */
   R estar, gstar;         /* Hertz-Mindlin */
+  R tg;                   /* Hertz-Mindlin: tangential over normal damping, sqrt(4 G* / E*) */
 };
+
+/* sqrt(St / Sn) = sqrt(8 G* / (2 E*)) from the float constants, in double */
+inline float damping_ratio_t(float estar, float gstar) {
+  return estar > 0.0f ? float(std::sqrt(4.0 * double(gstar) / double(estar))) : 0.0f;
+}
 
 template <class R>
 class Oracle {
@@ -174,7 +180,8 @@ class Oracle {
           gstar = 1.0 / (2.0 * (2.0 - na) * (1.0 + na) / ea + 2.0 * (2.0 - nb) * (1.0 + nb) / eb);
         }
         law[a][b] = LawPair<R>{R(float(kn)), R(float(kn * ktr)), R(float(beta)), R(float(mu)),
-                               R(float(estar)), R(float(gstar))};
+                               R(float(estar)), R(float(gstar)),
+                               R(damping_ratio_t(float(estar), float(gstar)))};
       }
       const bdem_material& ma = ext.material[a];
       double e = ma.restitution;
@@ -189,7 +196,8 @@ class Oracle {
                        2.0 * (2.0 - nw) * (1.0 + nw) / ew);
       }
       wlaw[a] = LawPair<R>{R(ma.kn), R(float(double(ma.kn) * ma.kt_over_kn)), R(float(beta)),
-                           R(ma.mu), R(float(estar)), R(float(gstar))};
+                           R(ma.mu), R(float(estar)), R(float(gstar)),
+                           R(damping_ratio_t(float(estar), float(gstar)))};
     }
   }
 
@@ -459,7 +467,7 @@ class Oracle {
       R sn = R(2.0f) * lp.estar * srd;
       R st = R(8.0f) * lp.gstar * srd;
       R gn = R(-1.8257418583505538f) * lp.beta * rsqrt_(sn * mstar);   /* 2 sqrt(5/6) */
-      gt = R(-1.8257418583505538f) * lp.beta * rsqrt_(st * mstar);
+      gt = gn * lp.tg;                                                   /* sqrt(St m*) = tg sqrt(Sn m*) */
       fn = (R(4.0f) / R(3.0f)) * lp.estar * srd * delta - gn * vn;
       kt = st;
     }
@@ -492,13 +500,16 @@ class Oracle {
                        const std::vector<PairHist<R>>& hin, const std::vector<int>& hcin) const {
     V3<R> rel = pos[a] - pos[b];
     R d = length(rel);
-    V3<R> n = rel / d;
+    V3<R> n = rel * (R(1.0f) / d);
     R delta = (radius[a] + radius[b]) - d;
     V3<R> la = (R(-1.0f) * n) * radius[a];
     V3<R> lb = n * radius[b];
     V3<R> vc = (vel[a] + cross(avel[a], la)) - (vel[b] + cross(avel[b], lb));
-    R rstar = (radius[a] * radius[b]) / (radius[a] + radius[b]);
-    R mstar = (mass[a] * mass[b]) / (mass[a] + mass[b]);
+    /* R* = ra rb / (ra + rb) and m* = ma mb / (ma + mb) from one division */
+    R sr = radius[a] + radius[b], sm = mass[a] + mass[b];
+    R q = R(1.0f) / (sr * sm);
+    R rstar = ((radius[a] * radius[b]) * sm) * q;
+    R mstar = ((mass[a] * mass[b]) * sr) * q;
     V3<R> xi = find_spring(hin, hcin, HMAX, pid[a], pid[b]);
     V3<R> f = history_law(n, delta, vc, rstar, mstar, law[ptype[a]][ptype[b]], xi);
     F = F + f;

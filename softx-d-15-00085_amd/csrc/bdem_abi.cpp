@@ -117,6 +117,10 @@ void derive_laws(const bdem_ext_params& ext, DevParams& P) {
   auto beta_of = [&](double e) {
     return (e > 0.0 && e < 1.0) ? std::log(e) / std::sqrt(std::log(e) * std::log(e) + pi * pi) : 0.0;
   };
+  /* Hertz-Mindlin tangential damping over normal: sqrt(St / Sn) = sqrt(4 G* / E*) */
+  auto tg_of = [](float estar, float gstar) {
+    return estar > 0.0f ? float(std::sqrt(4.0 * double(gstar) / double(estar))) : 0.0f;
+  };
   for (int a = 0; a < MAX_TYPES; ++a) {
     const bdem_material& ma = ext.material[a];
     for (int b = 0; b < MAX_TYPES; ++b) {
@@ -132,7 +136,7 @@ void derive_laws(const bdem_ext_params& ext, DevParams& P) {
         gstar = 1.0 / (2.0 * (2.0 - na) * (1.0 + na) / ea + 2.0 * (2.0 - nb) * (1.0 + nb) / eb);
       }
       P.law_pp[a][b] = LawPair{float(kn), float(kn * ktr), float(beta_of(e)), float(mu),
-                               float(estar), float(gstar), 0.f, 0.f};
+                               float(estar), float(gstar), tg_of(float(estar), float(gstar)), 0.f};
     }
     const double ew = ext.wall_youngs > 0 ? ext.wall_youngs : ma.youngs;
     const double nw = ext.wall_youngs > 0 ? ext.wall_poisson : ma.poisson;
@@ -143,7 +147,7 @@ void derive_laws(const bdem_ext_params& ext, DevParams& P) {
                      2.0 * (2.0 - nw) * (1.0 + nw) / ew);
     }
     P.law_w[a] = LawPair{ma.kn, float(double(ma.kn) * ma.kt_over_kn), float(beta_of(ma.restitution)),
-                         ma.mu, float(estar), float(gstar), 0.f, 0.f};
+                         ma.mu, float(estar), float(gstar), tg_of(float(estar), float(gstar)), 0.f};
   }
 }
 

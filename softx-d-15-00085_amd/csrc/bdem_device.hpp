@@ -70,7 +70,7 @@ __device__ __forceinline__ int packed_type(float w) { return __float_as_int(w) &
 __host__ __device__ __forceinline__ int pack_id_type(int pid, int type) { return pid * 8 + (type & 7); }
 
 /* Per type-pair constants of the history laws (derived on the host). */
-struct LawPair { float kn, kt, beta, mu, estar, gstar, pad0, pad1; };
+struct LawPair { float kn, kt, beta, mu, estar, gstar, tg, pad1; };   /* tg = sqrt(4 G* / E*) */
 
 /* Tangential spring record: (xi.x, xi.y, xi.z, bits(partner id)). */
 typedef float4 HistRec;
@@ -623,7 +623,7 @@ __device__ __forceinline__ f3 history_law(const DevParams& P, f3 n, float delta,
     const float sn = 2.0f * lp.estar * srd;
     const float st = 8.0f * lp.gstar * srd;
     const float gn = -1.8257418583505538f * lp.beta * __builtin_sqrtf(sn * mstar);
-    gt = -1.8257418583505538f * lp.beta * __builtin_sqrtf(st * mstar);
+    gt = gn * lp.tg;   /* sqrt(St m*) = tg sqrt(Sn m*) */
     fn = (4.0f / 3.0f) * lp.estar * srd * delta - gn * vn;
     kt = st;
   }

@@ -91,13 +91,16 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
   } else {
     const f3 rel = pa - xyz(pb);
     const float d = length(rel);
-    const f3 n = rel / d;
+    const f3 n = rel * (1.0f / d);
     const float delta = (ra + pb.w) - d;
     const f3 la = (-1.0f * n) * ra;
     const f3 lb = n * pb.w;
     const f3 vc = (va + cross(wa, la)) - (xyz(vb) + cross(xyz(ob), lb));
-    const float rstar = (ra * pb.w) / (ra + pb.w);
-    const float mstar = (ma * vb.w) / (ma + vb.w);
+    /* R* and m* from one division */
+    const float sr = ra + pb.w, sm = ma + vb.w;
+    const float q = 1.0f / (sr * sm);
+    const float rstar = ((ra * pb.w) * sm) * q;
+    const float mstar = ((ma * vb.w) * sr) * q;
     const int idb = packed_pid(ob.w);
     f3 xi = find_spring(hin, hin_cnt, idb, cursor);
     Fc = history_law(P, n, delta, vc, rstar, mstar, P.law_pp[ta][packed_type(ob.w)], xi);
