@@ -132,6 +132,24 @@ int bdem_record_wall_forces(bdem_ctx* ctx, int on);
  * adds to them while SimulationInfo::EnergyCalc is set (spheres, reference
  * law), one float atomic per contact as in Collision_Response.cuh. */
 int bdem_get_energy(bdem_ctx* ctx, float* energy);
+/* Ensemble statistics of the current state (synchronous).  Over the live
+ * particles with y > min_y: how many (the silo's particles left above the
+ * hatch, Silo_Stats, main.cpp:253-285), their mean and largest speed in m/s
+ * (|v| * 0.01) and the P_ID of the fastest (Output_SystemState,
+ * Input/FileIO.h:286-318).  Over the live particles whose centre lies in the
+ * box [box_lo, box_hi) (both NULL: none): how many, and their solid
+ * fraction, the sum of 4/3 pi r^3 over the box volume (spheres; -1 for
+ * polyhedra), which has no reference counterpart. */
+typedef struct bdem_particle_stats {
+  int    count;
+  int    max_pid;              /* -1 when count is 0 */
+  double mean_speed;           /* m/s */
+  float  max_speed;            /* m/s */
+  int    box_count;
+  double box_solid_fraction;
+} bdem_particle_stats;
+int bdem_get_particle_stats(bdem_ctx* ctx, float min_y, const bdem_float3* box_lo, const bdem_float3* box_hi,
+                            bdem_particle_stats* out);
 /* Kernel warnings accumulated since the last call (BDEM_WARN_* bits); clears them. */
 int bdem_get_status(bdem_ctx* ctx, unsigned int* warnings);
 /* Particles removed by discharge checks so far. */

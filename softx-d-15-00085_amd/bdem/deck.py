@@ -118,6 +118,30 @@ class Deck:
         return d
 
 
+def run_silo(dev, deck, seconds, open_hatch=True):
+    """The silo branch of UpdateSim (main.cpp:85-98) with Silo_Stats
+    (main.cpp:253-285): open the hatch, then every flow_steps steps count the
+    particles left above it and record (time s, percent discharged).
+    Returns the list of records.  `deck` is the Deck the context came from."""
+    info = deck.info
+    if info.sim_type != 2:
+        raise L.BdemError("not a silo deck")
+    if open_hatch:
+        dev.open_hatch()
+    dt = deck.sim.InitalDelta_t
+    flow = max(int(info.flow_steps), 1)
+    total = int(round(seconds / dt))
+    n0 = dev.particle_stats(info.hatch_height)["count"]
+    out, done = [], 0
+    while done < total:
+        k = min(flow, total - done)
+        dev.step(k)
+        done += k
+        rem = dev.particle_stats(info.hatch_height)["count"]
+        out.append((done * dt, 100.0 * (n0 - rem) / n0 if n0 else 0.0))
+    return out
+
+
 def read_state(path, sim_type, rotating_world, particle_type, n):
     lib = L.load()
     hdr = T.StateHeader()

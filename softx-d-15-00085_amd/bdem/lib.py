@@ -53,6 +53,7 @@ _PROTOS = {
     "bdem_record_wall_forces": (C.c_int, [_P, C.c_int]),
     "bdem_get_energy": (C.c_int, [_P, _fp]),
     "bdem_get_status": (C.c_int, [_P, _up]),
+    "bdem_get_particle_stats": (C.c_int, [_P, C.c_float, _f3p, _f3p, C.POINTER(T.ParticleStats)]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
     "bdem_calibration_copy": (C.c_int, [_P, C.c_ulonglong, C.c_int]),
@@ -227,6 +228,19 @@ class Device:
             self.h, _ptr(pos, T.Float3), _ptr(quat, T.Quaterion), _ptr(vel, T.Float3),
             _ptr(avel, T.Float3), None, _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
         return dict(pos=pos, vel=vel, avel=avel, quat=quat, ptype=ptype, pid=pid)
+
+    def particle_stats(self, min_y=-np.inf, box=None):
+        """Live particles above min_y (count, mean and largest speed in m/s,
+        P_ID of the fastest) and the solid fraction of box = (lo, hi)."""
+        out = T.ParticleStats()
+        lo = hi = None
+        if box is not None:
+            lo, hi = T.f3(*box[0]), T.f3(*box[1])
+        self._check(self.lib.bdem_get_particle_stats(self.h, float(np.float32(min_y)) if np.isfinite(min_y) else
+                                                 float(min_y), None if lo is None else C.byref(lo),
+                                                 None if hi is None else C.byref(hi), C.byref(out)))
+        return dict(count=out.count, max_pid=out.max_pid, mean_speed=out.mean_speed, max_speed=out.max_speed,
+                    box_count=out.box_count, box_solid_fraction=out.box_solid_fraction)
 
     def status(self):
         w = C.c_uint(0)

@@ -240,6 +240,18 @@ class DeckInfo(C.Structure):
     ]
 
 
+class ParticleStats(C.Structure):
+    """bdem_particle_stats (include/bdem_abi.h)."""
+    _fields_ = [
+        ("count", C.c_int),
+        ("max_pid", C.c_int),
+        ("mean_speed", C.c_double),
+        ("max_speed", C.c_float),
+        ("box_count", C.c_int),
+        ("box_solid_fraction", C.c_double),
+    ]
+
+
 class StateHeader(C.Structure):
     """bdem_state_header (include/bdem_deck.h)."""
     _fields_ = [

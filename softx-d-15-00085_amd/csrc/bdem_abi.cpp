@@ -36,6 +36,7 @@ struct bdem_ctx {
   bool start_bin = false;
   float tallys[6] = {0, 0, 0, 0, 0, 0};
   float* d_tally = nullptr;  /* device tallies {particle, lifter, surface} the step kernel adds to */
+  StatsWords* d_stats = nullptr;   /* bdem_particle_stats reduction words */
   /* lifters turned on the device before every step (Rotate_VolObject_Inc) */
   bool lift_on = false;
   double lift_c = 1.0, lift_s = 0.0;
@@ -586,6 +587,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(c->d_tally, 8);
+  ALLOC(c->d_stats, 1);
   ALLOC(b.wall_force, (size_t)n * 3);
   if (c->poly) {
     PolyBuffers& pb = c->pb;
@@ -819,6 +821,42 @@ int bdem_get_energy(bdem_ctx* c, float* e) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(c->tallys, c->d_tally, 3 * sizeof(float), hipMemcpyDeviceToHost));
   for (int i = 0; i < 3; ++i) e[i] = float(double(c->tallys[i]) * 1E-4);
+  return BDEM_OK;
+}
+
+int bdem_get_particle_stats(bdem_ctx* c, float min_y, const bdem_float3* box_lo, const bdem_float3* box_hi,
+                            bdem_particle_stats* out) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!out) return fail(c, BDEM_EINVAL, "null argument");
+  if ((box_lo == nullptr) != (box_hi == nullptr)) return fail(c, BDEM_EINVAL, "give both box corners or neither");
+  HIPCHK(c, hipSetDevice(c->device));
+  /* the current state is in the *_cur streams, behind the last step on the stream */
+  float3 lo = make_float3(1.0f, 1.0f, 1.0f), hi = make_float3(0.0f, 0.0f, 0.0f);   /* empty box */
+  if (box_lo) {
+    lo = make_float3(box_lo->x, box_lo->y, box_lo->z);
+    hi = make_float3(box_hi->x, box_hi->y, box_hi->z);
+  }
+  const StatsWords zero{};
+  HIPCHK(c, hipMemcpyAsync(c->d_stats, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+  launch_stats(c->b, min_y, lo, hi, !c->poly, c->d_stats, c->stream);
+  HIPCHK(c, hipGetLastError());
+  StatsWords w{};
+  HIPCHK(c, hipMemcpyAsync(&w, c->d_stats, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out->count = (int)w.count;
+  out->mean_speed = w.count ? w.sum_speed / (double)w.count : 0.0;
+  if (w.best) {
+    const uint32_t bits = (uint32_t)(w.best >> 32);
+    std::memcpy(&out->max_speed, &bits, 4);
+    out->max_pid = (int)(uint32_t)(w.best & 0xFFFFFFFFull) - 1;
+  } else {
+    out->max_speed = 0.0f;
+    out->max_pid = -1;
+  }
+  out->box_count = (int)w.box_count;
+  const double bv = box_lo ? (double(hi.x) - lo.x) * (double(hi.y) - lo.y) * (double(hi.z) - lo.z) : 0.0;
+  out->box_solid_fraction = c->poly ? -1.0 : (bv > 0.0 ? w.box_volume / bv : 0.0);
   return BDEM_OK;
 }
 

@@ -152,6 +152,18 @@ void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuf
 void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s);
 
+/* Ensemble statistics of the current state (bdem_stats.hip): live
+ * particles with y > min_y (count, sum of |v|*0.01, the fastest as
+ * (speed bits << 32 | P_ID + 1)), and live centres inside [lo, hi) (count,
+ * sum of 4/3 pi r^3 when want_volume).  `out` must be zeroed first. */
+struct StatsWords {
+  uint32_t count, box_count;
+  double sum_speed, box_volume;
+  unsigned long long best;
+};
+void launch_stats(const StepBuffers& b, float min_y, float3 lo, float3 hi, bool want_volume, StatsWords* out,
+                  hipStream_t s);
+
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,
                          hipStream_t s);

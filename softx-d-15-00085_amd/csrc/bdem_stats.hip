@@ -1,0 +1,84 @@
+/*
+ * bdem_stats.hip - ensemble statistics of the current state, on the device.
+ *
+ * The reference computes these on the host from a full state download:
+ * the particles left above the hatch (Silo_Stats, main.cpp:253-285), and the
+ * mean and largest speed in m/s with the P_ID of the fastest particle
+ * (Output_SystemState, Input/FileIO.h:286-318).  Here one grid-stride
+ * kernel reduces them, plus the solid fraction of an axis-aligned box (no
+ * reference counterpart), into a few words: per wave by shuffles, then one
+ * atomic per wave.  Sums are in double, so the result does not depend on
+ * the order the waves finish in except in the last bits.
+ */
+#include "bdem_kernels.hpp"
+
+namespace bdem {
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_count(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_stats(int n, const float4* __restrict__ pos_r, const float4* __restrict__ vel_m,
+                                               const float4* __restrict__ omg_t, float min_y, float3 lo, float3 hi,
+                                               int want_volume, StatsWords* __restrict__ out) {
+  uint32_t cnt = 0u, bcnt = 0u;
+  double sum_v = 0.0, vol = 0.0;
+  unsigned long long best = 0ull;   /* (speed bits << 32) | (P_ID + 1); 0: none */
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 p = pos_r[i], v = vel_m[i], o = omg_t[i];
+    const int pid = packed_pid(o.w);
+    if (pid < 0) continue;
+    if (p.y > min_y) {
+      /* length(v * 0.010) as the reference's host code writes it */
+      const float sx = v.x * 0.010f, sy = v.y * 0.010f, sz = v.z * 0.010f;
+      const float sp = __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
+      ++cnt;
+      sum_v += (double)sp;
+      const unsigned long long key = ((unsigned long long)__float_as_uint(sp) << 32) | (uint32_t)(pid + 1);
+      best = key > best ? key : best;
+    }
+    if (p.x >= lo.x && p.x < hi.x && p.y >= lo.y && p.y < hi.y && p.z >= lo.z && p.z < hi.z) {
+      ++bcnt;
+      if (want_volume) {
+        const double r = (double)p.w;
+        vol += (4.0 / 3.0) * 3.14159265358979323846 * r * r * r;
+      }
+    }
+  }
+  cnt = wave_sum_count(cnt);
+  bcnt = wave_sum_count(bcnt);
+  sum_v = wave_sum_f64(sum_v);
+  vol = wave_sum_f64(vol);
+  best = wave_max_key(best);
+  if ((threadIdx.x & 63) == 0) {
+    if (cnt) atomicAdd(&out->count, cnt);
+    if (bcnt) atomicAdd(&out->box_count, bcnt);
+    if (sum_v != 0.0) atomicAdd(&out->sum_speed, sum_v);
+    if (vol != 0.0) atomicAdd(&out->box_volume, vol);
+    if (best) atomicMax(&out->best, best);
+  }
+}
+
+void launch_stats(const StepBuffers& b, float min_y, float3 lo, float3 hi, bool want_volume, StatsWords* out,
+                  hipStream_t s) {
+  const int blocks = b.n <= 0 ? 1 : min((b.n + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_stats, dim3(blocks), dim3(256), 0, s, b.n, b.pos_cur, b.vel_cur, b.omg_cur, min_y, lo, hi,
+                     want_volume ? 1 : 0, out);
+}
+
+}  // namespace bdem
