@@ -17,9 +17,9 @@
  *     8 of the 13 shipped World files use, which the reference rejects);
  *   - a triangular polyhedron face gets the area 0.5 |n| (the reference
  *     overwrites it with a value left from an earlier face or uninitialised,
- *     Input/KSimulationData.cpp:1494);
+ *     Input/KSimulationData.cpp:1475);
  *   - SpecType 4 files are written with the acceleration columns the reader
- *     expects (the reference writer omits them, FileIO.h:262-275).
+ *     expects (the reference writer omits them, FileIO.h:232-250).
  */
 #ifndef BDEM_DECK_H_
 #define BDEM_DECK_H_
@@ -122,12 +122,25 @@ int bdem_state_write(const char* path, int sim_type, int rotating_world, int par
                      const bdem_float3* vel, const bdem_float3* avel, const bdem_float3* acc,
                      const unsigned int* ptype, const int* pid, const bdem_POBJ* pobj, int num_pobj);
 
-/* Read a state file of this deck (path NULL: <project_dir>/<start_file>)
+/* The initial state of a deck without a start file (Read_FromFile 0):
+ * Set_Position_DefaultGrid (GridType 0) or Set_Position_Random_Grid
+ * (GridType 1: for polyhedra a random turn of up to 30 degrees about an
+ * unnormalised random axis in [0,1]^3, then a shuffle of types and IDs, all
+ * drawn from the C library's unseeded rand() sequence in the reference's
+ * order), Device/DeviceInterface.cu:192-601.
+ * Num_Particles entries each; ornt may be NULL.  BDEM_ENOTSUP for a start
+ * file or a mill fill (GridType 2). */
+int bdem_deck_initial_state(const bdem_deck* deck, bdem_float3* pos, bdem_Quaterion* ornt, unsigned int* ptype,
+                            int* pid);
+
+/* Read a state file of this deck (path NULL: <project_dir>/<start_file>, or
+ * the grid fill of bdem_deck_initial_state through bdem_set_pos when the
+ * deck has no start file)
  * and hand it to the context as Read_FromFile does: SpecType 1 through
  * bdem_set_pos, 2 bdem_set_pos_velocity, 3 and 4 bdem_set_system_state; a
  * mill file's energies, times 1E4, through bdem_set_tallys.  A mill file of
  * an unfinished revolution (RevComplete 0) turns the lifters to its
- * RotAngle (Rotate_VolObject_angle, Input/Processing_Mills.h:88-160) and
+ * RotAngle (Rotate_VolObject_angle, Input/Processing_Mills.h:92-160) and
  * uploads them.  hdr (may be NULL) receives the header. */
 int bdem_deck_load_state(bdem_deck* deck, bdem_ctx* ctx, const char* path, bdem_state_header* hdr);
 

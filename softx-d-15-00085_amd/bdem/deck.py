@@ -97,11 +97,31 @@ class Deck:
         return read_state(path, self.info.sim_type, self.info.rotating_world, self.info.particle_type,
                           self.sim.Num_Particles)
 
+    def initial_state(self):
+        """The grid fill of a deck without a start file (bdem_deck_initial_state):
+        positions, orientations, types and IDs; every velocity is the deck's
+        Velocity line and every spin zero, as Device_Set_SimulationData sets them."""
+        n = self.sim.Num_Particles
+        pos = np.zeros((n, 3), np.float32)
+        quat = np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1))
+        ptype = np.zeros(n, np.uint32)
+        pid = np.zeros(n, np.int32)
+        rc = self.lib.bdem_deck_initial_state(self.h, L._ptr(pos, T.Float3), L._ptr(quat, T.Quaterion),
+                                              L._ptr(ptype, C.c_uint), L._ptr(pid, C.c_int))
+        if rc != 0:
+            raise L.BdemError(f"bdem_deck_initial_state: status {rc}")
+        v0 = self.init.velocity[0]
+        vel = np.tile(np.array([v0.x, v0.y, v0.z], np.float32), (n, 1))
+        hdr = T.StateHeader()
+        return dict(header=hdr, rows=n, pos=pos, vel=vel, avel=np.zeros((n, 3), np.float32),
+                    acc=np.zeros((n, 3), np.float32), quat=quat, ptype=ptype, pid=pid)
+
     def scenario(self, path=None):
-        """The deck with its start state in the dict form the parity tests
-        and the oracle take (sim, wobj, pobj, dobj, pos, vel, avel, ptype,
-        pid, quat), plus the mill rotation (lifter_about, rad_per_step)."""
-        st = self.read_state(path)
+        """The deck with its start state (or grid fill) in the dict form the
+        parity tests and the oracle take (sim, wobj, pobj, dobj, pos, vel,
+        avel, ptype, pid, quat), plus the mill rotation (lifter_about,
+        rad_per_step)."""
+        st = self.read_state(path) if (path is not None or self.info.read_file == 1) else self.initial_state()
         n = self.sim.Num_Particles
         if st["rows"] != n:
             raise L.BdemError(f"state file holds {st['rows']} of {n} rows")

@@ -98,6 +98,7 @@ _PROTOS = {
     "bdem_state_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(T.StateHeader),
                                    _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip, C.POINTER(T.POBJ), C.c_int]),
     "bdem_deck_load_state": (C.c_int, [_P, _P, C.c_char_p, C.POINTER(T.StateHeader)]),
+    "bdem_deck_initial_state": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
 }
 
 SPRING_TC, SPRING_KN = 0, 1

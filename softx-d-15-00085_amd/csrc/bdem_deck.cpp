@@ -855,6 +855,116 @@ void rotate_lifters_to(bdem_deck& D, double angle) {
   }
 }
 
+/* rand() as the reference's host calls it, never seeded: the C library's
+ * default generator with seed 1.  glibc's default (TYPE_3) is an additive
+ * lagged-Fibonacci generator of degree 31 and separation 3, as published
+ * in its documentation and source: r[0] = seed, r[i] = 16807 r[i-1]
+ * mod (2^31 - 1) for i < 31, r[i] = r[i-31] for 31 <= i < 34, then
+ * r[i] = r[i-31] + r[i-3] (mod 2^32), and the k-th output is
+ * r[k + 344] >> 1. */
+class DefaultRand {
+ public:
+  DefaultRand() {
+    int32_t r[34];
+    r[0] = 1;
+    for (int i = 1; i < 31; ++i) {
+      const int64_t hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+      int64_t w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      r[i] = (int32_t)w;
+    }
+    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+    for (int i = 0; i < 34; ++i) ring_[i] = (uint32_t)r[i];
+    k_ = 34;
+    for (int i = 34; i < 344; ++i) next_raw();
+  }
+  int operator()() { return (int)(next_raw() >> 1); }
+
+ private:
+  uint32_t next_raw() {
+    const uint32_t v = ring_[(k_ - 31) % 34] + ring_[(k_ - 3) % 34];
+    ring_[k_ % 34] = v;
+    ++k_;
+    return v;
+  }
+  uint32_t ring_[34];
+  uint64_t k_;
+};
+constexpr float RAND_MAX_F = 2147483647.0f;   /* (float)RAND_MAX, as the reference divides by it */
+
+/* make_quaterion(theta_deg, x, y, z) then normalise (DataStructures/Quart.h:34-80) */
+bdem_Quaterion quat_deg(float theta, float x, float y, float z) {
+  theta = ((2.0f * 3.14152f) / 360.0f) * theta;   /* the header's own pi */
+  bdem_Quaterion q{std::cos(theta / 2.0f), std::sin(theta / 2.0f) * x, std::sin(theta / 2.0f) * y,
+                   std::sin(theta / 2.0f) * z};
+  const float sq = q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z;
+  const float n = sq > 0.0f ? 1.0f / std::sqrt(sq) : 0.0f;
+  return bdem_Quaterion{q.w * n, q.x * n, q.y * n, q.z * n};
+}
+
+/* Set_Position_DefaultGrid / Set_Position_Random_Grid
+ * (Device/DeviceInterface.cu:192-362, 435-601): the lattice from Start with
+ * pitch p_size[0] + Space, walked y outer, z middle, x inner. */
+int grid_fill(const bdem_deck& D, bdem_float3* pos, bdem_Quaterion* ornt, unsigned* ptype, int* pid) {
+  const bdem_InitConfig& G = D.init;
+  const int n = D.sim.Num_Particles, NX = G.num.x, NY = G.num.y, NZ = G.num.z;
+  const bool orient = D.info.particle_type == 1 || D.sim.sphere_orient;
+  if (NX < 1 || NY < 1 || NZ < 1 || (long long)NX * NY * NZ < n) return BDEM_EINVAL;
+  const int np = D.info.num_pobj;
+  DefaultRand rnd;
+  float x = G.start.x, y = G.start.y, z = G.start.z;
+  int sel = -1, selcount[8] = {0, 0, 0, 0, 0, 0, 0, 0}, typed = 0;
+  for (int yd = 0; yd < NY; ++yd) {
+    for (int zd = 0; zd < NZ; ++zd) {
+      for (int xd = 0; xd < NX; ++xd) {
+        const long long k = (long long)yd * NZ * NX + (long long)zd * NX + xd;
+        if (k < n) {
+          pos[k] = mk(x, y, z);
+          if (ornt) ornt[k] = bdem_Quaterion{1.0f, 0.0f, 0.0f, 0.0f};
+          if (G.grid_type == 1) {
+            if (orient) {
+              const float rw = (float)(((float)rnd() / RAND_MAX_F) * 30.0);
+              const float rx = (float)rnd() / RAND_MAX_F;
+              const float ry = (float)rnd() / RAND_MAX_F;
+              const float rz = (float)rnd() / RAND_MAX_F;
+              if (ornt) ornt[k] = quat_deg(rw, rx, ry, rz);
+            }
+            /* types in turn, skipping a type whose count is used up */
+            if (++sel >= np) sel = 0;
+            int pty = sel;
+            if (selcount[pty] >= D.per_object[pty])
+              for (int i = 0; i < np; ++i)
+                if (selcount[i] <= D.per_object[i] - 1) { pty = i; break; }
+            selcount[pty]++;
+            ptype[k] = (unsigned)pty;
+            pid[k] = (int)k;
+          }
+        }
+        x = x + G.p_size[0].x + G.space.x;
+      }
+      x = G.start.x;
+      z = z + G.p_size[0].z + G.space.z;
+    }
+    z = G.start.z;
+    y = y + G.p_size[0].y + G.space.y;
+  }
+  if (G.grid_type == 0) {
+    for (int i = 0; i < np; ++i)
+      for (int j = 0; j < D.per_object[i]; ++j, ++typed) {
+        ptype[typed] = (unsigned)i;
+        pid[typed] = typed;
+      }
+  } else if (n > 1) {
+    /* the shuffle of types and IDs (:562-575) */
+    for (int i = 0; i < n; ++i) {
+      const int a = rnd() % (n - 1), b = rnd() % (n - 1);
+      std::swap(ptype[a], ptype[b]);
+      std::swap(pid[a], pid[b]);
+    }
+  }
+  return BDEM_OK;
+}
+
 void fmt_g(std::string& out, double v) {
   char b[48];
   std::snprintf(b, sizeof(b), "%g", v);
@@ -1057,10 +1167,28 @@ int bdem_state_write(const char* path, int sim_type, int rotating_world, int par
   return (std::fclose(f) == 0 && ok) ? BDEM_OK : BDEM_EINVAL;
 }
 
+int bdem_deck_initial_state(const bdem_deck* D, bdem_float3* pos, bdem_Quaterion* ornt, unsigned int* ptype,
+                            int* pid) {
+  if (!D || !pos || !ptype || !pid) return BDEM_EINVAL;
+  if (D->info.read_file == 1 || D->init.grid_type > 1) return BDEM_ENOTSUP;
+  return grid_fill(*D, pos, ornt, ptype, pid);
+}
+
 int bdem_deck_load_state(bdem_deck* D, bdem_ctx* ctx, const char* path, bdem_state_header* hdr_out) {
   if (!D || !ctx) return BDEM_EINVAL;
+  if (!path && D->info.read_file != 1) {
+    /* no start file: the grid fill of Device_Set_SimulationData (:1771-1787) */
+    const int n = D->sim.Num_Particles;
+    std::vector<bdem_float3> pos(n);
+    std::vector<bdem_Quaterion> q(n, bdem_Quaterion{1.0f, 0.0f, 0.0f, 0.0f});
+    std::vector<unsigned> ptype(n);
+    std::vector<int> pid(n);
+    const int rc = bdem_deck_initial_state(D, pos.data(), q.data(), ptype.data(), pid.data());
+    if (rc != BDEM_OK) return rc;
+    if (hdr_out) std::memset(hdr_out, 0, sizeof(*hdr_out));
+    return bdem_set_pos(ctx, pos.data(), D->info.particle_type == 1 ? q.data() : nullptr, ptype.data(), pid.data());
+  }
   const std::string file = path ? std::string(path) : D->dir + "/" + D->info.start_file;
-  if (!path && D->info.read_file != 1) return BDEM_EINVAL;
   const int n = D->sim.Num_Particles;
   std::vector<bdem_float3> pos(n), vel(n), avel(n), acc(n);
   std::vector<bdem_Quaterion> q(n, bdem_Quaterion{1.0f, 0.0f, 0.0f, 0.0f});

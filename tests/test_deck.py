@@ -334,3 +334,99 @@ def test_deck_scenario_matches_the_state_file():
     assert sc["pos"].shape == (242, 3) and len(sc["dobj"]) == 8
     assert sc["lifter_about"] == (45.0, 45.0, 0.0) and sc["rad_per_step"] == d.info.rad_per_step
     assert sc["quat"] is None and sc["sim"].Num_Particles == 242
+
+
+# ---- grid fills (decks without a start file) ------------------------------
+
+def test_default_grid_fill_is_the_lattice():
+    d = deck("1.Particles_Box", "Spheres_128cm")
+    st = d.initial_state()
+    g = d.init
+    n = d.sim.Num_Particles
+    assert (g.num.x, g.num.y, g.num.z) == (32, 32, 32) and n == 32768
+    pitch = 2 * d.pobj[0].radius + np.array([g.space.x, g.space.y, g.space.z], np.float64)
+    k = np.arange(n)
+    ijk = np.stack([k % 32, k // (32 * 32), (k // 32) % 32], axis=1)   # x inner, z middle, y outer
+    want = np.array([g.start.x, g.start.y, g.start.z]) + ijk * pitch
+    assert np.allclose(st["pos"], want, atol=2e-4)
+    assert np.array_equal(st["pid"], k) and not st["ptype"].any()
+    assert np.all(st["vel"] == np.float32([0.1, 0.0, 0.1]))
+
+
+def _random_grid_deck(tmp_path, project, world, spec):
+    """A copy of a shipped project whose World file asks for GridType 1."""
+    import shutil
+    dst = tmp_path / project
+    shutil.copytree(os.path.join(DECKS, project), dst)
+    wf = dst / "World" / (world + ".World")
+    text = open(wf).read()
+    text = re.sub(r"(GridType\(1=ran\):\s*)0", r"\g<1>1", text, count=1)
+    for key, val in spec.items():
+        text = re.sub(key, val, text, count=1)
+    open(wf, "w").write(text)
+    return Deck(str(dst), world)
+
+
+def _libc_random_grid(n, nx, ny, nz, n_types, counts, orient):
+    """Set_Position_Random_Grid's draws (Device/DeviceInterface.cu:435-575)
+    with the C library's own rand(), seeded as an unseeded program starts."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    rmax = np.float32(2147483647)
+    quats, types, ids = [], [], []
+    sel, cnt = -1, [0] * n_types
+    for k in range(nx * ny * nz):
+        if orient:
+            w = np.float32(np.float32(np.float32(libc.rand()) / rmax) * 30.0)
+            x, y, z = (np.float32(np.float32(libc.rand()) / rmax) for _ in range(3))
+            quats.append((w, x, y, z))
+        sel = 0 if sel + 1 >= n_types else sel + 1
+        pty = sel
+        if cnt[pty] >= counts[pty]:
+            pty = next(i for i in range(n_types) if cnt[i] <= counts[i] - 1)
+        cnt[pty] += 1
+        types.append(pty)
+        ids.append(k)
+    types, ids = np.array(types[:n]), np.array(ids[:n])
+    for _ in range(n):
+        a = libc.rand() % (n - 1)
+        b = libc.rand() % (n - 1)
+        types[a], types[b] = types[b], types[a]
+        ids[a], ids[b] = ids[b], ids[a]
+    return quats[:n], types, ids
+
+
+def test_random_grid_fill_matches_the_c_library_draws(tmp_path):
+    d = _random_grid_deck(tmp_path, "1.Particles_Box", "Poly_128cm", {})
+    st = d.initial_state()
+    n = d.sim.Num_Particles
+    quats, types, ids = _libc_random_grid(n, 16, 32, 16, 1, [n], True)
+    assert np.array_equal(st["ptype"], types) and np.array_equal(st["pid"], ids)
+    assert sorted(st["pid"]) == list(range(n))
+    q = st["quat"].astype(np.float64)
+    assert np.allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-6)
+    # make_quaterion(theta deg, axis) with the header's pi 3.14152, then normalised
+    for k in (0, 1, 2, 777, n - 1):
+        w, x, y, z = (float(v) for v in quats[k])
+        th = np.float32(np.float32(2.0 * 3.14152 / 360.0) * np.float32(w))
+        c, s = np.cos(th / 2), np.sin(th / 2)
+        raw = np.array([c, s * x, s * y, s * z])
+        assert np.allclose(q[k], raw / np.linalg.norm(raw), atol=2e-6)
+    # a turn of up to 30 degrees about an unnormalised axis in [0,1]^3, the
+    # quaternion normalised afterwards: the turn reaches 2 atan(sqrt(3) tan 15 deg)
+    angles = 2 * np.degrees(np.arccos(np.clip(q[:, 0], -1, 1)))
+    assert angles.max() <= 2 * np.degrees(np.arctan(np.sqrt(3) * np.tan(np.radians(15.0)))) + 1e-3
+    assert angles.mean() > 10.0
+
+
+def test_random_grid_fill_alternates_types(tmp_path):
+    d = _random_grid_deck(tmp_path, "2.Silo", "SPH_30",
+                          {r"Read_FromFile:\s*1\s*\n\s*FileName:\s*\S+":
+                           "Read_FromFile: 0\nGridType(1=ran): 1\nGridSize(X,Y,Z): 13 30 5\n"
+                           "Start(X,Y,Z): 0.55 30.60 0.55\nSpace: 0.05 0.05 0.20\nVelocity: 0.0 -50.0 25.0"})
+    st = d.initial_state()
+    n = d.sim.Num_Particles
+    _, types, ids = _libc_random_grid(n, 13, 30, 5, 2, [1425, 525], False)
+    assert np.array_equal(st["ptype"], types) and np.array_equal(st["pid"], ids)
+    assert np.bincount(st["ptype"]).tolist() == [1425, 525]

@@ -3,9 +3,11 @@
 Each case loads a shipped deck with the deck reader (bdem_deck_load), sets
 a context up from it (bdem_deck_setup) and reads its start file into the
 context (bdem_deck_load_state), exactly as the reference's main() does with
-KSimulationData, the KDevice constructor and Read_FromFile; the oracle gets
-the same deck and start state.  Mill decks turn their lifters before every
-step, as UpdateSim does (main.cpp:114-121).
+KSimulationData, the KDevice constructor and Read_FromFile -- or, for the
+decks without a start file, lays the particles on the deck's grid as
+Device_Set_SimulationData does; the oracle gets the same deck and state.
+Mill decks turn their lifters before every step, as UpdateSim does
+(main.cpp:114-121).
 """
 import os
 
@@ -24,6 +26,8 @@ DECKS = os.path.join(ROOT, "tests", "golden", "decks")
 TOL = 1e-5
 
 CASES = [
+    ("1.Particles_Box", "Spheres_128cm", 100),   # grid fill (Set_Position_DefaultGrid), the run selector's deck
+    ("1.Particles_Box", "Poly_128cm", 20),       # grid fill, 8192 dodecahedra
     ("3.BallMill", "Mill_SPH30", 300),
     ("3.BallMill", "Mill_SPH20", 300),
     ("2.Silo", "SPH_30", 300),
