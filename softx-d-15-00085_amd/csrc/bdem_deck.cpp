@@ -537,11 +537,10 @@ void repeat_lifters(std::vector<bdem_DOBJ>& L, int types, int per_type, float R)
 }
 
 /* ---- .World (KSimulationData constructor, :27-656) ---- */
-bool read_world(bdem_deck& D, Err& e) {
+bool read_world(bdem_deck& D, const std::string& world, Err& e) {
   bdem_deck_info& I = D.info;
   Reader r;
-  if (!r.open(D.dir + "/World/" + std::string(I.title) + ".World")) return e.set(r.error());
-  I.title[0] = 0;
+  if (!r.open(D.dir + "/World/" + world + ".World")) return e.set(r.error());
   r.word();
   copy_str(I.title, sizeof(I.title), r.word());
   float dt = 0.0f;
@@ -1007,9 +1006,8 @@ int bdem_deck_load(bdem_deck** out, const char* project_dir, const char* world, 
   std::memset(&D->init, 0, sizeof(D->init));
   std::memset(D->per_object, 0, sizeof(D->per_object));
   D->dir = project_dir;
-  copy_str(D->info.title, sizeof(D->info.title), world);   /* the World name until the title is read */
   Err e;
-  if (!read_world(*D, e)) {
+  if (!read_world(*D, world, e)) {
     if (err && err_cap > 0) copy_str(err, (size_t)err_cap, e.msg.empty() ? "deck error" : e.msg);
     delete D;
     return BDEM_EINVAL;
