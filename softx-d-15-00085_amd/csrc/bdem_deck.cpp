@@ -537,10 +537,10 @@ void repeat_lifters(std::vector<bdem_DOBJ>& L, int types, int per_type, float R)
 }
 
 /* ---- .World (KSimulationData constructor, :27-656) ---- */
-bool read_world(bdem_deck& D, const std::string& world, Err& e) {
+bool read_world(bdem_deck& D, const std::string& world_name, Err& e) {
   bdem_deck_info& I = D.info;
   Reader r;
-  if (!r.open(D.dir + "/World/" + world + ".World")) return e.set(r.error());
+  if (!r.open(D.dir + "/World/" + world_name + ".World")) return e.set(r.error());
   r.word();
   copy_str(I.title, sizeof(I.title), r.word());
   float dt = 0.0f;
