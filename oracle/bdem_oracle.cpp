@@ -143,6 +143,18 @@ int bo_forces(void* h, double* fpp, double* tpp, double* fw, double* tw, int* so
   return 0;
 }
 
+/* Lifter force and torque of the current state (sorted order), as bo_forces. */
+int bo_lifter_forces(void* h, double* fl, double* tl, int* sorted_pid) {
+  WITH(h, {
+    auto c = o;
+    c.prepare();
+    c.compute_forces();
+    put3(c.f_l, fl); put3(c.t_l, tl);
+    if (sorted_pid) for (int i = 0; i < c.N; ++i) sorted_pid[i] = c.pid[i];
+  });
+  return 0;
+}
+
 int bo_step(void* h, int nsteps) {
   WITH(h, { for (int s = 0; s < nsteps; ++s) o.step(); });
   return 0;
@@ -174,6 +186,9 @@ int bo_set_dobj(void* h, const bdem_DOBJ* d, int nd) {
   WITH(h, o.dobj.assign(d, d + (d ? nd : 0)));
   return 0;
 }
+
+/* Polyhedra that touched a lifter in the last step. */
+int bo_lifter_contacts(void* h) { WITH(h, return o.lifter_contacts); return 0; }
 
 /* The lifter table as it stands (after any turning). */
 int bo_get_dobj(void* h, bdem_DOBJ* out) {

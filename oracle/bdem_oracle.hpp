@@ -136,6 +136,7 @@ class Oracle {
   float tally[3] = {0.f, 0.f, 0.f};
   double tally_exact[3] = {0.0, 0.0, 0.0};
   mutable std::vector<std::vector<R>> incs[3];   /* this step's increments, per particle */
+  int lifter_contacts = 0;           /* polyhedra in contact with a lifter in the last step */
   int warn_out_of_grid = 0;
   int warn_list_overflow = 0;
   int warn_hist_overflow = 0;
@@ -1386,6 +1387,271 @@ class Oracle {
     Tq = sim.Rotation ? cross(Rc, Ftr) : zero;
   }
 
+  /* ---- polyhedra against lifters (VolumeObject_InteractionParticle,
+   * DEM_Kernels_Compute.cuh:100-224) ---- */
+
+  /* is_SP_VolumeObject_Poly (Device/Utilities.cuh:272-309): a lifter face
+   * with the particle's centre in front and every particle vertex in front. */
+  bool sp_dobj_poly(const bdem_DOBJ& D, V3<R> PosP, const PolyRot& A) const {
+    for (int f = 0; f < D.num_faces; ++f) {
+      const V3<R> n = cvt<R>(D.faces[f].normal), c = cvt<R>(D.faces[f].centroid);
+      if (double(dot(n, PosP - c)) > 0.0) {
+        bool contact = false;
+        for (int v = 0; v < A.nv; ++v)
+          if (dot(n, A.V[v] - c) <= R(0.00000f)) { contact = true; break; }
+        if (!contact) return true;
+      }
+    }
+    return false;
+  }
+
+  /* is_SP_Poly_VolumeObject (Device/Utilities.cuh:212-247): a particle face
+   * with the lifter's COM in front and every lifter vertex in front. */
+  bool sp_poly_dobj(const PolyRot& A, V3<R> PosD, const bdem_DOBJ& D) const {
+    for (int f = 0; f < A.nf; ++f) {
+      if (double(dot(A.Fn[f], PosD - A.Fc[f])) > 0.0) {
+        bool contact = false;
+        for (int v = 0; v < D.num_vertex; ++v)
+          if (dot(A.Fn[f], cvt<R>(D.vertex[v]) - A.Fc[f]) <= R(0.00000f)) { contact = true; break; }
+        if (!contact) return true;
+      }
+    }
+    return false;
+  }
+
+  /* Collision_Detection_Vertex_InDObject_3Faces (Collision_Detection_Polyhedra.cuh:383-476):
+   * behind (or within 1e-5 of) every lifter face; the three closest faces
+   * that have the particle's centre in front. */
+  InPoly vertex_in_dobj(V3<R> PosP, V3<R> v, const bdem_DOBJ& D) const {
+    InPoly r{false, {-1, -1, -1}, {R(10.0f), R(10.0f), R(10.0f)}};
+    for (int f = 0; f < D.num_faces; ++f) {
+      const V3<R> n = cvt<R>(D.faces[f].normal), c = cvt<R>(D.faces[f].centroid);
+      const R vd = dot(n, v - c);
+      if (!(vd <= R(0.000010f))) return r;
+      if (dot(n, PosP - c) > R(0.0f)) {
+        const R a = std::fabs(vd);
+        if (a < r.dist[0]) {
+          if (r.face[0] > -1) {
+            if (r.face[1] > -1) { r.dist[2] = r.dist[1]; r.face[2] = r.face[1]; }
+            r.dist[1] = r.dist[0]; r.face[1] = r.face[0];
+          }
+          r.dist[0] = a; r.face[0] = f;
+        } else if (a < r.dist[1]) {
+          if (r.face[1] > -1) { r.dist[2] = r.dist[1]; r.face[2] = r.face[1]; }
+          r.dist[1] = a; r.face[1] = f;
+        } else if (a < r.dist[2]) {
+          r.dist[2] = a; r.face[2] = f;
+        }
+      }
+    }
+    r.inside = true;
+    return r;
+  }
+
+  /* Lifter face f; f = -1 (no face had the centre in front) reads as a zero
+   * plane, as for polyhedra. */
+  V3<R> dnorm(const bdem_DOBJ& D, int f) const { return f >= 0 ? cvt<R>(D.faces[f].normal) : mk<R>(0, 0, 0); }
+  V3<R> dcent(const bdem_DOBJ& D, int f) const { return f >= 0 ? cvt<R>(D.faces[f].centroid) : mk<R>(0, 0, 0); }
+
+  bool ray_hits_dface(V3<R> PosP, V3<R> through, const bdem_DOBJ& D, const V3<R>* DV, int f, R tol) const {
+    if (f < 0) return false;
+    V3<R> Ray = through - PosP;
+    Ray = Ray / length(Ray);
+    const R dir = dot(Ray, dnorm(D, f));
+    if (!(std::fabs(dir) > R(0))) return false;
+    const R d = dot((dcent(D, f) - PosP), dnorm(D, f)) / dir;
+    return point_on_poly_face(D.faces[f], DV, PosP + d * Ray, tol);
+  }
+
+  /* Collision_Detection_Polyhedara_VolumeObject_VertexFace
+   * (Collision_Detection_Polyhedra.cuh:1638-1990): particle vertices inside
+   * the lifter; one -> vertex-face, two -> edge-face (ray tests against the
+   * three closest faces), more -> the face every inside vertex has among its
+   * three closest.  The first eight inside vertices count. */
+  PColl vertex_faces_dobj(V3<R> PosP, const PolyRot& A, const bdem_DOBJ& D) const {
+    PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
+    V3<R> DV[8];
+    for (int k = 0; k < D.num_vertex && k < 8; ++k) DV[k] = cvt<R>(D.vertex[k]);
+    int np = 0, valid[8];
+    InPoly sel[8];
+    for (int v = 0; v < A.nv; ++v) {
+      InPoly t = vertex_in_dobj(PosP, A.V[v], D);
+      if (t.inside) {
+        if (np < 8) { sel[np] = t; valid[np] = v; }
+        ++np;
+      }
+    }
+    if (np == 0) return res;
+    np = std::min(np, 8);
+    res.hit = true;
+    const InPoly& s0 = sel[0];
+    if (np == 1) {
+      res.cls = C_VF;
+      res.dis = s0.dist[0];
+      res.n = dnorm(D, s0.face[0]);
+      res.cp = A.V[valid[0]];
+      if (s0.face[1] == -1) return res;
+      if (ray_hits_dface(PosP, A.V[valid[0]], D, DV, s0.face[0], R(0.1f))) return res;
+      for (int q = 1; q < 3; ++q) {
+        if (s0.face[q] > -1 && ray_hits_dface(PosP, A.V[valid[0]], D, DV, s0.face[q], R(0.10f))) {
+          res.n = dnorm(D, s0.face[q]);
+          res.dis = s0.dist[q];
+          return res;
+        }
+      }
+      return res;
+    }
+    auto depth = [&](int f) { return std::fabs(dot(dnorm(D, f), (res.cp - dcent(D, f)))); };
+    if (np == 2) {
+      res.cls = C_EF;
+      res.cp = (A.V[valid[0]] + A.V[valid[1]]) * R(0.50f);
+      res.n = dnorm(D, s0.face[0]);
+      res.dis = depth(s0.face[0]);
+      if (s0.face[1] == -1 && sel[1].face[1] == -1) return res;
+      for (int q = 0; q < 3; ++q) {
+        if (q > 0 && !(s0.face[q] > -1)) continue;
+        if (ray_hits_dface(PosP, res.cp, D, DV, s0.face[q], R(0.10f))) {
+          res.dis = depth(s0.face[q]);
+          res.n = dnorm(D, s0.face[q]);
+          return res;
+        }
+      }
+      return res;
+    }
+    res.cls = C_FF;
+    V3<R> cp = mk<R>(R(0.0f), R(0.0f), R(0.0f));
+    for (int q = 0; q < np; ++q) cp = cp + A.V[valid[q]];
+    res.cp = cp / R(float(np));
+    /* count, per lifter face, the inside vertices that have it among their
+     * three closest, with the smallest distance */
+    int common[3] = {0, 0, 0};
+    R cdis[3] = {R(10.0), R(10.0), R(10.0)};
+    for (int q = 0; q < np; ++q)
+      for (int j = 0; j < 3; ++j) {
+        const int f = sel[q].face[j];
+        if (f > -1 && f < 3) {
+          common[f]++;
+          if (sel[q].dist[j] < cdis[f]) cdis[f] = sel[q].dist[j];
+        }
+      }
+    for (int f = 0; f < D.num_faces && f < 3; ++f)
+      if (common[f] == np) {
+        res.dis = cdis[f];
+        res.n = dnorm(D, f);
+        return res;
+      }
+    return res;   /* no common face: normal and depth left unset (zero here) */
+  }
+
+  /* Collision_Detection_DObject_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:716-890):
+   * closest crossing of a lifter edge (A) and a particle edge (B) where the
+   * lifter's point is nearer the particle's centre.  The reference leaves
+   * the normal unset when no such crossing is within 0.1 cm of the other
+   * point; that case is no contact here. */
+  PColl edges_dobj(const bdem_DOBJ& D, const PolyRot& B, const bdem_POBJ& pb, V3<R> PosB, V3<R> RelPos) const {
+    PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_SP};
+    R min_edgedis = R(0.10f);
+    V3<R> selA = mk<R>(0, 0, 0), selB = selA, nsel = selA, cpsum = selA;
+    int count = 0;
+    bool found = false, have_min = false;
+    for (int i = 0; i < D.num_edges; ++i) {
+      const V3<R> PA = cvt<R>(D.vertex[D.edge[i].point_vindex[0]]);
+      const V3<R> DA = cvt<R>(D.vertex[D.edge[i].point_vindex[1]]) - PA;
+      const R DEAD = dot(DA, DA);
+      for (int j = 0; j < pb.num_edges; ++j) {
+        const V3<R> PB = B.V[pb.edge[j].point_vindex[0]];
+        const V3<R> DB = B.V[pb.edge[j].point_vindex[1]] - PB;
+        const V3<R> dir = cross(DA / length(DA), DB / length(DB));
+        const R mag = dot(dir, dir);
+        if (!(mag > R(0.00000f))) continue;
+        V3<R> DV = PA - PB;
+        const R DEBD = dot(DB, DB);
+        const R DEABD = dot(DA, DB);
+        const R DEADAB = dot(DA, DV);
+        const R DEBDAB = dot(DB, DV);
+        const R detJinv = R(1.00000f) / (DEABD * DEABD - DEAD * DEBD);
+        const R dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+        if (!(dA >= R(-0.010f) && dA <= R(1.0100f))) continue;
+        const R dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+        if (!(dB >= R(-0.0100f) && dB <= R(1.010f))) continue;
+        const V3<R> pA = PA + DA * dA;
+        const V3<R> pB = PB + DB * dB;
+        DV = (pA - pB);
+        R dis = dot(DV, DV);
+        if (!(dis > R(0.00000f))) continue;
+        dis = rsqrt_(dis);
+        if (length(PosB - pB) > length(PosB - pA)) {
+          ++count;
+          cpsum = cpsum + pA;
+          found = true;
+          if (dis <= min_edgedis) {
+            min_edgedis = dis;
+            nsel = cross(DA, DB);
+            selA = pA;
+            selB = pB;
+            have_min = true;
+          }
+        }
+      }
+    }
+    if (!found || !have_min) return res;
+    V3<R> n = nsel * (R(1.0f) / length(nsel));
+    res.hit = true;
+    res.cls = C_EE;
+    const R dn = dot(RelPos, n);
+    if (std::fabs(dn) > R(0.0f)) {
+      res.n = R(-1.0f) * ((dot(RelPos, n) / std::fabs(dot(RelPos, n))) * n);
+      res.dis = std::fabs(dot(res.n, (selB - selA)));
+      res.cp = cpsum / R(float(count));
+    } else {
+      res.n = R(-1.0f) * RelPos / length(RelPos);
+      res.cp = PosB;
+      res.dis = std::fabs(dot(res.n, (selB - selA))) + R(0.000010f);
+    }
+    return res;
+  }
+
+  /* Lifter force and torque on polyhedron i (the particle_type 1 branch of
+   * VolumeObject_InteractionParticle): first lifter in contact acts. */
+  void lifter_forces_poly(int i, const PolyRot& A, V3<R>& F, V3<R>& T) const {
+    F = mk<R>(0, 0, 0);
+    T = mk<R>(0, 0, 0);
+    const bdem_POBJ& pa = pobj[ptype[i]];
+    const R r = R(pa.radius);
+    const V3<R> P = pos[i];
+    const bool check = dobj[0].is_translating || !sim.isMillSim || sim.Rotating_WObject ||
+                       length(mk<R>(P.x, P.y, R(0.0)) - cvt<R>(sim.cent)) >
+                           (R(sim.DObject_CylinderBoundR) - R(2.0f) * r);
+    if (!check) return;
+    for (size_t d = 0; d < dobj.size(); ++d) {
+      const bdem_DOBJ& D = dobj[d];
+      const V3<R> BV = (P - cvt<R>(D.COM)) * cvt<R>(D.Axis);
+      if (!(length(BV) < (R(D.boundR) + r))) continue;
+      PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_SP};
+      bool sp = sp_dobj_poly(D, P, A);
+      if (!sp) sp = sp_poly_dobj(A, cvt<R>(D.COM), D);
+      if (!sp) {
+        V3<R> PosD = cvt<R>(D.COM);
+        PosD.z = P.z;
+        res = vertex_faces_dobj(P, A, D);
+        if (!res.hit) {
+          const V3<R> RelPos = (PosD - P) / length(PosD - P);
+          res = edges_dobj(D, A, pa, P, RelPos);
+        }
+      }
+      if (!res.hit) continue;
+      V3<R> Vl = mk<R>(0, 0, 0);
+      if (D.is_attached_DWorld) Vl = drum_velocity(res.cp);
+      else if (D.is_translating) Vl = cvt<R>(D.velocity);
+      const DColl c{true, 0, res.dis, res.n, res.cp};
+      V3<R> Ftr, Tq;
+      resp_dobject(i, c, Vl, Ftr, Tq);
+      F = F + Ftr;
+      if (sim.Rotation) T = T + Tq;
+      break;
+    }
+  }
+
   /* Lifter force and torque on particle i; returns the contact type kept
    * for the next step (0 none, 1 face, 2 edge).  Only the first lifter in
    * contact acts (`break`, DEM_Kernels_Compute.cuh:224). */
@@ -1437,8 +1703,8 @@ class Oracle {
     }
     int hov = 0;
     if (poly()) {
-      int lov = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(| : lov)
+      int lov = 0, lhit = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(| : lov) reduction(+ : lhit)
       for (int i = 0; i < N; ++i) {
         if (pid[i] < 0) continue;
         V3<R> F = cvt<R>(sim.Force_Field) * mass[i];
@@ -1454,8 +1720,16 @@ class Oracle {
         wall_forces_poly(i, A, Fw, Tw);
         f_w[i] = Fw;
         t_w[i] = Tw;
+        if (!dobj.empty()) {
+          V3<R> Fl, Tl;
+          lifter_forces_poly(i, A, Fl, Tl);
+          f_l[i] = Fl;
+          t_l[i] = Tl;
+          if (dot(Fl, Fl) > R(0)) lhit = lhit + 1;
+        }
       }
       if (lov) warn_list_overflow = 1;
+      lifter_contacts = lhit;
       return;
     }
 #pragma omp parallel for schedule(static) reduction(| : hov)

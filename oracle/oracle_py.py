@@ -52,6 +52,8 @@ def load():
     lib.bo_angular_acc.argtypes = [P, C.c_int, dp, dp]
     lib.bo_poly_walls.argtypes = [P, C.c_int, dp]
     lib.bo_get_dobj.argtypes = [P, C.c_void_p]
+    lib.bo_lifter_contacts.argtypes = [P]
+    lib.bo_lifter_forces.argtypes = [P, dp, dp, ip]
     lib.bo_turn_lifters.argtypes = [P, C.c_double, C.c_float, C.c_float, C.c_float]
     lib.bo_run.argtypes = [P, C.c_int, C.c_double, C.c_float, C.c_float, C.c_float]
     lib.bo_energy.argtypes = [P, fp, dp]
@@ -143,6 +145,14 @@ class Oracle:
         self.lib.bo_forces(self.h, *[_p(a, C.c_double) for a in out], _p(pid, C.c_int))
         return out[0], out[1], out[2], out[3], pid
 
+    def lifter_forces(self):
+        """Lifter force and torque of the current state, by P_ID."""
+        fl, tl = np.zeros((self.n, 3)), np.zeros((self.n, 3))
+        pid = np.zeros(self.n, np.int32)
+        self.lib.bo_lifter_forces(self.h, _p(fl, C.c_double), _p(tl, C.c_double), _p(pid, C.c_int))
+        order = np.argsort(pid, kind="stable")
+        return fl[order], tl[order]
+
     def step(self, n=1):
         self.lib.bo_step(self.h, int(n))
 
@@ -154,6 +164,10 @@ class Oracle:
         arr = (dobj[0].__class__ * len(dobj))(*dobj) if dobj else None
         self._d = arr
         self.lib.bo_set_dobj(self.h, arr, len(dobj) if dobj else 0)
+
+    def lifter_contacts(self):
+        """Polyhedra that touched a lifter in the last step."""
+        return int(self.lib.bo_lifter_contacts(self.h))
 
     def get_dobj(self):
         """The lifter table as it stands (after any turning)."""

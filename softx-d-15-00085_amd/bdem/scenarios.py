@@ -812,3 +812,26 @@ def poly_pair_deck(pa, pb, qa, qb, shape_a=(CUBE_VERTS, CUBE_FACES), shape_b=Non
     d["avel"] = np.array([wa, wb], np.float32)
     d["quat"] = np.array([qa, qb], np.float32)
     return d
+
+
+def poly_mill_deck(n=300, seed=11, size=1.0, gap=0.6, R=15.0, L=10.0, n_lifters=8, lifter_w=1.5,
+                   lifter_h=1.5, rpm=30.0, density=2700.0, kn=2e5, e=0.4, mu=0.6, dt=1e-5, vmax=10.0):
+    """Cubes of side `size` in the drum of mill_deck (8 attached lifters),
+    randomly oriented; the first ones sit against every lifter face and top
+    edge with their centre `gap`*size from it, so a vertex, an edge or a face
+    of the cube is inside the lifter (a randomly turned cube reaches 0.5 to
+    0.87 of its side along any direction)."""
+    rb = 0.5 * math.sqrt(3.0) * size
+    deck = mill_deck(n=n, seed=seed, R=R, L=L, n_lifters=n_lifters, lifter_w=lifter_w, lifter_h=lifter_h,
+                     rpm=rpm, radius=rb, density=density, kn=kn, e=e, mu=mu, dt=dt, vmax=vmax)
+    pobj = [poly_pobj(CUBE_VERTS * size, CUBE_FACES, density, e, kn, mu)]
+    cross_pair_params(pobj, kn, e, mu)
+    deck["pobj"] = pobj
+    deck["sim"].particle_type = 1
+    deck["sim"].Num_ParticleObjects = 1
+    deck["quat"] = random_quats(seed, n)
+    deck["avel"] = np.zeros((n, 3), np.float32)
+    placed = place_at_lifters(dict(deck, radius=np.full(n, gap * size, np.float32)), seed=seed + 1, overlap=0.0)
+    deck["pos"] = placed["pos"]
+    deck["name"] = f"polymill{n}"
+    return deck

@@ -523,10 +523,10 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     if (rc0) return rc0;
   }
   if (sim->particle_type == 1) {
-    if (num_dobj > 0)
-      return fail(c, BDEM_ENOTSUP, "lifters with polyhedra (Collision_Detection_Polyhedra.cuh:716-890) are not implemented");
     if (sim->use_hist)
-      return fail(c, BDEM_ENOTSUP, "polyhedra with contact history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) are not implemented");
+      return fail(c, BDEM_ENOTSUP,
+                  "polyhedra contact-type history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) is "
+                  "disabled in the reference (main.cpp:830) and undefined as written; not built");
     int rc0 = check_poly(c, pobj, num_pobj);
     if (rc0) return rc0;
   }
@@ -977,7 +977,6 @@ int bdem_set_lifter_rotation(bdem_ctx* c, double rad_per_step, float ax, float a
   if (rc) return rc;
   if (!std::isfinite(rad_per_step) || !std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az))
     return fail(c, BDEM_EINVAL, "rotation angle and axis point must be finite");
-  if (c->poly && rad_per_step != 0.0) return fail(c, BDEM_ENOTSUP, "lifters are not supported with polyhedra");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->lift_on = rad_per_step != 0.0;
@@ -1038,7 +1037,7 @@ int bdem_update_sim(bdem_ctx* c) {
   if (ev) c->t_lists[(size_t)c->t_used] = sa.on ? 1 : 0;
   if (c->poly)
     launch_poly_step(c->dp, c->dpobj, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
-                     c->record_wf, c->stream);
+                     c->record_wf, !c->dobj.empty(), c->stream);
   else
     launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
                 c->append_par, do_kill, c->record_wf, c->sim.EnergyCalc, sa, c->dbg_list, c->dbg_nn, c->stream);

@@ -148,7 +148,7 @@ void launch_skin(const DevParams* dP, const StepBuffers& b, const uint32_t* old_
  * with the same outputs as launch_step; and the force stage alone. */
 void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sorted_keys, int append_par, bool do_kill, bool want_wall_force,
-                      hipStream_t s);
+                      bool lifters, hipStream_t s);
 void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s);
 

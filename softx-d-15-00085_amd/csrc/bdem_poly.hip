@@ -581,6 +581,276 @@ __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p
   }
 }
 
+/* ---- polyhedra against lifters (VolumeObject_InteractionParticle,
+ * DEM_Kernels_Compute.cuh:100-224), one thread per particle: lifter contacts
+ * are few, so the serial form of the restatement is kept. */
+
+/* is_SP_VolumeObject_Poly (Utilities.cuh:272-309) */
+__device__ __forceinline__ bool sp_dobj_poly(const bdem_DOBJ& D, f3 PosP, const Shape& A) {
+  for (int f = 0; f < D.num_faces; ++f) {
+    const f3 n = cvt(D.faces[f].normal), c = cvt(D.faces[f].centroid);
+    if (dot(n, PosP - c) > 0.0f) {
+      bool contact = false;
+      for (int v = 0; v < A.nv; ++v)
+        if (dot(n, ld3(A.V + v) - c) <= 0.00000f) { contact = true; break; }
+      if (!contact) return true;
+    }
+  }
+  return false;
+}
+
+/* is_SP_Poly_VolumeObject (Utilities.cuh:212-247) */
+__device__ __forceinline__ bool sp_poly_dobj(const Shape& A, f3 PosD, const bdem_DOBJ& D) {
+  for (int f = 0; f < A.nf; ++f) {
+    const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
+    if (dot(n, PosD - c) > 0.0f) {
+      bool contact = false;
+      for (int v = 0; v < D.num_vertex; ++v)
+        if (dot(n, cvt(D.vertex[v]) - c) <= 0.00000f) { contact = true; break; }
+      if (!contact) return true;
+    }
+  }
+  return false;
+}
+
+/* Collision_Detection_Vertex_InDObject_3Faces (Collision_Detection_Polyhedra.cuh:383-476) */
+__device__ __forceinline__ bool vertex_in_dobj(f3 PosP, f3 v, const bdem_DOBJ& D, InPoly& r) {
+  r = InPoly{-1, -1, -1, 10.0f, 10.0f, 10.0f};
+  for (int f = 0; f < D.num_faces; ++f) {
+    const f3 n = cvt(D.faces[f].normal), c = cvt(D.faces[f].centroid);
+    const float vd = dot(n, v - c);
+    if (!(vd <= 0.000010f)) return false;
+    if (dot(n, PosP - c) > 0.0f) {
+      const float a = fabsf(vd);
+      if (a < r.d0) {
+        if (r.f0 > -1) {
+          if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
+          r.d1 = r.d0; r.f1 = r.f0;
+        }
+        r.d0 = a; r.f0 = f;
+      } else if (a < r.d1) {
+        if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
+        r.d1 = a; r.f1 = f;
+      } else if (a < r.d2) {
+        r.d2 = a; r.f2 = f;
+      }
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ f3 dnorm(const bdem_DOBJ& D, int f) {
+  return f >= 0 ? cvt(D.faces[f].normal) : mk3(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ f3 dcent(const bdem_DOBJ& D, int f) {
+  return f >= 0 ? cvt(D.faces[f].centroid) : mk3(0.f, 0.f, 0.f);
+}
+__device__ __forceinline__ float dist_at(const InPoly& s, int q) { return q == 0 ? s.d0 : (q == 1 ? s.d1 : s.d2); }
+
+/* is_PointOnSurface with an explicit tolerance over a lifter's vertices */
+__device__ __forceinline__ bool point_on_dface(const bdem_Surface& f, const bdem_float3* V, f3 point, float tol) {
+  float area = 0.000000f;
+  const int nv = f.num_vertex;
+  for (int q = 0; q < nv - 1; ++q) {
+    const f3 V1 = point - cvt(V[f.vertex_Order[q]]);
+    const f3 V2 = point - cvt(V[f.vertex_Order[q + 1]]);
+    area = area + 0.500000f * length(cross(V1, V2));
+  }
+  const f3 V1 = point - cvt(V[f.vertex_Order[nv - 1]]);
+  const f3 V2 = point - cvt(V[f.vertex_Order[0]]);
+  area = area + 0.500000f * length(cross(V1, V2));
+  return (fabsf(area - f.area) / f.area) * 100.0000f <= tol;
+}
+
+__device__ __forceinline__ bool ray_hits_dface(f3 PosP, f3 through, const bdem_DOBJ& D, int f, float tol) {
+  if (f < 0) return false;
+  f3 Ray = through - PosP;
+  Ray = Ray / length(Ray);
+  const float dir = dot(Ray, dnorm(D, f));
+  if (!(fabsf(dir) > 0.0f)) return false;
+  const float d = dot((dcent(D, f) - PosP), dnorm(D, f)) / dir;
+  return point_on_dface(D.faces[f], D.vertex, PosP + d * Ray, tol);
+}
+
+/* Collision_Detection_Polyhedara_VolumeObject_VertexFace (Collision_Detection_Polyhedra.cuh:1638-1990) */
+__device__ __forceinline__ PColl vertex_faces_dobj(f3 PosP, const Shape& A, const bdem_DOBJ& D) {
+  PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+  int np = 0, valid[8];
+  InPoly sel[8];
+  for (int v = 0; v < A.nv; ++v) {
+    InPoly t;
+    if (vertex_in_dobj(PosP, ld3(A.V + v), D, t)) {
+      if (np < 8) { sel[np] = t; valid[np] = v; }
+      ++np;
+    }
+  }
+  if (np == 0) return res;
+  np = min(np, 8);
+  res.hit = true;
+  const InPoly s0 = sel[0];
+  if (np == 1) {
+    res.cls = C_VF;
+    res.dis = s0.d0;
+    res.n = dnorm(D, s0.f0);
+    res.cp = ld3(A.V + valid[0]);
+    if (s0.f1 == -1) return res;
+    if (ray_hits_dface(PosP, res.cp, D, s0.f0, 0.1f)) return res;
+    for (int q = 1; q < 3; ++q) {
+      const int f = face_at(s0, q);
+      if (f > -1 && ray_hits_dface(PosP, res.cp, D, f, 0.10f)) {
+        res.n = dnorm(D, f);
+        res.dis = dist_at(s0, q);
+        return res;
+      }
+    }
+    return res;
+  }
+  if (np == 2) {
+    res.cls = C_EF;
+    res.cp = (ld3(A.V + valid[0]) + ld3(A.V + valid[1])) * 0.50f;
+    res.n = dnorm(D, s0.f0);
+    res.dis = fabsf(dot(dnorm(D, s0.f0), (res.cp - dcent(D, s0.f0))));
+    if (s0.f1 == -1 && sel[1].f1 == -1) return res;
+    for (int q = 0; q < 3; ++q) {
+      const int f = face_at(s0, q);
+      if (q > 0 && !(f > -1)) continue;
+      if (ray_hits_dface(PosP, res.cp, D, f, 0.10f)) {
+        res.dis = fabsf(dot(dnorm(D, f), (res.cp - dcent(D, f))));
+        res.n = dnorm(D, f);
+        return res;
+      }
+    }
+    return res;
+  }
+  res.cls = C_FF;
+  f3 cp = mk3(0.0f, 0.0f, 0.0f);
+  for (int q = 0; q < np; ++q) cp = cp + ld3(A.V + valid[q]);
+  res.cp = cp / (float)np;
+  int common[3] = {0, 0, 0};
+  float cdis[3] = {10.0f, 10.0f, 10.0f};
+  for (int q = 0; q < np; ++q)
+    for (int j = 0; j < 3; ++j) {
+      const int f = face_at(sel[q], j);
+      if (f > -1 && f < 3) {
+        common[f]++;
+        const float dj = dist_at(sel[q], j);
+        if (dj < cdis[f]) cdis[f] = dj;
+      }
+    }
+  for (int f = 0; f < D.num_faces && f < 3; ++f)
+    if (common[f] == np) {
+      res.dis = cdis[f];
+      res.n = dnorm(D, f);
+      return res;
+    }
+  return res;
+}
+
+/* Collision_Detection_DObject_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:716-890);
+ * no crossing within 0.1 cm of the other point (the reference then reads an
+ * unset normal) is no contact. */
+__device__ __forceinline__ PColl edges_dobj(const bdem_DOBJ& D, const Shape& B, const bdem_POBJ& pb, f3 PosB,
+                                            f3 RelPos) {
+  PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
+  float min_edgedis = 0.10f;
+  f3 selA = mk3(0.f, 0.f, 0.f), selB = selA, nsel = selA, cpsum = selA;
+  int count = 0;
+  bool found = false, have_min = false;
+  for (int i = 0; i < D.num_edges; ++i) {
+    const f3 PA = cvt(D.vertex[D.edge[i].point_vindex[0]]);
+    const f3 DA = cvt(D.vertex[D.edge[i].point_vindex[1]]) - PA;
+    const float DEAD = dot(DA, DA);
+    for (int j = 0; j < pb.num_edges; ++j) {
+      const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
+      const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
+      const f3 dir = cross(DA / length(DA), DB / length(DB));
+      const float mag = dot(dir, dir);
+      if (!(mag > 0.00000f)) continue;
+      f3 DV = PA - PB;
+      const float DEBD = dot(DB, DB);
+      const float DEABD = dot(DA, DB);
+      const float DEADAB = dot(DA, DV);
+      const float DEBDAB = dot(DB, DV);
+      const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
+      const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+      if (!(dA >= -0.010f && dA <= 1.0100f)) continue;
+      const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+      if (!(dB >= -0.0100f && dB <= 1.010f)) continue;
+      const f3 pA = PA + DA * dA;
+      const f3 pB = PB + DB * dB;
+      DV = (pA - pB);
+      float dis = dot(DV, DV);
+      if (!(dis > 0.00000f)) continue;
+      dis = __builtin_sqrtf(dis);
+      if (length(PosB - pB) > length(PosB - pA)) {
+        ++count;
+        cpsum = cpsum + pA;
+        found = true;
+        if (dis <= min_edgedis) {
+          min_edgedis = dis;
+          nsel = cross(DA, DB);
+          selA = pA;
+          selB = pB;
+          have_min = true;
+        }
+      }
+    }
+  }
+  if (!found || !have_min) return res;
+  const f3 n = nsel * (1.0f / length(nsel));
+  res.hit = true;
+  res.cls = C_EE;
+  const float dn = dot(RelPos, n);
+  if (fabsf(dn) > 0.0f) {
+    res.n = -1.0f * ((dot(RelPos, n) / fabsf(dot(RelPos, n))) * n);
+    res.dis = fabsf(dot(res.n, (selB - selA)));
+    res.cp = cpsum / (float)count;
+  } else {
+    res.n = -1.0f * RelPos / length(RelPos);
+    res.cp = PosB;
+    res.dis = fabsf(dot(res.n, (selB - selA))) + 0.000010f;
+  }
+  return res;
+}
+
+/* Lifter force and torque on a polyhedron: the first lifter in contact acts. */
+__device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
+                                                   const Shape& A, const bdem_POBJ& pa, f3& F, f3& T) {
+  F = mk3(0.f, 0.f, 0.f);
+  T = F;
+  const float r = pa.radius;
+  const bool check = P.dobj[0].is_translating || !P.mill_sim || P.rotating_wobj ||
+                     length(mk3(p.x, p.y, 0.0f) - cvt(P.mill_cent)) > (P.dobj_bound_r - 2.0f * r);
+  if (!check) return;
+  for (int d = 0; d < P.num_dobj; ++d) {
+    const bdem_DOBJ& D = P.dobj[d];
+    const f3 BV = (p - cvt(D.COM)) * cvt(D.Axis);
+    if (!(length(BV) < (D.boundR + r))) continue;
+    PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+    bool sp = sp_dobj_poly(D, p, A);
+    if (!sp) sp = sp_poly_dobj(A, cvt(D.COM), D);
+    if (!sp) {
+      f3 PosD = cvt(D.COM);
+      PosD.z = p.z;
+      res = vertex_faces_dobj(p, A, D);
+      if (!res.hit) {
+        const f3 RelPos = (PosD - p) / length(PosD - p);
+        res = edges_dobj(D, A, pa, p, RelPos);
+      }
+    }
+    if (!res.hit) continue;
+    f3 Vl = mk3(0.f, 0.f, 0.f);
+    if (D.is_attached_DWorld) Vl = drum_velocity(P, res.cp);
+    else if (D.is_translating) Vl = cvt(D.velocity);
+    const DColl c{true, 0, res.dis, res.n, res.cp};
+    f3 Ftr, Tq;
+    resp_dobject(P, t, p, v, w, m, c, Vl, Ftr, Tq);
+    F = F + Ftr;
+    if (P.rotation) T = T + Tq;
+    break;
+  }
+}
+
 }  // namespace
 
 /* ------------------------------------------------------------------ kernels */
@@ -749,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
 /* Per particle: gravity, the pair slots in list order, walls, integration,
  * discharge, next key, changed flag and changed-list append (as k_step).
  * FORCES_ONLY stops after the forces and writes them (debug stage). */
-template <bool FORCES_ONLY>
+template <bool FORCES_ONLY, bool LIFTERS>
 __global__ __launch_bounds__(256) void k_poly_finish(
     const DevParams* __restrict__ Pp, const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
     const float4* __restrict__ rot, const uint32_t* __restrict__ nn, const float4* __restrict__ pf,
@@ -789,6 +1059,8 @@ __global__ __launch_bounds__(256) void k_poly_finish(
       const Shape A = shape_of(rot, L, i, po);
       f3 Fw, Tw;
       wall_forces_poly(P, ta, p, r, v, w, m, A, Fw, Tw);
+      f3 Fl = mk3(0.f, 0.f, 0.f), Tl = Fl;
+      if (LIFTERS && P.num_dobj > 0) lifter_forces_poly(P, ta, p, v, w, m, A, po, Fl, Tl);
       if (FORCES_ONLY) {
         fpp[3 * i] = Fpp.x; fpp[3 * i + 1] = Fpp.y; fpp[3 * i + 2] = Fpp.z;
         tpp[3 * i] = Tpp.x; tpp[3 * i + 1] = Tpp.y; tpp[3 * i + 2] = Tpp.z;
@@ -799,8 +1071,7 @@ __global__ __launch_bounds__(256) void k_poly_finish(
       if (wall_force) {
         wall_force[3 * i] = Fw.x; wall_force[3 * i + 1] = Fw.y; wall_force[3 * i + 2] = Fw.z;
       }
-      const f3 zero = mk3(0.f, 0.f, 0.f);
-      integrate(P, 0.0f, m, Fpp, Tpp, Fw, Tw, zero, zero, p, v, w, &q, po.InertiaT);
+      integrate(P, 0.0f, m, Fpp, Tpp, Fw, Tw, Fl, Tl, p, v, w, &q, po.InertiaT);
       /* Remove_Silo_Discharged_Particles, GPU_Memory_Transfer.cuh:650-664 */
       if (do_kill && p.y < P.kill_height) {
         p = cvt(P.max_size);
@@ -862,9 +1133,11 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_POBJ* dpob
 }
 
 void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
-                      const uint32_t* sk, int ap, bool do_kill, bool want_wf, hipStream_t s) {
+                      const uint32_t* sk, int ap, bool do_kill, bool want_wf, bool lifters, hipStream_t s) {
   launch_poly_forces_common(dP, dpobj, b, pb, s);
-  hipLaunchKernelGGL(k_poly_finish<false>, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+  /* the lifter search needs private arrays (scratch): only its own instantiation carries it */
+  auto k = lifters ? k_poly_finish<false, true> : k_poly_finish<false, false>;
+  hipLaunchKernelGGL(k, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur,
                      b.vel_cur, b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf,
                      b.meta + META_APPEND + ap, b.hwc_srt, b.hwc_cur, want_wf ? b.wall_force : nullptr,
@@ -874,7 +1147,7 @@ void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuf
 void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s) {
   launch_poly_forces_common(dP, dpobj, b, pb, s);
-  hipLaunchKernelGGL(k_poly_finish<true>, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+  hipLaunchKernelGGL((k_poly_finish<true, false>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, nullptr, nullptr,
                      nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                      b.status, b.kill_count, fpp, tpp, fw, tw);

@@ -229,3 +229,53 @@ def test_poly_pack_fp32_tracks_fp64():
     assert np.all(np.isfinite(sa["pos"]))
     err = np.abs(sa["pos"][oa] - sb["pos"][ob]).max()
     assert err < 1e-3, err
+
+
+# ---- polyhedra against a lifter (VolumeObject_InteractionParticle, particle_type 1) ----
+
+def _qmul(a, b):
+    w1, x1, y1, z1 = (float(c) for c in a)
+    w2, x2, y2, z2 = (float(c) for c in b)
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2], np.float32)
+
+
+def _lifter_case(centre, q):
+    """A unit cube at rest (no gravity) near one lifter: x 3..7, y 0..1.5,
+    z 1..9 (top face y = 1.5, side faces x = 3 and x = 7); the second cube of
+    the pair deck is far away."""
+    deck = S.poly_pair_deck(centre, (9, 8, 9), q, I, gravity=False)
+    V = [(3, 0, 1), (3, 0, 9), (7, 0, 9), (7, 0, 1), (3, 1.5, 1), (3, 1.5, 9), (7, 1.5, 9), (7, 1.5, 1)]
+    deck["dobj"] = [S.lifter_dobj(V, (5, 0.75, 5), 4.0, attached=False)]
+    return deck, deck["dobj"][0]
+
+
+CORNER_DOWN = _qmul(S.axis_quat((0, 0, 1), -90), _q_corner_to_x())   # corner (1,1,1)/sqrt(3) to -y
+
+
+@pytest.mark.parametrize("name,centre,q,face", [
+    ("face-face", (5, 1.5 + 0.5 - D, 5), I, 2),
+    ("edge-face", (5, 1.5 + S2 - D, 5), S.axis_quat((0, 0, 1), 45), 2),
+    ("vertex-face", (5, 1.5 + S3 - D, 5), CORNER_DOWN, 2),
+    ("side vertex-face", (3 - S3 + D, 1.0, 5), _q_corner_to_x(), 0),
+])
+@pytest.mark.parametrize("bits", [64, 32])
+def test_cube_against_lifter_closed_forms(name, centre, q, face, bits):
+    deck, L = _lifter_case(centre, q)
+    n = np.array([L.faces[face].normal.x, L.faces[face].normal.y, L.faces[face].normal.z])
+    o = Oracle(deck, bits=bits)
+    fl, tl = o.lifter_forces()
+    kn = deck["pobj"][0].Dsurface_Kn
+    # at rest: the lifter spring alone, Dsurface_Kn * depth along the face normal, no moment
+    assert fl[0] == pytest.approx(kn * D * n, rel=2e-4, abs=1e-3)
+    assert np.linalg.norm(tl[0]) < 1e-4 * np.linalg.norm(fl[0])
+    assert np.all(fl[1] == 0.0)
+    o.close()
+
+
+def test_cube_clear_of_the_lifter_feels_nothing():
+    deck, _ = _lifter_case((5, 1.5 + 0.5 + 0.02, 5), I)
+    o = Oracle(deck, bits=32)
+    fl, tl = o.lifter_forces()
+    assert np.all(fl == 0.0) and np.all(tl == 0.0)
+    o.close()
