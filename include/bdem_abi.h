@@ -150,6 +150,21 @@ typedef struct bdem_particle_stats {
 } bdem_particle_stats;
 int bdem_get_particle_stats(bdem_ctx* ctx, float min_y, const bdem_float3* box_lo, const bdem_float3* box_hi,
                             bdem_particle_stats* out);
+/* Free-surface profile of the current state (synchronous), for the angle
+ * of repose; no reference counterpart (the reference computes no angle).
+ * [lo, hi) along axis 0 (x) or 2 (z) is cut into `bins` (1..4096) equal
+ * bins; top[k] receives the highest particle top y + r among the live
+ * particles whose centre falls in bin k (polyhedra: centre of mass plus the
+ * bounding radius), NaN for an empty bin.  A centre's bin is
+ * (int)((u - lo) * inv_w) with inv_w = (float)bins / (hi - lo), in float. */
+int bdem_get_surface_profile(bdem_ctx* ctx, int axis, float lo, float hi, int bins, float* top);
+/* Host only: least-squares line h = intercept + slope * u through the
+ * non-empty (non-NaN) bins of a profile, u at the bin centres
+ * lo + (k + 0.5)(hi - lo)/bins, in double; angle_deg = atan|slope| in
+ * degrees.  BDEM_EINVAL with fewer than two non-empty bins.  Any output
+ * may be NULL. */
+int bdem_surface_angle(const float* top, int bins, float lo, float hi, double* slope, double* intercept,
+                       double* angle_deg);
 /* Kernel warnings accumulated since the last call (BDEM_WARN_* bits); clears them. */
 int bdem_get_status(bdem_ctx* ctx, unsigned int* warnings);
 /* Particles removed by discharge checks so far. */

@@ -55,6 +55,8 @@ _PROTOS = {
     "bdem_get_status": (C.c_int, [_P, _up]),
     "bdem_get_particle_stats": (C.c_int, [_P, C.c_float, _f3p, _f3p, C.POINTER(T.ParticleStats)]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
+    "bdem_get_surface_profile": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
+    "bdem_surface_angle": (C.c_int, [_fp, C.c_int, C.c_float, C.c_float, _dp, _dp, _dp]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
     "bdem_calibration_copy": (C.c_int, [_P, C.c_ulonglong, C.c_int]),
     "bdem_num_particles": (C.c_int, [_P]),
@@ -117,6 +119,19 @@ def derive_spring_constants(pobj, mode, cor_pp, cor_surface, cor_dsurface, tc=(0
     for k in range(len(pobj)):
         pobj[k] = arr[k]
     return pobj
+
+
+def surface_angle(top, lo, hi):
+    """bdem_surface_angle (host only): least-squares line through the
+    non-empty bins of a surface profile -> (slope, intercept, angle in degrees)."""
+    lib = load()
+    top = np.ascontiguousarray(top, np.float32)
+    sl, ic, ang = C.c_double(), C.c_double(), C.c_double()
+    rc = lib.bdem_surface_angle(_ptr(top, C.c_float), len(top), float(lo), float(hi), C.byref(sl), C.byref(ic),
+                                C.byref(ang))
+    if rc != 0:
+        raise BdemError(f"bdem_surface_angle: status {rc}")
+    return sl.value, ic.value, ang.value
 
 EXPORTED = tuple(_PROTOS)
 
@@ -242,6 +257,14 @@ class Device:
                                                  None if hi is None else C.byref(hi), C.byref(out)))
         return dict(count=out.count, max_pid=out.max_pid, mean_speed=out.mean_speed, max_speed=out.max_speed,
                     box_count=out.box_count, box_solid_fraction=out.box_solid_fraction)
+
+    def surface_profile(self, lo, hi, bins, axis=0):
+        """Highest particle top y + r per bin of [lo, hi) along x (axis 0) or
+        z (axis 2); NaN for empty bins."""
+        top = np.zeros(int(bins), np.float32)
+        self._check(self.lib.bdem_get_surface_profile(self.h, int(axis), float(lo), float(hi), int(bins),
+                                                      _ptr(top, C.c_float)))
+        return top
 
     def status(self):
         w = C.c_uint(0)

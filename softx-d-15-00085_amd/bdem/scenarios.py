@@ -261,6 +261,22 @@ def box_deck(law="ref", seed=1, nx=32, ny=16, nz=16, L=50.0, radius=0.5, density
     return _deck(sim, [box_walls(L)], pobj, pos, vel, ext=ext, name=f"box{n}-{law}")
 
 
+def collapse_deck(nx=6, ny=14, nz=19, L=20.0, seed=4, radius=0.5, density=2500.0, kn=1e5, e=0.3, mu=0.6,
+                  roll_res=0.001, dt=1e-5, spacing=1.05, jitter=0.02):
+    """Column collapse: a jittered lattice of spheres against the x = 0 wall
+    of a closed cube [0, L]^3, released at rest.  It slumps into a ramp that
+    falls away from the wall; the ramp's slope is the angle of repose."""
+    n = nx * ny * nz
+    gi = np.indices((nx, ny, nz)).reshape(3, -1).T.astype(np.float64)
+    pos = np.array([radius + 0.05] * 3) + gi * spacing
+    pos += uniform(seed, STREAM_JITTER, 3 * n, -jitter, jitter).reshape(n, 3)
+    vel = np.zeros((n, 3))
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (L, L, L), 2.0 * radius, rotation=True,
+                   roll_res=roll_res)
+    pobj = [sphere_pobj(radius, density, e, kn, mu)]
+    return _deck(sim, [box_walls(L)], pobj, pos, vel, name=f"collapse{n}")
+
+
 def random_deck(n, seed=7, L=12.0, law="ref", radius=0.5, polydisperse=False, rmin=0.2, rmax=0.4,
                 density=2500.0, kn=1e5, e=0.6, mu=0.4, dt=1e-5, vmax=50.0, wmax=20.0,
                 cell=None, origin=-1.0, aliased=False):

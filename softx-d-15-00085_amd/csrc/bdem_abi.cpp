@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 #include "bdem_abi.h"
@@ -37,6 +38,7 @@ struct bdem_ctx {
   float tallys[6] = {0, 0, 0, 0, 0, 0};
   float* d_tally = nullptr;  /* device tallies {particle, lifter, surface} the step kernel adds to */
   StatsWords* d_stats = nullptr;   /* bdem_particle_stats reduction words */
+  uint32_t* d_profile = nullptr;   /* bdem_get_surface_profile bins */
   /* lifters turned on the device before every step (Rotate_VolObject_Inc) */
   bool lift_on = false;
   double lift_c = 1.0, lift_s = 0.0;
@@ -588,6 +590,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(c->d_tally, 8);
   ALLOC(c->d_stats, 1);
+  ALLOC(c->d_profile, PROFILE_MAX_BINS);
   ALLOC(b.wall_force, (size_t)n * 3);
   if (c->poly) {
     PolyBuffers& pb = c->pb;
@@ -857,6 +860,57 @@ int bdem_get_particle_stats(bdem_ctx* c, float min_y, const bdem_float3* box_lo,
   out->box_count = (int)w.box_count;
   const double bv = box_lo ? (double(hi.x) - lo.x) * (double(hi.y) - lo.y) * (double(hi.z) - lo.z) : 0.0;
   out->box_solid_fraction = c->poly ? -1.0 : (bv > 0.0 ? w.box_volume / bv : 0.0);
+  return BDEM_OK;
+}
+
+int bdem_get_surface_profile(bdem_ctx* c, int axis, float lo, float hi, int bins, float* top) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!top) return fail(c, BDEM_EINVAL, "null argument");
+  if (axis != 0 && axis != 2) return fail(c, BDEM_EINVAL, "profile axis must be 0 (x) or 2 (z)");
+  if (bins < 1 || bins > PROFILE_MAX_BINS) return fail(c, BDEM_EINVAL, "bins must be 1..4096");
+  if (!(hi > lo)) return fail(c, BDEM_EINVAL, "profile range must have hi > lo");
+  HIPCHK(c, hipSetDevice(c->device));
+  const float inv_w = (float)bins / (hi - lo);
+  HIPCHK(c, hipMemsetAsync(c->d_profile, 0, sizeof(uint32_t) * (size_t)bins, c->stream));
+  launch_profile(c->b, axis, lo, inv_w, bins, c->d_profile, c->stream);
+  HIPCHK(c, hipGetLastError());
+  std::vector<uint32_t> keys((size_t)bins);
+  HIPCHK(c, hipMemcpyAsync(keys.data(), c->d_profile, sizeof(uint32_t) * (size_t)bins, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < bins; ++k) {
+    if (keys[k] == 0u) {
+      top[k] = std::numeric_limits<float>::quiet_NaN();
+      continue;
+    }
+    const uint32_t u = (keys[k] & 0x80000000u) ? (keys[k] & 0x7FFFFFFFu) : ~keys[k];
+    std::memcpy(&top[k], &u, 4);
+  }
+  return BDEM_OK;
+}
+
+int bdem_surface_angle(const float* top, int bins, float lo, float hi, double* slope, double* intercept,
+                       double* angle_deg) {
+  if (!top || bins < 2 || !(hi > lo)) return BDEM_EINVAL;
+  /* least squares h = a + b u over the non-empty bins, u at the bin centres */
+  const double w = ((double)hi - (double)lo) / bins;
+  double su = 0.0, sh = 0.0, m = 0.0;
+  for (int k = 0; k < bins; ++k)
+    if (top[k] == top[k]) { su += lo + (k + 0.5) * w; sh += top[k]; m += 1.0; }
+  if (m < 2.0) return BDEM_EINVAL;
+  const double ub = su / m, hb = sh / m;
+  double suu = 0.0, suh = 0.0;
+  for (int k = 0; k < bins; ++k)
+    if (top[k] == top[k]) {
+      const double du = lo + (k + 0.5) * w - ub;
+      suu += du * du;
+      suh += du * ((double)top[k] - hb);
+    }
+  const double b = suh / suu;
+  if (slope) *slope = b;
+  if (intercept) *intercept = hb - b * ub;
+  if (angle_deg) *angle_deg = std::atan(std::fabs(b)) * (180.0 / 3.14159265358979323846);
   return BDEM_OK;
 }
 

@@ -163,6 +163,12 @@ struct StatsWords {
 };
 void launch_stats(const StepBuffers& b, float min_y, float3 lo, float3 hi, bool want_volume, StatsWords* out,
                   hipStream_t s);
+/* Free-surface profile (bdem_stats.hip): per bin of width 1/inv_w along
+ * axis 0 (x) or 2 (z) from lo, the largest y + r over the live particles, as
+ * an order-preserving uint32 key of the float (0: empty).  `keys` (bins
+ * words, bins <= PROFILE_MAX_BINS) must be zeroed first. */
+constexpr int PROFILE_MAX_BINS = 4096;
+void launch_profile(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* keys, hipStream_t s);
 
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,

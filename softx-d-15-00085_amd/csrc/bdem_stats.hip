@@ -81,4 +81,38 @@ void launch_stats(const StepBuffers& b, float min_y, float3 lo, float3 hi, bool 
                      want_volume ? 1 : 0, out);
 }
 
+/* Free-surface profile: over bins of width 1/inv_w along `axis` (0 x, 2 z)
+ * from lo, the highest particle top y + r of each bin (r: the bounding
+ * radius for polyhedra), as an order-preserving key of the float (empty: 0).
+ * Each block keeps its maxima in LDS and merges the bins it touched with
+ * one global atomicMax each; the particles arrive in cell order, so a block
+ * touches few bins. */
+__device__ __forceinline__ uint32_t order_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void k_profile(int n, const float4* __restrict__ pos_r, const float4* __restrict__ omg_t,
+                                                 int axis, float lo, float inv_w, int bins, uint32_t* __restrict__ keys) {
+  __shared__ uint32_t sh[PROFILE_MAX_BINS];
+  for (int k = threadIdx.x; k < bins; k += blockDim.x) sh[k] = 0u;
+  __syncthreads();
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 p = pos_r[i];
+    if (packed_pid(omg_t[i].w) < 0) continue;
+    const float t = ((axis == 0 ? p.x : p.z) - lo) * inv_w;
+    if (!(t >= 0.0f && t < (float)bins)) continue;
+    const int k = min((int)t, bins - 1);
+    atomicMax(&sh[k], order_key(p.y + p.w));
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < bins; k += blockDim.x)
+    if (sh[k]) atomicMax(&keys[k], sh[k]);
+}
+
+void launch_profile(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* keys, hipStream_t s) {
+  const int blocks = b.n <= 0 ? 1 : min((b.n + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_profile, dim3(blocks), dim3(256), 0, s, b.n, b.pos_cur, b.omg_cur, axis, lo, inv_w, bins, keys);
+}
+
 }  // namespace bdem

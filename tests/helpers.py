@@ -42,3 +42,21 @@ def oracle_positions_out(deck, state):
         return pos.astype(np.float64)
     com = np.array([[p.COM.x, p.COM.y, p.COM.z] for p in deck["pobj"]], np.float32)
     return (pos - com[state["ptype"]]).astype(np.float64)
+
+
+def surface_profile_np(pos, radius, lo, hi, bins, axis=0, live=None):
+    """The free-surface profile of bdem_get_surface_profile restated in numpy
+    float32: bin (int)((u - lo) * inv_w), inv_w = (float)bins / (hi - lo),
+    highest y + r per bin, NaN where empty."""
+    f = np.float32
+    pos = np.asarray(pos, f)
+    r = np.broadcast_to(np.asarray(radius, f), (len(pos),))
+    inv_w = f(f(bins) / (f(hi) - f(lo)))
+    t = (pos[:, axis] - f(lo)) * inv_w
+    ok = (t >= 0) & (t < f(bins))
+    if live is not None:
+        ok &= live
+    out = np.full(bins, -np.inf, f)
+    np.maximum.at(out, t[ok].astype(np.int64), pos[ok, 1] + r[ok])
+    out[np.isneginf(out)] = np.nan
+    return out
