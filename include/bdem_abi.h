@@ -150,6 +150,16 @@ typedef struct bdem_particle_stats {
 } bdem_particle_stats;
 int bdem_get_particle_stats(bdem_ctx* ctx, float min_y, const bdem_float3* box_lo, const bdem_float3* box_hi,
                             bdem_particle_stats* out);
+/* Point cloud of a polyhedra context (synchronous): for each particle in
+ * the library's order, its vertices in the world frame, the local vertices
+ * turned by the orientation plus the centre of mass (as RotatePolyhedra,
+ * Device/Utilities.cuh:420-479, makes them for the contact search), in
+ * verts[i * max_vertex + v]; slots past the type's count are zero.
+ * num_vertex and pid (either may be NULL) receive each particle's count and
+ * P_ID.  The reference writes only the unturned vertices (SpecType 5,
+ * Input/FileIO.h:253-263).  BDEM_EINVAL for a sphere context or a
+ * max_vertex below the largest vertex count. */
+int bdem_get_poly_vertices(bdem_ctx* ctx, int max_vertex, bdem_float3* verts, int* num_vertex, int* pid);
 /* Free-surface profile of the current state (synchronous), for the angle
  * of repose; no reference counterpart (the reference computes no angle).
  * [lo, hi) along axis 0 (x) or 2 (z) is cut into `bins` (1..4096) equal

@@ -56,6 +56,7 @@ _PROTOS = {
     "bdem_get_particle_stats": (C.c_int, [_P, C.c_float, _f3p, _f3p, C.POINTER(T.ParticleStats)]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
     "bdem_get_surface_profile": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
+    "bdem_get_poly_vertices": (C.c_int, [_P, C.c_int, _f3p, _ip, _ip]),
     "bdem_surface_angle": (C.c_int, [_fp, C.c_int, C.c_float, C.c_float, _dp, _dp, _dp]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
     "bdem_calibration_copy": (C.c_int, [_P, C.c_ulonglong, C.c_int]),
@@ -257,6 +258,19 @@ class Device:
                                                  None if hi is None else C.byref(hi), C.byref(out)))
         return dict(count=out.count, max_pid=out.max_pid, mean_speed=out.mean_speed, max_speed=out.max_speed,
                     box_count=out.box_count, box_solid_fraction=out.box_solid_fraction)
+
+    def poly_vertices(self, max_vertex=None):
+        """World-frame vertices of every polyhedron: (verts [n, max_vertex, 3],
+        num_vertex [n], pid [n]) in the library's order."""
+        n = self.n
+        if max_vertex is None:
+            max_vertex = max(p.num_vertex for p in self._pobj)
+        verts = np.zeros((n, int(max_vertex), 3), np.float32)
+        nv = np.zeros(n, np.int32)
+        pid = np.zeros(n, np.int32)
+        self._check(self.lib.bdem_get_poly_vertices(self.h, int(max_vertex), _ptr(verts, T.Float3),
+                                                    _ptr(nv, C.c_int), _ptr(pid, C.c_int)))
+        return verts, nv, pid
 
     def surface_profile(self, lo, hi, bins, axis=0):
         """Highest particle top y + r per bin of [lo, hi) along x (axis 0) or

@@ -863,6 +863,35 @@ int bdem_get_particle_stats(bdem_ctx* c, float min_y, const bdem_float3* box_lo,
   return BDEM_OK;
 }
 
+int bdem_get_poly_vertices(bdem_ctx* c, int max_vertex, bdem_float3* verts, int* num_vertex, int* pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!verts) return fail(c, BDEM_EINVAL, "null argument");
+  if (!c->poly) return fail(c, BDEM_EINVAL, "not a polyhedra context (particle_type 1)");
+  const PolyLayout L = c->pb.L;
+  if (max_vertex < L.pv) return fail(c, BDEM_EINVAL, "max_vertex is below the largest vertex count of the types");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int n = c->b.n;
+  launch_poly_world_shapes(c->dp, c->dpobj, c->b, c->pb, c->stream);
+  HIPCHK(c, hipGetLastError());
+  std::vector<float4> R((size_t)n * L.stride), O((size_t)n);
+  HIPCHK(c, hipMemcpyAsync(R.data(), c->pb.rot, R.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(O.data(), c->b.omg_cur, O.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; ++i) {
+    const int packed = f_bits(O[i].w);
+    const int t = packed & 7;
+    const int nv = t < (int)c->pobj.size() ? c->pobj[t].num_vertex : 0;
+    if (num_vertex) num_vertex[i] = nv;
+    if (pid) pid[i] = packed >> 3;
+    for (int v = 0; v < max_vertex; ++v) {
+      const float4 r = v < nv ? R[(size_t)i * L.stride + v] : make_float4(0.f, 0.f, 0.f, 0.f);
+      verts[(size_t)i * max_vertex + v] = bdem_float3{r.x, r.y, r.z};
+    }
+  }
+  return BDEM_OK;
+}
+
 int bdem_get_surface_profile(bdem_ctx* c, int axis, float lo, float hi, int bins, float* top) {
   int rc = require(c);
   if (rc) return rc;

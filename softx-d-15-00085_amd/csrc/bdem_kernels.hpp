@@ -149,6 +149,9 @@ void launch_skin(const DevParams* dP, const StepBuffers& b, const uint32_t* old_
 void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sorted_keys, int append_par, bool do_kill, bool want_wall_force,
                       bool lifters, hipStream_t s);
+/* rot <- world-frame vertices, normals and centroids of the current state */
+void launch_poly_world_shapes(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+                              const PolyBuffers& pb, hipStream_t s);
 void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s);
 

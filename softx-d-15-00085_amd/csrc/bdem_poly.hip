@@ -1132,6 +1132,16 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_POBJ* dpob
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.qcount, pb.equeue, pb.pf, pb.pt);
 }
 
+/* World-frame shapes of the current (unsorted) state into pb.rot: the
+ * point-cloud output, between steps (the next step rewrites pb.rot). */
+void launch_poly_world_shapes(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+                              const PolyBuffers& pb, hipStream_t s) {
+  const size_t rot_threads = (size_t)b.n * pb.L.stride;
+  if (rot_threads == 0) return;
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dpobj,
+                     b.n, pb.L, b.pos_cur, b.omg_cur, b.quat_cur, pb.rot, pb.qcount);
+}
+
 void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sk, int ap, bool do_kill, bool want_wf, bool lifters, hipStream_t s) {
   launch_poly_forces_common(dP, dpobj, b, pb, s);

@@ -430,3 +430,31 @@ def test_random_grid_fill_alternates_types(tmp_path):
     _, types, ids = _libc_random_grid(n, 13, 30, 5, 2, [1425, 525], False)
     assert np.array_equal(st["ptype"], types) and np.array_equal(st["pid"], ids)
     assert np.bincount(st["ptype"]).tolist() == [1425, 525]
+
+
+def test_spec5_rows_are_position_plus_file_vertices(tmp_path):
+    """SpecType 5 (Output_SystemState, Input/FileIO.h:253-263): P_ID, type,
+    position, then every vertex of the type as vertex + position, the file
+    frame's vertices without the orientation, as the reference writes them."""
+    rng = np.random.default_rng(5)
+    pobj = [S.poly_pobj(S.CUBE_VERTS * 1.5, S.CUBE_FACES, 2500, 0.5, 1e5, 0.4),
+            S.poly_pobj(*S.hull_shape(3), 2500, 0.5, 1e5, 0.4)]
+    n = 40
+    pos = rng.uniform(-20, 20, (n, 3)).astype(np.float32)
+    ptype = rng.integers(0, 2, n).astype(np.uint32)
+    pid = np.arange(n, dtype=np.int32)
+    pid[3] = -1
+    h = T.StateHeader()
+    h.spec_type, h.time = 5, 0.5
+    path = str(tmp_path / "s5")
+    write_state(path, h, 0, 0, 1, pos, ptype, pid, quat=rng.normal(0, 1, (n, 4)).astype(np.float32), pobj=pobj)
+    rows = [ln.split() for ln in open(path).read().splitlines()[3:] if ln.strip()]
+    assert len(rows) == n - 1
+    for row, i in zip(rows, [k for k in range(n) if pid[k] != -1]):
+        assert int(row[0]) == pid[i] and int(row[1]) == ptype[i]
+        assert np.allclose([float(x) for x in row[2:5]], pos[i], rtol=1e-5, atol=1e-5)
+        p = pobj[ptype[i]]
+        want = np.array([[p.vertex[k].x, p.vertex[k].y, p.vertex[k].z] for k in range(p.num_vertex)],
+                        np.float32) + pos[i]
+        got = np.array([float(x) for x in row[5:]]).reshape(-1, 3)
+        assert got.shape == want.shape and np.allclose(got, want, rtol=1e-5, atol=1e-5)

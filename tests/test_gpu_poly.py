@@ -104,3 +104,44 @@ def test_poly_closed_form_pairs(name, qa, dx, qb, big):
     assert float(np.abs(o[0]).max()) > 500.0   # Kn * 0.01 = 1000 on each cube
     dev.close()
     ora.close()
+
+
+def _rot_np(q, v):
+    """Turn v by the unit quaternion q = (w, x, y, z): the rotation matrix of
+    q, as the CPU closed-form tests use it."""
+    w, x, y, z = (np.float64(c) for c in q)
+    R = np.array([[w * w + x * x - y * y - z * z, 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), w * w - x * x + y * y - z * z, 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), w * w - x * x - y * y + z * z]])
+    return v @ R.T
+
+
+def test_point_cloud_is_the_turned_shapes():
+    """bdem_get_poly_vertices: every vertex is the type's local vertex turned
+    by the particle's orientation plus its centre of mass."""
+    print("[case] poly point cloud, mixed deck 20 steps", flush=True)
+    deck = DECKS["mixed"]()
+    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"])
+    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"], quat=deck["quat"])
+    dev.step(20)
+    st = dev.get_state()
+    verts, nv, pid = dev.poly_vertices()
+    assert np.array_equal(pid, st["pid"])
+    pobj = deck["pobj"]
+    worst = 0.0
+    for i in range(len(pid)):
+        p = pobj[st["ptype"][i]]
+        assert nv[i] == p.num_vertex
+        com = np.array([p.COM.x, p.COM.y, p.COM.z])
+        centre = st["pos"][i].astype(np.float64) + com          # the library returns position - COM
+        local = np.array([[p.Local_Vertex[k].x, p.Local_Vertex[k].y, p.Local_Vertex[k].z]
+                          for k in range(p.num_vertex)])
+        q = st["quat"][i].astype(np.float64)
+        want = _rot_np(q / np.linalg.norm(q), local) + centre
+        worst = max(worst, float(np.abs(verts[i, :nv[i]] - want).max()))
+        assert np.all(verts[i, nv[i]:] == 0.0)
+    print(f"point cloud: max |device - numpy| {worst:.3e} cm")
+    assert worst < 2e-5 * max(16.0, 1.0)     # coordinates up to the box size, 16 cm
+    with pytest.raises(bdem.lib.BdemError):
+        dev.poly_vertices(max_vertex=int(nv.max()) - 1)
+    dev.close()
