@@ -165,8 +165,10 @@ __device__ __forceinline__ int particle_pair_forces(const DevParams& P, const ui
  * order -- the same additions in the same order as the per-particle loop,
  * with the wave's force body running ceil(T/64) times instead of
  * max(contacts) times.  A map from contact number to (owner lane, index)
- * replaces a search per contact.  All lanes of the wave must call it
- * (count 0 for idle lanes). */
+ * replaces a search per contact.  The owner's own state (position, velocity,
+ * spin, spring count) comes from the owner lane's registers by wave shuffle,
+ * not by a second gather.  All lanes of the wave must call it (count 0 for
+ * idle lanes). */
 struct WaveScratch {
   float f[6][64];
   uint16_t own[64 * CLIST];   /* contact q -> owner lane | index << 6 */
@@ -179,7 +181,7 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
                                                  const float4* __restrict__ vel_m,
                                                  const float4* __restrict__ omg_t,
                                                  const uint32_t* clist, int nl, int base_i,
-                                                 const uint32_t* __restrict__ hwc_in,
+                                                 float4 own_p, float4 own_v, float4 own_o, uint32_t own_hwc,
                                                  const HistRec* __restrict__ hr_in, HistRec* hr_out,
                                                  int hist_cap, WaveScratch& ws, f3& F, f3& T,
                                                  float* tally = nullptr) {
@@ -198,20 +200,28 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
   for (int base = 0; base < total; base += 64) {
     const int p = base + lane;
     f3 Fc = mk3(0.f, 0.f, 0.f), Tc = Fc;
+    /* every lane shuffles (the source lane must be active); lanes past the
+     * last contact read their own registers and discard them */
+    const uint32_t e = p < total ? (uint32_t)ws.own[p] : (uint32_t)lane;
+    const int src = (int)(e & 63u);
+    const float4 po = make_float4(__shfl(own_p.x, src, 64), __shfl(own_p.y, src, 64), __shfl(own_p.z, src, 64),
+                                  __shfl(own_p.w, src, 64));
+    const float4 vo = make_float4(__shfl(own_v.x, src, 64), __shfl(own_v.y, src, 64), __shfl(own_v.z, src, 64),
+                                  __shfl(own_v.w, src, 64));
+    const float4 oo = make_float4(__shfl(own_o.x, src, 64), __shfl(own_o.y, src, 64), __shfl(own_o.z, src, 64),
+                                  __shfl(own_o.w, src, 64));
+    const uint32_t ohwc = (uint32_t)__shfl((int)own_hwc, src, 64);
     if (p < total) {
-      const uint32_t e = ws.own[p];
       const int k = (int)(e >> 6);
-      const int otid = wbase_tid + (int)(e & 63u);
-      const int oi = base_i + otid;
+      const int otid = wbase_tid + src;
       const int j = (int)clist[k * STEP_THREADS + otid];
-      const float4 po = pos_r[oi], vo = vel_m[oi], oo = omg_t[oi];
       const int opid = packed_pid(oo.w);
       const HistRec* hin = nullptr;
       int hin_cnt = 0;
       HistRec* rec = nullptr;
       if (LAW != BDEM_LAW_REFERENCE && opid >= 0 && opid < hist_cap) {
         hin = hr_in + (size_t)opid * HMAX;
-        hin_cnt = min((int)(hwc_in[oi] & 0xFFFFu), HMAX);
+        hin_cnt = min((int)(ohwc & 0xFFFFu), HMAX);
         if (k < HMAX) rec = hr_out + (size_t)opid * HMAX + k;
       }
       contact_force<LAW>(P, j, pos_r, vel_m, omg_t, xyz(po), po.w, xyz(vo), vo.w, xyz(oo),
@@ -302,7 +312,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   f3 Fpp = alive ? cvt(P.gravity) * m : mk3(0.f, 0.f, 0.f);
   f3 Tpp = mk3(0.f, 0.f, 0.f);
   const int nl = nc < CLIST ? nc : CLIST;
-  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, base_i, hwc_in, hr_in, hr_out, hist_cap,
+  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, base_i, pr, vm, ot, hwc, hr_in, hr_out, hist_cap,
                         *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp, tally);
 
   bool chg = false;
