@@ -813,6 +813,27 @@ __device__ __forceinline__ f3 find_spring(const HistRec* __restrict__ recs, int 
   return mk3(0.f, 0.f, 0.f);
 }
 
+/* find_spring with the first probe, recs[start(cursor)], already loaded by
+ * the caller (issued beside the partner's state loads); the same probes in
+ * the same order otherwise. */
+__device__ __forceinline__ int spring_start(int cnt, int cursor) {
+  return (cursor >= 0 && cursor < cnt) ? cursor : 0;
+}
+__device__ __forceinline__ f3 find_spring_first(const HistRec* __restrict__ recs, int cnt, int id, int& cursor,
+                                                HistRec first) {
+  const int start = spring_start(cnt, cursor);
+  for (int k = 0; k < cnt; ++k) {
+    int s = start + k;
+    if (s >= cnt) s -= cnt;
+    const HistRec r = k == 0 ? first : recs[s];
+    if (__float_as_int(r.w) == id) {
+      cursor = s + 1;
+      return mk3(r.x, r.y, r.z);
+    }
+  }
+  return mk3(0.f, 0.f, 0.f);
+}
+
 /* Wall force and torque on one particle.  The reference law keeps the
  * accumulate/overwrite pattern of DEM_Kernels_Compute.cuh:283-604 (plane hits
  * add; cylinder and cap hits assign the force; torques add).  History laws

@@ -85,6 +85,8 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
   const float4 pb = pos_r[j];
   const float4 vb = vel_m[j];
   const float4 ob = omg_t[j];
+  HistRec first = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (LAW != BDEM_LAW_REFERENCE && hin_cnt > 0) first = hin[spring_start(hin_cnt, cursor)];
   if (LAW == BDEM_LAW_REFERENCE) {
     pair_ref(P, pa, ra, va, ma, wa, ta, xyz(pb), pb.w, xyz(vb), vb.w, xyz(ob), packed_type(ob.w), Fc, Tc,
              tally);
@@ -102,7 +104,7 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
     const float rstar = ((ra * pb.w) * sm) * q;
     const float mstar = ((ma * vb.w) * sr) * q;
     const int idb = packed_pid(ob.w);
-    f3 xi = find_spring(hin, hin_cnt, idb, cursor);
+    f3 xi = find_spring_first(hin, hin_cnt, idb, cursor, first);
     Fc = history_law(P, n, delta, vc, rstar, mstar, P.law_pp[ta][packed_type(ob.w)], xi);
     Tc = cross(la, Fc);
     if (hrec) *hrec = make_float4(xi.x, xi.y, xi.z, __int_as_float(idb));
