@@ -179,6 +179,7 @@ void build_params(bdem_ctx* c) {
   P.kill_height = s.Silo_Kill_Height;
   P.max_size = s.max_size;
   P.kill_enabled = s.Kill_Particles ? 1 : 0;
+  P.one_type = c->pobj.size() == 1 ? 1 : 0;   /* uploads reject types >= num_pobj */
   for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
     const bdem_POBJ& po = c->pobj[t];
     P.t_radius[t] = po.radius;

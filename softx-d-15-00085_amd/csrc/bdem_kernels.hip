@@ -105,7 +105,10 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
     const float mstar = ((ma * vb.w) * sr) * q;
     const int idb = packed_pid(ob.w);
     f3 xi = find_spring_first(hin, hin_cnt, idb, cursor, first);
-    Fc = history_law(P, n, delta, vc, rstar, mstar, P.law_pp[ta][packed_type(ob.w)], xi);
+    /* one type: a wave-uniform address, read by scalar loads that need not
+     * wait for the partner's type word */
+    Fc = history_law(P, n, delta, vc, rstar, mstar, P.one_type ? P.law_pp[0][0] : P.law_pp[ta][packed_type(ob.w)],
+                     xi);
     Tc = cross(la, Fc);
     if (hrec) *hrec = make_float4(xi.x, xi.y, xi.z, __int_as_float(idb));
   }
