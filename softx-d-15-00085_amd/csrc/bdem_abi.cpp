@@ -183,6 +183,12 @@ void build_params(bdem_ctx* c) {
   for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
     const bdem_POBJ& po = c->pobj[t];
     P.t_radius[t] = po.radius;
+    double vmax = 0.0;
+    for (int v = 0; v < po.num_vertex && v < 32; ++v) {
+      const double x = po.Local_Vertex[v].x, y = po.Local_Vertex[v].y, z = po.Local_Vertex[v].z;
+      vmax = std::fmax(vmax, std::sqrt(x * x + y * y + z * z));
+    }
+    P.t_vbound[t] = float(vmax * 1.0001);
     P.t_mass[t] = po.mass;
     P.t_inertia[t] = po.InertiaT[0];
     for (size_t u = 0; u < c->pobj.size() && u < size_t(MAX_TYPES); ++u) {

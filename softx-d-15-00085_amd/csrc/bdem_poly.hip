@@ -80,11 +80,25 @@ __device__ __forceinline__ bool point_on_face(const bdem_Surface& f, const float
   return (fabsf(area - f.area) / f.area) * 100.0000f <= tol;
 }
 
-/* is_SP_Poly_Poly (Utilities.cuh:105-156) */
-__device__ __forceinline__ bool sp_poly_poly(const Shape& A, f3 PosB, const Shape& B) {
+/* B's centre farther beyond a face plane of A than B's vertex bound vb (any
+ * turned vertex lies within vb of the centre): then every vertex of B is in
+ * front of that plane, and the vertex test of the face must find them so.
+ * The margin (10 um plus 4e-6 of the coordinates' size) exceeds the float
+ * error of the turned vertices and of the plane distances by a wide factor,
+ * and vb carries 1e-4 for the unit-length tolerance of the quaternion. */
+__device__ __forceinline__ bool beyond_by_sphere(float dist, f3 c, f3 PosB, float vb) {
+  const float size = fabsf(PosB.x) + fabsf(PosB.y) + fabsf(PosB.z) + fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+  return dist > vb + 0.001f + 4.0e-6f * size;
+}
+
+/* is_SP_Poly_Poly (Utilities.cuh:105-156); vb: B's vertex bound (the sphere
+ * shortcut changes which face is found first, never whether one is) */
+__device__ __forceinline__ bool sp_poly_poly(const Shape& A, f3 PosB, const Shape& B, float vb) {
   for (int f = 0; f < A.nf; ++f) {
     const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
-    if (dot(n, PosB - c) > 0.0f) {
+    const float dist = dot(n, PosB - c);
+    if (dist > 0.0f) {
+      if (beyond_by_sphere(dist, c, PosB, vb)) return true;
       bool contact = false;
       for (int v = 0; v < B.nv; ++v) {
         if (dot(n, ld3(B.V + v) - c) <= 0.00000f) { contact = true; break; }
@@ -323,9 +337,11 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const bdem_POBJ& pa,
 
 /* Face f of A separates B: B's centre and every vertex of B strictly in
  * front (the per-face test of is_SP_Poly_Poly). */
-__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B) {
+__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb) {
   const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
-  if (!(dot(n, PosB - c) > 0.0f)) return false;
+  const float dist = dot(n, PosB - c);
+  if (!(dist > 0.0f)) return false;
+  if (beyond_by_sphere(dist, c, PosB, vb)) return true;
   for (int v = 0; v < B.nv; ++v)
     if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
   return true;
@@ -348,12 +364,13 @@ __device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
  * matter, only whether one exists, so each body's face turned most towards
  * the other is tried first: it separates most separated pairs, and the
  * full loops run only when it does not. */
-__device__ __forceinline__ bool pair_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB) {
+__device__ __forceinline__ bool pair_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
+                                               float vbB) {
   const f3 dAB = PosB - PosA;
-  if (face_separates(A, facing_face(A, dAB), PosB, B)) return true;
-  if (face_separates(B, facing_face(B, -1.0f * dAB), PosA, A)) return true;
-  bool sp = sp_poly_poly(A, PosB, B);
-  if (!sp) sp = sp_poly_poly(B, PosA, A);
+  if (face_separates(A, facing_face(A, dAB), PosB, B, vbB)) return true;
+  if (face_separates(B, facing_face(B, -1.0f * dAB), PosA, A, vbA)) return true;
+  bool sp = sp_poly_poly(A, PosB, B, vbB);
+  if (!sp) sp = sp_poly_poly(B, PosA, A, vbA);
   return sp;
 }
 
@@ -961,7 +978,7 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
       const bdem_POBJ& pb = pobj[o.tb];
       const Shape A = shape_of(rot, L, o.a, pa);
       const Shape B = shape_of(rot, L, o.b, pb);
-      need = !pair_separated(A, B, o.PosA, o.PosB);
+      need = !pair_separated(A, B, o.PosA, o.PosB, P.t_vbound[o.ta], P.t_vbound[o.tb]);
       if (!need) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
     }
     const uint64_t mask = __ballot(need);
