@@ -300,4 +300,25 @@ int bo_poly_walls(void* h, int i, double* out) {
   return 0;
 }
 
+/* History-law probe: one evaluation of the normal/tangential law for the
+ * pair constants of types (ta, tb) (wall = 0) or the wall constants of type
+ * ta (wall = 1), with normal n, overlap delta, contact velocity vc,
+ * effective radius and mass, and the spring xi (in: the stored spring; out:
+ * the spring the law keeps).  F receives the force. */
+int bo_history_law(void* h, int wall, int ta, int tb, const double* n, double delta, const double* vc,
+                   double rstar, double mstar, double* xi, double* F) {
+  WITH(h, {
+    if (ta < 0 || ta >= 8 || tb < 0 || tb >= 8 || o.law_kind() == BDEM_LAW_REFERENCE) return -1;
+    using RR = std::remove_reference_t<decltype(o.pos[0].x)>;
+    auto v3 = [](const double* a) { return mk<RR>(RR(a[0]), RR(a[1]), RR(a[2])); };
+    auto x = v3(xi);
+    const auto& lp = wall ? o.wlaw[ta] : o.law[ta][tb];
+    const auto f = o.history_law(v3(n), RR(delta), v3(vc), RR(rstar), RR(mstar), lp, x);
+    xi[0] = double(x.x); xi[1] = double(x.y); xi[2] = double(x.z);
+    F[0] = double(f.x); F[1] = double(f.y); F[2] = double(f.z);
+    return 0;
+  });
+  return 0;
+}
+
 }  // extern "C"

@@ -58,6 +58,8 @@ def load():
     lib.bo_run.argtypes = [P, C.c_int, C.c_double, C.c_float, C.c_float, C.c_float]
     lib.bo_energy.argtypes = [P, fp, dp]
     lib.bo_set_tallys.argtypes = [P, fp]
+    lib.bo_history_law.argtypes = [P, C.c_int, C.c_int, C.c_int, dp, C.c_double, dp, C.c_double, C.c_double,
+                                   dp, dp]
     _lib = lib
     return lib
 
@@ -240,3 +242,17 @@ class Oracle:
         if self.lib.bo_poly_walls(self.h, int(i), _p(out, C.c_double)) != 0:
             raise ValueError("poly_walls needs a polyhedra deck")
         return out[:3], out[3:]
+
+    def history_law(self, n, delta, vc, rstar, mstar, xi=(0.0, 0.0, 0.0), ta=0, tb=0, wall=False):
+        """One evaluation of the history law (pair constants of types ta, tb,
+        or the wall constants of ta): returns (force, spring kept)."""
+        nn = np.ascontiguousarray(n, np.float64)
+        v = np.ascontiguousarray(vc, np.float64)
+        x = np.array(xi, np.float64)
+        F = np.zeros(3)
+        rc = self.lib.bo_history_law(self.h, 1 if wall else 0, int(ta), int(tb), _p(nn, C.c_double), float(delta),
+                                     _p(v, C.c_double), float(rstar), float(mstar), _p(x, C.c_double),
+                                     _p(F, C.c_double))
+        if rc != 0:
+            raise ValueError("history_law needs a history-law deck and types in [0, 8)")
+        return F, x

@@ -168,6 +168,15 @@ def _f3(a):
     return a
 
 
+def _shaped(a, dtype, shape, name):
+    """a as a contiguous array of dtype and exactly this shape; the C side
+    reads Num_Particles entries, so a short buffer must not reach it."""
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.shape != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
 class Device:
     """One simulation context on one GPU (KDevice, Interface/KDevice.h:17-99)."""
 
@@ -213,10 +222,15 @@ class Device:
         self._check(self.lib.bdem_update_simulation_info(self.h, C.byref(sim)))
 
     def set_state(self, pos, ptype, pid, vel=None, avel=None, quat=None):
-        pos = _f3(pos)
-        ptype = np.ascontiguousarray(ptype, dtype=np.uint32)
-        pid = np.ascontiguousarray(pid, dtype=np.int32)
-        q = None if quat is None else np.ascontiguousarray(quat, dtype=np.float32)
+        n = self.n
+        pos = _shaped(pos, np.float32, (n, 3), "pos")
+        ptype = _shaped(ptype, np.uint32, (n,), "ptype")
+        pid = _shaped(pid, np.int32, (n,), "pid")
+        q = None if quat is None else _shaped(quat, np.float32, (n, 4), "quat")
+        if vel is not None:
+            vel = _shaped(vel, np.float32, (n, 3), "vel")
+        if avel is not None:
+            avel = _shaped(avel, np.float32, (n, 3), "avel")
         if vel is None:
             self._check(self.lib.bdem_set_pos(self.h, _ptr(pos, T.Float3), _ptr(q, T.Quaterion),
                                               _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
@@ -228,8 +242,8 @@ class Device:
                 _ptr(avel, T.Float3), _ptr(ptype, C.c_uint), _ptr(pid, C.c_int)))
 
     def set_radii(self, radius, mass):
-        r = np.ascontiguousarray(radius, dtype=np.float32)
-        m = np.ascontiguousarray(mass, dtype=np.float32)
+        r = _shaped(radius, np.float32, (self.n,), "radius")
+        m = _shaped(mass, np.float32, (self.n,), "mass")
         self._check(self.lib.bdem_set_radii(self.h, _ptr(r, C.c_float), _ptr(m, C.c_float)))
 
     # -- state out --------------------------------------------------------
@@ -302,7 +316,7 @@ class Device:
 
     def set_tallys(self, tallys):
         """Device_Set_Tallys: the three dissipation tallies (unscaled)."""
-        t = np.ascontiguousarray(tallys, dtype=np.float32)
+        t = _shaped(tallys, np.float32, (3,), "tallys")
         self._check(self.lib.bdem_set_tallys(self.h, _ptr(t, C.c_float)))
 
     def energy(self):

@@ -284,6 +284,30 @@ int check_dobj(bdem_ctx* c, const bdem_DOBJ* d, int nd) {
   return BDEM_OK;
 }
 
+/* WOBJ tables (KSimulationObjects.h:104-152): surface kinds and the index
+ * ranges the wall kernels read (is_PointOnSurface walks vertex_Order).
+ * Called by every entry point that uploads a world-object table. */
+int check_wobj(bdem_ctx* c, const bdem_WOBJ* wobj, int num_wobj) {
+  for (int k = 0; k < num_wobj; ++k) {
+    const bdem_WOBJ& w = wobj[k];
+    if (w.surface_type != BDEM_SURFACE_PLANE && w.surface_type != BDEM_SURFACE_CYLINDER)
+      return fail(c, BDEM_EINVAL, "unknown world-object surface type");
+    if (w.surface_type != BDEM_SURFACE_PLANE) continue;
+    if (w.num_surfaces < 0 || w.num_surfaces > 16 || w.num_vertex < 0 || w.num_vertex > 16)
+      return fail(c, BDEM_EINVAL, "world object surface/vertex count out of range");
+    for (int s = 0; s < w.num_surfaces; ++s) {
+      const bdem_Surface& S = w.surfaces[s];
+      if (S.area == -1.0f) continue;  /* infinite plane: no polygon */
+      if (S.num_vertex < 1 || S.num_vertex > 8)
+        return fail(c, BDEM_EINVAL, "surface vertex count out of range");
+      for (int q = 0; q < S.num_vertex; ++q)
+        if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= w.num_vertex)
+          return fail(c, BDEM_EINVAL, "surface vertex index out of range");
+    }
+  }
+  return BDEM_OK;
+}
+
 /* POBJ polyhedron tables (KSimulationObjects.h:272-300): the index ranges
  * the polyhedra kernels read. */
 int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
@@ -297,6 +321,13 @@ int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
     for (int f = 0; f < P.num_faces; ++f) {
       const bdem_Surface& S = P.face[f];
       if (S.num_vertex < 1 || S.num_vertex > 8) return fail(c, BDEM_EINVAL, "polyhedron face vertex count out of range");
+      /* unit normals: the separating-plane test reads face-plane distances as
+       * lengths (its vertex-bound shortcut is exact only then) */
+      const double nn = std::sqrt(double(S.normal.x) * S.normal.x + double(S.normal.y) * S.normal.y +
+                                  double(S.normal.z) * S.normal.z);
+      if (!(std::fabs(nn - 1.0) <= 1e-5))
+        return fail(c, BDEM_EINVAL, "polyhedron face normal is not unit length (type " + std::to_string(t) +
+                                        ", face " + std::to_string(f) + ")");
       for (int q = 0; q < S.num_vertex; ++q)
         if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= P.num_vertex)
           return fail(c, BDEM_EINVAL, "polyhedron face vertex index out of range");
@@ -361,6 +392,28 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
                  const bdem_float3* vel, const bdem_float3* avel, const unsigned int* ptype,
                  const int* pid, bool keep_vel) {
   const int n = c->b.n;
+  /* every check before the first copy: a rejected upload leaves the state,
+   * keys and flags of the previous one untouched */
+  const bool hist = need_history(c);
+  const bool check_q = c->orient && ornt && !c->sim.sphere_orient;
+  for (int i = 0; i < n; ++i) {
+    const unsigned t = ptype[i];
+    if (t >= c->pobj.size() || t >= (unsigned)MAX_TYPES)
+      return fail(c, BDEM_EINVAL, "particle type out of range at index " + std::to_string(i));
+    /* P_ID * 8 + type is packed into one int; -1 marks a dead particle */
+    if (pid[i] < -1 || pid[i] >= (1 << 28))
+      return fail(c, BDEM_EINVAL, "P_ID out of range [-1, 2^28) at index " + std::to_string(i));
+    if (hist && pid[i] >= c->b.hist_cap)
+      return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
+    if (check_q && pid[i] >= 0) {
+      /* q v q* scales by |q|^2; the separating-plane shortcut and the
+       * contact depths assume a rotation */
+      const double s2 = double(ornt[i].w) * ornt[i].w + double(ornt[i].x) * ornt[i].x +
+                        double(ornt[i].y) * ornt[i].y + double(ornt[i].z) * ornt[i].z;
+      if (!(std::fabs(s2 - 1.0) <= 1e-4))
+        return fail(c, BDEM_EINVAL, "orientation quaternion is not unit length at index " + std::to_string(i));
+    }
+  }
   std::vector<float4> P(n), V(n), O(n), Q;
   std::vector<float4> oldP, oldV, oldO;
   if (keep_vel || c->hp.per_particle) {
@@ -371,8 +424,6 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
   }
   for (int i = 0; i < n; ++i) {
     const unsigned t = ptype[i];
-    if (t >= c->pobj.size() || t >= (unsigned)MAX_TYPES)
-      return fail(c, BDEM_EINVAL, "particle type out of range at index " + std::to_string(i));
     const float r = c->hp.per_particle ? oldP[i].w : c->pobj[t].radius;
     const float m = c->hp.per_particle ? oldV[i].w : c->pobj[t].mass;
     P[i] = make_float4(pos[i].x, pos[i].y, pos[i].z, r);
@@ -381,6 +432,11 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     if (avel) O[i] = make_float4(avel[i].x, avel[i].y, avel[i].z, bits_f(pack_id_type(pid[i], t)));
     else O[i] = make_float4(oldO[i].x, oldO[i].y, oldO[i].z, bits_f(pack_id_type(pid[i], t)));
   }
+  /* from here on the device state changes: whatever happens, the next step
+   * re-hashes and fully re-sorts */
+  c->keys_valid = false;
+  c->flags_valid = false;
+  c->lists_full = true;
   HIPCHK(c, hipMemcpy(c->b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->b.omg_cur, O.data(), n * sizeof(float4), hipMemcpyHostToDevice));
@@ -391,11 +447,6 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
       else Q[i] = make_float4(1.f, 0.f, 0.f, 0.f);  /* Set_Pos: identity for sphere_orient */
     }
     HIPCHK(c, hipMemcpy(c->b.quat_cur, Q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-  }
-  if (need_history(c)) {
-    for (int i = 0; i < n; ++i)
-      if (pid[i] >= c->b.hist_cap)
-        return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
   }
   if (c->b.hwc_cur) {   /* spring counts and lifter contact types restart */
     launch_fill_u32(c->b.hwc_cur, c->b.n, 0u, c->stream);
@@ -527,35 +578,16 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   if (C + 2 > 0xFFFFFFF0ull) return fail(c, BDEM_EINVAL, "grid too large for 32-bit keys");
   if (num_dobj < 0 || num_dobj > MAX_DOBJ) return fail(c, BDEM_EINVAL, "num_dobj must be in [0, 32]");
   if (num_dobj > 0 && !dobj) return fail(c, BDEM_EINVAL, "null dynamic-object table");
-  if (num_dobj > 0) {
-    int rc0 = check_dobj(c, dobj, num_dobj);
-    if (rc0) return rc0;
-  }
+  int rc0 = BDEM_OK;
+  if (num_dobj > 0 && (rc0 = check_dobj(c, dobj, num_dobj)) != BDEM_OK) return rc0;
   if (sim->particle_type == 1) {
     if (sim->use_hist)
       return fail(c, BDEM_ENOTSUP,
                   "polyhedra contact-type history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) is "
                   "disabled in the reference (main.cpp:830) and undefined as written; not built");
-    int rc0 = check_poly(c, pobj, num_pobj);
-    if (rc0) return rc0;
+    if ((rc0 = check_poly(c, pobj, num_pobj)) != BDEM_OK) return rc0;
   }
-  for (int k = 0; k < num_wobj; ++k) {
-    const bdem_WOBJ& w = wobj[k];
-    if (w.surface_type != BDEM_SURFACE_PLANE && w.surface_type != BDEM_SURFACE_CYLINDER)
-      return fail(c, BDEM_EINVAL, "unknown world-object surface type");
-    if (w.surface_type != BDEM_SURFACE_PLANE) continue;
-    if (w.num_surfaces < 0 || w.num_surfaces > 16 || w.num_vertex < 0 || w.num_vertex > 16)
-      return fail(c, BDEM_EINVAL, "world object surface/vertex count out of range");
-    for (int s = 0; s < w.num_surfaces; ++s) {
-      const bdem_Surface& S = w.surfaces[s];
-      if (S.area == -1.0f) continue;  /* infinite plane: no polygon */
-      if (S.num_vertex < 1 || S.num_vertex > 8)
-        return fail(c, BDEM_EINVAL, "surface vertex count out of range");
-      for (int q = 0; q < S.num_vertex; ++q)
-        if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= w.num_vertex)
-          return fail(c, BDEM_EINVAL, "surface vertex index out of range");
-    }
-  }
+  if ((rc0 = check_wobj(c, wobj, num_wobj)) != BDEM_OK) return rc0;
 
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1020,6 +1052,7 @@ int bdem_update_world_object_positions(bdem_ctx* c, const bdem_WOBJ* w) {
   int rc = require(c);
   if (rc) return rc;
   if (!w) return fail(c, BDEM_EINVAL, "null argument");
+  if ((rc = check_wobj(c, w, int(c->wobj.size()))) != BDEM_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (size_t k = 0; k < c->wobj.size(); ++k) c->wobj[k] = w[k];
@@ -1033,6 +1066,7 @@ int bdem_open_hatch(bdem_ctx* c, const bdem_WOBJ* w, int nw) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (w) {
     if (nw < 0 || nw > MAX_WOBJ) return fail(c, BDEM_EINVAL, "num_wobj out of range");
+    if ((rc = check_wobj(c, w, nw)) != BDEM_OK) return rc;
     c->wobj.assign(w, w + nw);
   }
   if (c->num_wobj_active > 0) c->num_wobj_active -= 1;

@@ -137,3 +137,29 @@ def test_reference_style_caller_compiles_and_links(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
                     "-L", pkg, "-lbdem", f"-Wl,-rpath,{pkg}"], check=True, capture_output=True)
     assert exe.exists()
+
+
+def test_binding_rejects_misshapen_state_before_calling_the_library():
+    # no context is needed: the checks run before any call into C
+    dev = object.__new__(bdem.Device)
+    dev.n = 4
+    dev.h = None
+    ok = dict(pos=np.zeros((4, 3)), ptype=np.zeros(4), pid=np.arange(4))
+    with pytest.raises(ValueError, match="pos"):
+        dev.set_state(np.zeros((3, 3)), ok["ptype"], ok["pid"])
+    with pytest.raises(ValueError, match="ptype"):
+        dev.set_state(ok["pos"], np.zeros(5), ok["pid"])
+    with pytest.raises(ValueError, match="pid"):
+        dev.set_state(ok["pos"], ok["ptype"], np.arange(3))
+    with pytest.raises(ValueError, match="quat"):
+        dev.set_state(ok["pos"], ok["ptype"], ok["pid"], quat=np.zeros((4, 3)))
+    with pytest.raises(ValueError, match="vel"):
+        dev.set_state(ok["pos"], ok["ptype"], ok["pid"], vel=np.zeros((2, 3)))
+    with pytest.raises(ValueError, match="avel"):
+        dev.set_state(ok["pos"], ok["ptype"], ok["pid"], vel=np.zeros((4, 3)), avel=np.zeros(12))
+    with pytest.raises(ValueError, match="radius"):
+        dev.set_radii(np.ones(3), np.ones(4))
+    with pytest.raises(ValueError, match="mass"):
+        dev.set_radii(np.ones(4), np.ones((4, 1)))
+    with pytest.raises(ValueError, match="tallys"):
+        dev.set_tallys(np.zeros(2))
