@@ -133,6 +133,19 @@ int bdem_state_write(const char* path, int sim_type, int rotating_world, int par
 int bdem_deck_initial_state(const bdem_deck* deck, bdem_float3* pos, bdem_Quaterion* ornt, unsigned int* ptype,
                             int* pid);
 
+/* Seeded random non-overlapping fill (random sequential addition; no
+ * reference counterpart: the reference's fills are lattices,
+ * Device/DeviceInterface.cu:192-601).  Sphere k (in index order) gets a
+ * centre drawn uniformly from the box [lo + r_k, hi - r_k] (also within
+ * radius cyl[2] - r_k of the vertical axis x = cyl[0], z = cyl[1] when cyl
+ * is not NULL), redrawn until its distance to every placed centre is at
+ * least r_j + r_k + gap, at most max_tries draws.  Coordinates come from
+ * splitmix64: state += 0x9E3779B97F4A7C15, then the standard mix; a draw is
+ * (z >> 40) * 2^-24, three per try (x, y, z).  Host only.  BDEM_EINVAL with
+ * the index in *failed (may be NULL) when a sphere cannot be placed. */
+int bdem_fill_random(int n, const float* radius, bdem_float3 lo, bdem_float3 hi, const float* cyl, float gap,
+                     unsigned long long seed, int max_tries, bdem_float3* pos, int* failed);
+
 /* Read a state file of this deck (path NULL: <project_dir>/<start_file>, or
  * the grid fill of bdem_deck_initial_state through bdem_set_pos when the
  * deck has no start file)

@@ -6,4 +6,4 @@ scenarios  seeded synthetic decks (box fill, silo, dense random packs)
 deck       the reference project decks and state files
 """
 from . import types, lib, scenarios, deck  # noqa: F401
-from .lib import Device, load  # noqa: F401
+from .lib import Batch, BdemError, Device, load  # noqa: F401

@@ -102,6 +102,24 @@ _PROTOS = {
                                    _f3p, _qp, _f3p, _f3p, _f3p, _up, _ip, C.POINTER(T.POBJ), C.c_int]),
     "bdem_deck_load_state": (C.c_int, [_P, _P, C.c_char_p, C.POINTER(T.StateHeader)]),
     "bdem_deck_initial_state": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
+    "bdem_fill_random": (C.c_int, [C.c_int, _fp, T.Float3, T.Float3, _fp, C.c_float, C.c_ulonglong, C.c_int,
+                                   _f3p, _ip]),
+    # include/bdem_batch.h
+    "bdem_batch_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "bdem_batch_destroy": (C.c_int, [_P]),
+    "bdem_batch_last_error": (C.c_char_p, [_P]),
+    "bdem_batch_add": (C.c_int, [_P, _P, _ip]),
+    "bdem_batch_set_tuning": (C.c_int, [_P, C.POINTER(T.BatchTuning)]),
+    "bdem_batch_get_tuning": (C.c_int, [_P, C.POINTER(T.BatchTuning)]),
+    "bdem_batch_finalize": (C.c_int, [_P]),
+    "bdem_batch_step": (C.c_int, [_P, C.c_int]),
+    "bdem_batch_sync": (C.c_int, [_P]),
+    "bdem_batch_autotune": (C.c_int, [_P, C.c_int, C.POINTER(T.BatchTuning), _dp]),
+    "bdem_batch_num_instances": (C.c_int, [_P]),
+    "bdem_batch_num_particles": (C.c_int, [_P, C.c_int]),
+    "bdem_batch_instance_sim": (C.c_int, [_P, C.c_int, C.POINTER(T.SimulationInfo)]),
+    "bdem_batch_get_state": (C.c_int, [_P, C.c_int, _f3p, _f3p, _f3p, _ip]),
+    "bdem_batch_get_status": (C.c_int, [_P, _up]),
 }
 
 SPRING_TC, SPRING_KN = 0, 1
@@ -120,6 +138,23 @@ def derive_spring_constants(pobj, mode, cor_pp, cor_surface, cor_dsurface, tc=(0
     for k in range(len(pobj)):
         pobj[k] = arr[k]
     return pobj
+
+
+def fill_random(radius, lo, hi, seed, gap=0.01, cyl=None, max_tries=100000):
+    """bdem_fill_random (host only): a seeded random non-overlapping fill of
+    spheres of these radii (placed in index order) in the box [lo, hi],
+    optionally inside the vertical cylinder cyl = (x, z, R)."""
+    lib = load()
+    r = np.ascontiguousarray(radius, np.float32)
+    n = len(r)
+    pos = np.zeros((n, 3), np.float32)
+    c = None if cyl is None else np.ascontiguousarray(cyl, np.float32)
+    failed = C.c_int(-1)
+    rc = lib.bdem_fill_random(n, _ptr(r, C.c_float), T.f3(*lo), T.f3(*hi), _ptr(c, C.c_float), float(gap),
+                              int(seed) & 0xFFFFFFFFFFFFFFFF, int(max_tries), _ptr(pos, T.Float3), C.byref(failed))
+    if rc != 0:
+        raise BdemError(f"bdem_fill_random: status {rc}, sphere {failed.value} could not be placed")
+    return pos
 
 
 def surface_angle(top, lo, hi):
@@ -424,3 +459,86 @@ class Device:
         self._check(self.lib.bdem_debug_forces(self.h, *[_ptr(a, C.c_float) for a in out],
                                                _ptr(pid, C.c_int)))
         return out[0], out[1], out[2], out[3], pid
+
+
+class Batch:
+    """Many independent sphere simulations advanced together
+    (include/bdem_batch.h).  Each is added from a configured Device whose
+    state has been uploaded and not yet stepped; the Device is not used by
+    the batch afterwards."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = _P()
+        rc = self.lib.bdem_batch_create(C.byref(self.h), int(device))
+        if rc != BDEM_OK:
+            raise BdemError(f"bdem_batch_create failed ({rc})")
+        self.sizes = []
+
+    def _check(self, rc):
+        if rc != BDEM_OK:
+            msg = self.lib.bdem_batch_last_error(self.h).decode()
+            raise BdemError(f"bdem batch error {rc}: {msg}")
+
+    def close(self):
+        if self.h:
+            self.lib.bdem_batch_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, dev):
+        k = C.c_int(-1)
+        self._check(self.lib.bdem_batch_add(self.h, dev.h, C.byref(k)))
+        self.sizes.append(dev.n)
+        return k.value
+
+    def set_tuning(self, threads=256, waves_per_cu=0, grab=1, cell_factor=1.0, use_graph=1):
+        t = T.BatchTuning(int(threads), int(waves_per_cu), int(grab), float(cell_factor), int(use_graph))
+        self._check(self.lib.bdem_batch_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = T.BatchTuning()
+        self._check(self.lib.bdem_batch_get_tuning(self.h, C.byref(t)))
+        return dict(threads=t.threads, waves_per_cu=t.waves_per_cu, grab=t.grab, cell_factor=t.cell_factor,
+                    use_graph=t.use_graph)
+
+    def finalize(self):
+        self._check(self.lib.bdem_batch_finalize(self.h))
+
+    def step(self, nsteps):
+        self._check(self.lib.bdem_batch_step(self.h, int(nsteps)))
+
+    def sync(self):
+        self._check(self.lib.bdem_batch_sync(self.h))
+
+    def autotune(self, steps=10):
+        t = T.BatchTuning()
+        ms = C.c_double(0.0)
+        self._check(self.lib.bdem_batch_autotune(self.h, int(steps), C.byref(t), C.byref(ms)))
+        return dict(threads=t.threads, waves_per_cu=t.waves_per_cu, grab=t.grab, cell_factor=t.cell_factor,
+                    use_graph=t.use_graph), ms.value
+
+    def instance_sim(self, k):
+        s = T.SimulationInfo()
+        self._check(self.lib.bdem_batch_instance_sim(self.h, int(k), C.byref(s)))
+        return s
+
+    def get_state(self, k):
+        n = self.sizes[k]
+        pos = np.zeros((n, 3), np.float32)
+        vel = np.zeros((n, 3), np.float32)
+        avel = np.zeros((n, 3), np.float32)
+        pid = np.zeros(n, np.int32)
+        self._check(self.lib.bdem_batch_get_state(self.h, int(k), _ptr(pos, T.Float3), _ptr(vel, T.Float3),
+                                                  _ptr(avel, T.Float3), _ptr(pid, C.c_int)))
+        return dict(pos=pos, vel=vel, avel=avel, pid=pid)
+
+    def status(self):
+        w = C.c_uint(0)
+        self._check(self.lib.bdem_batch_get_status(self.h, C.byref(w)))
+        return w.value

@@ -261,6 +261,50 @@ def box_deck(law="ref", seed=1, nx=32, ny=16, nz=16, L=50.0, radius=0.5, density
     return _deck(sim, [box_walls(L)], pobj, pos, vel, ext=ext, name=f"box{n}-{law}")
 
 
+STREAM_COUNT = 6
+
+
+def batch_box_decks(count=512, seed=4, nmin=1024, nmax=65536, rmin=0.2, rmax=0.6, packing=0.3,
+                    density=2500.0, dt=1e-5, vmax=10.0, knmin=5e4, knmax=5e5, emin=0.3, emax=0.9,
+                    mumin=0.2, mumax=0.8):
+    """Independent box fills for a batch: counts log-uniform in [nmin, nmax],
+    radii U[rmin, rmax] per particle, each box edge L = (N 4/3 pi E[r^3] /
+    packing)^(1/3), a seeded random non-overlapping fill of [0, L]^3 (larger
+    spheres first), velocities U(+-vmax), and per box kn log-uniform in
+    [knmin, knmax], restitution U[emin, emax], friction U[mumin, mumax];
+    linear spring-dashpot with Coulomb friction, kt = 2/7 kn and tangential
+    history; hash cell 2 rmax of the box's own radii."""
+    from . import lib as L
+    u = uniform(seed, STREAM_COUNT, count)
+    counts = np.rint(np.exp(math.log(nmin) + (math.log(nmax) - math.log(nmin)) * u)).astype(np.int64)
+    mat = uniform(seed, STREAM_MATERIAL, 3 * count).reshape(count, 3)
+    kn = np.exp(math.log(knmin) + (math.log(knmax) - math.log(knmin)) * mat[:, 0])
+    e = emin + (emax - emin) * mat[:, 1]
+    mu = mumin + (mumax - mumin) * mat[:, 2]
+    er3 = (rmax ** 4 - rmin ** 4) / (4.0 * (rmax - rmin))
+    decks = []
+    for i in range(count):
+        n = int(counts[i])
+        sub = seed * 1000003 + i
+        rad = uniform(sub, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+        rad = np.sort(rad)[::-1].copy()
+        Lb = float((n * (4.0 / 3.0) * math.pi * er3 / packing) ** (1.0 / 3.0))
+        pos = L.fill_random(rad, (0.0, 0.0, 0.0), (Lb, Lb, Lb), seed=sub * 64 + STREAM_POS, gap=0.01)
+        vel = uniform(sub, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+        # sphere_mass in double, rounded once to float, for every radius at once
+        mass = ((4.0 / 3.0) * 2.0 * math.asin(1.0) * rad.astype(np.float64) ** 3 * density * 1e-6).astype(np.float32)
+        cell = float(2.0 * rad.max())
+        sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (Lb, Lb, Lb), cell, rotation=True,
+                       roll_res=0.0)
+        pobj = [sphere_pobj(float(rad.max()), density, float(e[i]), float(kn[i]), float(mu[i]))]
+        ext = lsd_ext(float(kn[i]), float(e[i]), float(mu[i]))
+        ext.per_particle_radius = 1
+        decks.append(_deck(sim, [box_walls(Lb)], pobj, pos, vel, ext=ext, radius=rad, mass=mass,
+                           name=f"batchbox{i}-{n}"))
+        decks[-1]["params"] = dict(kn=float(kn[i]), e=float(e[i]), mu=float(mu[i]), L=Lb)
+    return decks
+
+
 def collapse_deck(nx=6, ny=14, nz=19, L=20.0, seed=4, radius=0.5, density=2500.0, kn=1e5, e=0.3, mu=0.6,
                   roll_res=0.001, dt=1e-5, spacing=1.05, jitter=0.02):
     """Column collapse: a jittered lattice of spheres against the x = 0 wall

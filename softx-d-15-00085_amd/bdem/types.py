@@ -273,3 +273,9 @@ EXPECTED_SIZES = {
 
 def f3(x, y, z):
     return Float3(float(x), float(y), float(z))
+
+
+class BatchTuning(C.Structure):
+    """bdem_batch_tuning (include/bdem_batch.h)."""
+    _fields_ = [("threads", C.c_int), ("waves_per_cu", C.c_int), ("grab", C.c_int), ("cell_factor", C.c_float),
+                ("use_graph", C.c_int)]

@@ -7,6 +7,7 @@
  * with no host synchronisation and no allocation.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -15,61 +16,11 @@
 #include <string>
 #include <vector>
 #include "bdem_abi.h"
+#include "bdem_ctx.hpp"
 #include "bdem_kernels.hpp"
 
 using namespace bdem;
 
-struct bdem_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  bool configured = false;
-
-  bdem_SimulationInfo sim{};
-  std::vector<bdem_WOBJ> wobj;
-  std::vector<bdem_POBJ> pobj;
-  std::vector<bdem_DOBJ> dobj;
-  bdem_ext_params ext{};
-  bool has_ext = false;
-  int num_wobj_active = 0;
-  bool cyl_rotation = false;
-  bool record_wf = false;
-  bool start_bin = false;
-  float tallys[6] = {0, 0, 0, 0, 0, 0};
-  float* d_tally = nullptr;  /* device tallies {particle, lifter, surface} the step kernel adds to */
-  StatsWords* d_stats = nullptr;   /* bdem_particle_stats reduction words */
-  uint32_t* d_profile = nullptr;   /* bdem_get_surface_profile bins */
-  /* lifters turned on the device before every step (Rotate_VolObject_Inc) */
-  bool lift_on = false;
-  double lift_c = 1.0, lift_s = 0.0;
-  float lift_about[3] = {0.f, 0.f, 0.f};
-  bool dobj_dirty = false;   /* the device lifter table is ahead of c->dobj */
-
-  DevParams hp{};
-  DevParams* dp = nullptr;
-  StepBuffers b{};
-  std::vector<void*> allocs;
-  bool orient = false;
-  bool poly = false;         /* particle_type 1 */
-  bdem_POBJ* dpobj = nullptr;  /* device copy of the POBJ table (polyhedra) */
-  PolyBuffers pb{};
-  bool keys_valid = false;   /* keys_new holds the keys of the current state */
-  bool flags_valid = false;  /* ... and the step kernel's changed flags for them */
-  int hist_par = 0;
-  int sorted_par = 0;        /* keys_sorted[sorted_par] = latest sorted keys */
-  int append_par = 0;        /* meta[META_APPEND + append_par] counts the changed list */
-  bool full_next = false;    /* a discharge check ran: many keys changed at once */
-  bool lists_full = true;    /* every stencil row is tested in full at the next step */
-  int vpar = 0;              /* skin-mask epoch words of the last step */
-  float skin_req = -1.0f;    /* requested skin (cm); negative or 0: masks off */
-  uint32_t* dbg_list = nullptr;  /* debug: the contacts the step kernel used */
-  uint32_t* dbg_nn = nullptr;
-  int stepc = 0;
-  /* stage timing: one event per stage boundary per recorded step */
-  std::vector<hipEvent_t> tev;
-  std::vector<char> t_lists;   /* per recorded step: the lists boundary was recorded */
-  int t_cap = 0, t_used = 0;
-};
 
 /* Stage boundaries recorded per step when timing is on:
  * 0 start | sort (hash if needed, resort or full sort, gather, cell table)
@@ -188,7 +139,7 @@ void build_params(bdem_ctx* c) {
       const double x = po.Local_Vertex[v].x, y = po.Local_Vertex[v].y, z = po.Local_Vertex[v].z;
       vmax = std::fmax(vmax, std::sqrt(x * x + y * y + z * z));
     }
-    P.t_vbound[t] = float(vmax * 1.0001);
+    P.t_vbound[t] = float(vmax * 1.0001 * c->vbound_scale);
     P.t_mass[t] = po.mass;
     P.t_inertia[t] = po.InertiaT[0];
     for (size_t u = 0; u < c->pobj.size() && u < size_t(MAX_TYPES); ++u) {
@@ -406,14 +357,25 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     if (hist && pid[i] >= c->b.hist_cap)
       return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
     if (check_q && pid[i] >= 0) {
-      /* q v q* scales by |q|^2; the separating-plane shortcut and the
-       * contact depths assume a rotation */
       const double s2 = double(ornt[i].w) * ornt[i].w + double(ornt[i].x) * ornt[i].x +
                         double(ornt[i].y) * ornt[i].y + double(ornt[i].z) * ornt[i].z;
-      if (!(std::fabs(s2 - 1.0) <= 1e-4))
-        return fail(c, BDEM_EINVAL, "orientation quaternion is not unit length at index " + std::to_string(i));
+      if (!std::isfinite(s2))
+        return fail(c, BDEM_EINVAL, "orientation quaternion is not finite at index " + std::to_string(i));
     }
   }
+  /* The reference turns vertices by q v q* as the state gives q, unit or not
+   * (the unit-test start files hold (1, 0.5, 0, 0)); q v q* scales by |q|^2
+   * until the first step renormalises q.  The separating-plane shortcut's
+   * vertex bound is scaled by the largest |q|^2 (at least 1) of the upload,
+   * so it stays a bound. */
+  double qscale = 1.0;
+  if (check_q)
+    for (int i = 0; i < n; ++i)
+      if (pid[i] >= 0) {
+        const double s2 = double(ornt[i].w) * ornt[i].w + double(ornt[i].x) * ornt[i].x +
+                          double(ornt[i].y) * ornt[i].y + double(ornt[i].z) * ornt[i].z;
+        qscale = std::max(qscale, s2 * (1.0 + 1e-6));
+      }
   std::vector<float4> P(n), V(n), O(n), Q;
   std::vector<float4> oldP, oldV, oldO;
   if (keep_vel || c->hp.per_particle) {
@@ -455,6 +417,11 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
   }
   c->keys_valid = false;
   c->lists_full = true;
+  if (c->poly && qscale != c->vbound_scale) {
+    c->vbound_scale = qscale;
+    int rc = upload_params(c);
+    if (rc != BDEM_OK) return rc;
+  }
   return BDEM_OK;
 }
 
@@ -501,7 +468,6 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   a.rk[0] = b.rk[0]; a.rk[1] = b.rk[1];
   a.rv[0] = b.rv[0]; a.rv[1] = b.rv[1];
   a.hist = b.hist;
-  a.part = b.part;
   a.meta = b.meta;
   a.cells = b.cells;
   a.status = b.status;
@@ -601,6 +567,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   std::memset(&c->ext, 0, sizeof(c->ext));
   c->poly = sim->particle_type == 1;
   c->orient = sim->sphere_orient || c->poly;
+  c->vbound_scale = 1.0;
   c->stepc = 0;
   c->hist_par = 0;
   c->lift_on = false;
@@ -624,7 +591,6 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.meta, META_WORDS);
   ALLOC(b.rk[0], n); ALLOC(b.rk[1], n); ALLOC(b.rv[0], n); ALLOC(b.rv[1], n);
   ALLOC(b.hist, (size_t)RADIX * sort_blocks(n));
-  ALLOC(b.part, 1024);
   ALLOC(b.cells, (size_t)C);
   ALLOC(b.status, 1); ALLOC(b.kill_count, 1);
   ALLOC(c->d_tally, 8);

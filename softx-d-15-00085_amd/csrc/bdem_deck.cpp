@@ -1032,6 +1032,78 @@ int bdem_deck_get(const bdem_deck* D, const bdem_deck_info** info, const bdem_Si
   return BDEM_OK;
 }
 
+int bdem_fill_random(int n, const float* radius, bdem_float3 lo, bdem_float3 hi, const float* cyl, float gap,
+                     unsigned long long seed, int max_tries, bdem_float3* pos, int* failed) {
+  if (failed) *failed = -1;
+  if (n < 0 || (n > 0 && (!radius || !pos)) || max_tries < 1 || !(gap >= 0.0f)) return BDEM_EINVAL;
+  if (n == 0) return BDEM_OK;
+  float rmax = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    if (!(radius[k] > 0.0f)) return BDEM_EINVAL;
+    rmax = std::max(rmax, radius[k]);
+  }
+  /* a uniform grid of cells no smaller than the largest contact distance:
+   * every sphere that can touch a candidate is in its 27 cells */
+  const double cell = 2.0 * rmax + gap;
+  const double span[3] = {double(hi.x) - lo.x, double(hi.y) - lo.y, double(hi.z) - lo.z};
+  int dim[3];
+  for (int a = 0; a < 3; ++a) {
+    if (!(span[a] > 0.0)) return BDEM_EINVAL;
+    dim[a] = std::max(1, (int)std::ceil(span[a] / cell));
+  }
+  const long long ncell = (long long)dim[0] * dim[1] * dim[2];
+  if (ncell > (1ll << 31)) return BDEM_EINVAL;
+  std::vector<int> head((size_t)ncell, -1), next((size_t)n, -1);
+  unsigned long long state = seed;
+  auto draw = [&]() {
+    state += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 40) * (1.0 / 16777216.0);
+  };
+  auto cell_of = [&](double v, int a, double o) {
+    int c = (int)std::floor((v - o) / cell);
+    return std::min(std::max(c, 0), dim[a] - 1);
+  };
+  for (int k = 0; k < n; ++k) {
+    const double r = radius[k];
+    const double a0 = lo.x + r, a1 = hi.x - r, b0 = lo.y + r, b1 = hi.y - r, c0 = lo.z + r, c1 = hi.z - r;
+    if (a1 < a0 || b1 < b0 || c1 < c0) { if (failed) *failed = k; return BDEM_EINVAL; }
+    bool placed = false;
+    for (int t = 0; t < max_tries && !placed; ++t) {
+      const double x = a0 + (a1 - a0) * draw(), y = b0 + (b1 - b0) * draw(), z = c0 + (c1 - c0) * draw();
+      const float fx = float(x), fy = float(y), fz = float(z);
+      if (cyl) {
+        const double dx = double(fx) - cyl[0], dz = double(fz) - cyl[1], lim = double(cyl[2]) - r;
+        if (lim < 0.0 || dx * dx + dz * dz > lim * lim) continue;
+      }
+      const int gx = cell_of(fx, 0, lo.x), gy = cell_of(fy, 1, lo.y), gz = cell_of(fz, 2, lo.z);
+      bool clear = true;
+      for (int dy = -1; dy <= 1 && clear; ++dy)
+        for (int dz = -1; dz <= 1 && clear; ++dz)
+          for (int dx = -1; dx <= 1 && clear; ++dx) {
+            const int X = gx + dx, Y = gy + dy, Z = gz + dz;
+            if (X < 0 || Y < 0 || Z < 0 || X >= dim[0] || Y >= dim[1] || Z >= dim[2]) continue;
+            for (int j = head[((size_t)Y * dim[2] + Z) * dim[0] + X]; j >= 0; j = next[j]) {
+              const double ex = double(pos[j].x) - fx, ey = double(pos[j].y) - fy, ez = double(pos[j].z) - fz;
+              const double need = double(radius[j]) + r + gap;
+              if (ex * ex + ey * ey + ez * ez < need * need) { clear = false; break; }
+            }
+          }
+      if (!clear) continue;
+      pos[k] = bdem_float3{fx, fy, fz};
+      const size_t c = ((size_t)gy * dim[2] + gz) * dim[0] + gx;
+      next[k] = head[c];
+      head[c] = k;
+      placed = true;
+    }
+    if (!placed) { if (failed) *failed = k; return BDEM_EINVAL; }
+  }
+  return BDEM_OK;
+}
+
 int bdem_deck_setup(const bdem_deck* D, bdem_ctx* ctx) {
   if (!D || !ctx) return BDEM_EINVAL;
   return bdem_set_simulation_data(ctx, D->wobj.data(), D->pobj.data(), D->dobj.empty() ? nullptr : D->dobj.data(),

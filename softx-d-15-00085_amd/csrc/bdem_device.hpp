@@ -83,6 +83,7 @@ struct DevParams {
   bdem_float3 cell;
   int nx, ny, nz;
   uint32_t C;                 /* nx*ny*nz */
+  uint32_t key_base;          /* added to every key written: simulations sharing one sort (0 alone) */
   /* time stepping */
   float dt;
   bdem_float3 gravity;

@@ -65,6 +65,24 @@ void launch_hash(const DevParams* dP, const StepBuffers& b, hipStream_t s) {
                      b.omg_cur, b.keys_new, b.status);
 }
 
+/* Keys of many simulations sharing one sort: slot i belongs to the
+ * simulation of its 64-slot tile, whose key_base offsets its keys. */
+__global__ __launch_bounds__(256) void k_hash_batch(const DevParams* __restrict__ Pv,
+                                                    const uint32_t* __restrict__ tile_sim, int n,
+                                                    const float4* __restrict__ pos_r,
+                                                    const float4* __restrict__ omg_t,
+                                                    uint32_t* __restrict__ keys, uint32_t* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevParams& P = Pv[tile_sim[i >> 6]];
+  keys[i] = P.key_base + key_of(P, xyz(pos_r[i]), packed_pid(omg_t[i].w), status);
+}
+
+void launch_hash_batch(const DevParams* Pv, const uint32_t* tile_sim, const StepBuffers& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_hash_batch, dim3((b.n + 255) / 256), dim3(256), 0, s, Pv, tile_sim, b.n, b.pos_cur,
+                     b.omg_cur, b.keys_new, b.status);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Pair forces of one particle (all laws).                                    */
 /* ------------------------------------------------------------------------ */
@@ -181,7 +199,7 @@ struct WaveScratch {
 constexpr int WAVE_LDS_BYTES =
     sizeof(WaveStage) > sizeof(WaveScratch) ? (int)sizeof(WaveStage) : (int)sizeof(WaveScratch);
 
-template <int LAW>
+template <int LAW, int BT = STEP_THREADS>
 __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float4* __restrict__ pos_r,
                                                  const float4* __restrict__ vel_m,
                                                  const float4* __restrict__ omg_t,
@@ -219,7 +237,7 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
     if (p < total) {
       const int k = (int)(e >> 6);
       const int otid = wbase_tid + src;
-      const int j = (int)clist[k * STEP_THREADS + otid];
+      const int j = (int)clist[k * BT + otid];
       const int opid = packed_pid(oo.w);
       const HistRec* hin = nullptr;
       int hin_cnt = 0;
@@ -249,27 +267,38 @@ __device__ __forceinline__ void wave_pair_forces(const DevParams& P, const float
 /* Fused step: contacts -> pair forces (+gravity) -> walls -> integrate ->    */
 /* discharge check -> next key.  Reads the sorted scratch, writes the state.  */
 /* ------------------------------------------------------------------------ */
-template <int LAW, bool ORIENT, bool SKIN, bool ENERGY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_step(
-    SkinArgs sk, uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
-    const DevParams* __restrict__ Pp, int n, const uint2* __restrict__ cells,
-    const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
-    const float4* __restrict__ omg_in, const float4* __restrict__ quat_in,
-    float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ omg_out,
-    float4* __restrict__ quat_out, uint32_t* __restrict__ keys_out,
-    const uint32_t* __restrict__ sorted_keys, uint8_t* __restrict__ changed,
-    unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ append,
-    const uint32_t* __restrict__ hwc_in, uint32_t* __restrict__ hwc_out,
-    const HistRec* __restrict__ hr_in, HistRec* __restrict__ hr_out,
-    const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out, int hist_cap,
-    float* __restrict__ wall_force, int do_kill, uint32_t* status, int* kill_count) {
-  const DevParams& P = *Pp;
+
+#define BDEM_STEP_PARAMS                                                                            \
+  int n, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,                      \
+      const float4* __restrict__ omg_in, const float4* __restrict__ quat_in,                        \
+      float4* __restrict__ pos_out, float4* __restrict__ vel_out, float4* __restrict__ omg_out,     \
+      float4* __restrict__ quat_out, uint32_t* __restrict__ keys_out,                               \
+      const uint32_t* __restrict__ sorted_keys, uint8_t* __restrict__ changed,                      \
+      unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ append,                         \
+      const uint32_t* __restrict__ hwc_in, uint32_t* __restrict__ hwc_out,                          \
+      const HistRec* __restrict__ hr_in, HistRec* __restrict__ hr_out,                              \
+      const HistRec* __restrict__ wr_in, HistRec* __restrict__ wr_out, int hist_cap,                \
+      float* __restrict__ wall_force, int do_kill, uint32_t *status, int *kill_count
+#define BDEM_STEP_ARGS                                                                               \
+  n, pos_in, vel_in, omg_in, quat_in, pos_out, vel_out, omg_out, quat_out, keys_out, sorted_keys, changed, \
+      sbuf, append, hwc_in, hwc_out, hr_in, hr_out, wr_in, wr_out, hist_cap, wall_force, do_kill, status,  \
+      kill_count
+
+/* One wave's 64 consecutive sorted slots (lane i & 63 of the wave).  P and
+ * cells are the slot range's simulation: its parameter block and its cell
+ * table (indexed by the simulation's own hash); key_base is added to every
+ * key written, so that simulations sharing one sort keep disjoint key ranges
+ * (0 for a lone simulation).  clist is the block's contact list, wl this
+ * wave's staging area.  Every lane of the wave must call it.  The streams
+ * are restrict parameters (not struct members) so that the inlined body
+ * keeps their no-alias guarantee. */
+template <int LAW, bool ORIENT, bool SKIN, bool ENERGY, int BT = STEP_THREADS>
+__device__ __forceinline__ void step_wave(const DevParams& P, const uint2* __restrict__ cells, uint32_t key_base,
+                                          int i, uint32_t* clist, unsigned char* wl, const SkinArgs& sk,
+                                          uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
+                                          BDEM_STEP_PARAMS) {
   float* const tally = ENERGY ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
-  __shared__ uint32_t clist[CLIST * STEP_THREADS];
-  /* per wave: candidate staging in phase 1, pair results in phase 2 */
-  __shared__ __align__(16) unsigned char wlds[STEP_THREADS / 64][WAVE_LDS_BYTES];
-  const int base_i = xcd_block(blockIdx.x, gridDim.x) * STEP_THREADS;
-  const int i = base_i + threadIdx.x;
+  const int base_i = i - (int)(threadIdx.x & 63);
   const bool in = i < n;
   float4 pr = make_float4(0.f, 0.f, 0.f, 0.f), vm = pr, ot = pr;
   uint32_t hwc = 0u;
@@ -285,7 +314,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 
   /* phase 1: contact partners in the reference order, into this thread's list */
   auto record = [&](int j, float4, int k) {
-    if (k < CLIST) clist[k * STEP_THREADS + threadIdx.x] = (uint32_t)j;
+    if (k < CLIST) clist[k * BT + threadIdx.x] = (uint32_t)j;
     if (dbg_list && k < LIST_STRIDE) dbg_list[(size_t)i * LIST_STRIDE + k] = (uint32_t)j;
   };
   int nc = 0;
@@ -305,11 +334,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       if ((flags >> rr) & 1u) m0 &= ~(0x7Full << (7 * rr));
     MaskedRows rows{m0, flags, sk.skin};
     nc = for_each_contact_wave_rows(P, cells, pos_in, (uint32_t)n, i, p, r, alive,
-                                    *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]), record, rows);
+                                    *reinterpret_cast<WaveStage*>(wl), record, rows);
     new_mask = rows.mask;
   } else {
     nc = for_each_contact_wave(P, cells, pos_in, (uint32_t)n, i, p, r, alive,
-                               *reinterpret_cast<WaveStage*>(wlds[threadIdx.x >> 6]), record);
+                               *reinterpret_cast<WaveStage*>(wl), record);
   }
   if (dbg_nn && in) dbg_nn[i] = (uint32_t)nc;
   __builtin_amdgcn_wave_barrier();
@@ -317,8 +346,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   f3 Fpp = alive ? cvt(P.gravity) * m : mk3(0.f, 0.f, 0.f);
   f3 Tpp = mk3(0.f, 0.f, 0.f);
   const int nl = nc < CLIST ? nc : CLIST;
-  wave_pair_forces<LAW>(P, pos_in, vel_in, omg_in, clist, nl, base_i, pr, vm, ot, hwc, hr_in, hr_out, hist_cap,
-                        *reinterpret_cast<WaveScratch*>(wlds[threadIdx.x >> 6]), Fpp, Tpp, tally);
+  wave_pair_forces<LAW, BT>(P, pos_in, vel_in, omg_in, clist, nl, base_i, pr, vm, ot, hwc, hr_in, hr_out, hist_cap,
+                        *reinterpret_cast<WaveScratch*>(wl), Fpp, Tpp, tally);
 
   bool chg = false;
   float dmove = 0.0f;
@@ -405,7 +434,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   }
   uint32_t new_key = 0u;
   if (in) {
-    new_key = key_of(P, p, pid, status);
+    new_key = key_base + key_of(P, p, pid, status);
     keys_out[i] = new_key;
     chg = new_key != sorted_keys[i];
     changed[i] = chg ? 1 : 0;
@@ -433,13 +462,119 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   }
 }
 
+/* One simulation: block b (renumbered XCD-aware) owns sorted slots
+ * [256 b, 256 b + 256). */
+template <int LAW, bool ORIENT, bool SKIN, bool ENERGY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_step(
+    SkinArgs sk, uint32_t* __restrict__ dbg_list, uint32_t* __restrict__ dbg_nn,
+    const DevParams* __restrict__ Pp, const uint2* __restrict__ cells, BDEM_STEP_PARAMS) {
+  __shared__ uint32_t clist[CLIST * STEP_THREADS];
+  /* per wave: candidate staging in phase 1, pair results in phase 2 */
+  __shared__ __align__(16) unsigned char wlds[STEP_THREADS / 64][WAVE_LDS_BYTES];
+  const int i = xcd_block(blockIdx.x, gridDim.x) * STEP_THREADS + threadIdx.x;
+  step_wave<LAW, ORIENT, SKIN, ENERGY>(*Pp, cells, 0u, i, clist, wlds[threadIdx.x >> 6], sk, dbg_list, dbg_nn,
+                                       BDEM_STEP_ARGS);
+}
+
+/* Many independent simulations in one launch.  Simulation s owns the sorted
+ * slots [64 t, 64 t + 64) of every tile t with tile_sim[t] == s (its slot
+ * range is padded to whole tiles with dead slots), its own parameter block
+ * Pv[s] and the cell table cells + Pv[s].key_base.
+ *
+ * PERSIST: a persistent grid deals the tiles from NUM_XCD queues, queue x
+ * holding the x-th contiguous eighth of the tiles.  A wave first drains the
+ * queue of the XCD its workgroup was dispatched to (workgroup w runs on XCD
+ * w % 8), so neighbouring tiles, which read each other's cells and
+ * particles, share an L2; then it helps the other queues in turn.  Each grab
+ * takes `grab` consecutive tiles.  Counters queue[8 par .. 8 par + 7] serve
+ * this step; the other set is cleared here for the next step (both start at
+ * zero).  Otherwise each wave of the grid takes one tile, the workgroups
+ * renumbered XCD-aware as in k_step. */
+template <int LAW, int BT, bool PERSIST>
+__device__ __forceinline__ void batch_body(const DevParams* __restrict__ Pv, const uint32_t* __restrict__ tile_sim,
+                                           uint32_t ntiles, uint32_t grab, uint32_t* __restrict__ queue, int par,
+                                           const uint2* __restrict__ cells, uint32_t* clist,
+                                           unsigned char (*wlds)[WAVE_LDS_BYTES], BDEM_STEP_PARAMS) {
+  const int lane = threadIdx.x & 63;
+  const SkinArgs none{};
+  if (!PERSIST) {
+    const uint32_t tile = (uint32_t)xcd_block(blockIdx.x, gridDim.x) * (BT / 64) + (threadIdx.x >> 6);
+    if (tile >= ntiles) return;   /* whole waves */
+    const DevParams& P = Pv[__builtin_amdgcn_readfirstlane(tile_sim[tile])];
+    step_wave<LAW, false, false, false, BT>(P, cells + P.key_base, P.key_base, (int)(tile * 64u) + lane, clist,
+                                            wlds[threadIdx.x >> 6], none, nullptr, nullptr, BDEM_STEP_ARGS);
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < NUM_XCD) queue[NUM_XCD * (par ^ 1) + threadIdx.x] = 0u;
+  uint32_t* const q = queue + NUM_XCD * par;
+  const int home = (int)(blockIdx.x % NUM_XCD);
+  for (int hop = 0; hop < NUM_XCD;) {
+    const int x = (home + hop) % NUM_XCD;
+    const uint32_t lo = (uint32_t)(((uint64_t)ntiles * (uint32_t)x) / NUM_XCD);
+    const uint32_t hi = (uint32_t)(((uint64_t)ntiles * (uint32_t)(x + 1)) / NUM_XCD);
+    uint32_t t = 0u;
+    if (lane == 0) t = atomicAdd(q + x, grab);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= hi - lo) { ++hop; continue; }
+    const uint32_t tend = min(lo + t + grab, hi);
+    for (uint32_t tile = lo + t; tile < tend; ++tile) {
+      const DevParams& P = Pv[__builtin_amdgcn_readfirstlane(tile_sim[tile])];
+      step_wave<LAW, false, false, false, BT>(P, cells + P.key_base, P.key_base, (int)(tile * 64u) + lane, clist,
+                                              wlds[threadIdx.x >> 6], none, nullptr, nullptr, BDEM_STEP_ARGS);
+    }
+  }
+}
+
+/* The persistent loop keeps more values live across tiles: it is held to
+ * four waves per SIMD (128 VGPRs, no spills) where the one-tile-per-wave
+ * grid runs at six. */
+template <int LAW, int BT, bool PERSIST>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(PERSIST ? 4 : 6))) void k_step_batch(
+    const DevParams* __restrict__ Pv, const uint32_t* __restrict__ tile_sim, uint32_t ntiles, uint32_t grab,
+    uint32_t* __restrict__ queue, int par, const uint2* __restrict__ cells, BDEM_STEP_PARAMS) {
+  __shared__ uint32_t clist[CLIST * BT];
+  __shared__ __align__(16) unsigned char wlds[BT / 64][WAVE_LDS_BYTES];
+  batch_body<LAW, BT, PERSIST>(Pv, tile_sim, ntiles, grab, queue, par, cells, clist, wlds, BDEM_STEP_ARGS);
+}
+
+template <int LAW, int BT, bool PERSIST>
+static void launch_batch_t(const BatchLaunch& L, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
+                           hipStream_t s) {
+  const unsigned blocks = PERSIST ? (unsigned)L.blocks : (L.ntiles + (BT / 64) - 1) / (BT / 64);
+  hipLaunchKernelGGL((k_step_batch<LAW, BT, PERSIST>), dim3(blocks), dim3(BT), 0, s, L.params, L.tile_sim, L.ntiles,
+                     L.grab, L.queue, L.par, L.cells, b.n, b.pos_srt, b.vel_srt, b.omg_srt, nullptr, b.pos_cur,
+                     b.vel_cur, b.omg_cur, nullptr, b.keys_new, sk, b.changed, b.sbuf, b.meta + META_APPEND + ap,
+                     b.hwc_srt, b.hwc_cur, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp], b.wrec[hp ^ 1], b.hist_cap,
+                     nullptr, 0, b.status, b.kill_count);
+}
+
+template <int LAW>
+static void launch_batch_l(const BatchLaunch& L, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
+                           hipStream_t s) {
+  if (L.blocks > 0) {
+    if (L.threads == 64) launch_batch_t<LAW, 64, true>(L, b, sk, hp, ap, s);
+    else if (L.threads == 128) launch_batch_t<LAW, 128, true>(L, b, sk, hp, ap, s);
+    else launch_batch_t<LAW, 256, true>(L, b, sk, hp, ap, s);
+  } else {
+    if (L.threads == 64) launch_batch_t<LAW, 64, false>(L, b, sk, hp, ap, s);
+    else if (L.threads == 128) launch_batch_t<LAW, 128, false>(L, b, sk, hp, ap, s);
+    else launch_batch_t<LAW, 256, false>(L, b, sk, hp, ap, s);
+  }
+}
+
+void launch_step_batch(const BatchLaunch& L, const StepBuffers& b, const uint32_t* sorted_keys, int hp, int ap,
+                       hipStream_t s) {
+  if (L.law == BDEM_LAW_REFERENCE) launch_batch_l<BDEM_LAW_REFERENCE>(L, b, sorted_keys, hp, ap, s);
+  else if (L.law == BDEM_LAW_LSD_COULOMB) launch_batch_l<BDEM_LAW_LSD_COULOMB>(L, b, sorted_keys, hp, ap, s);
+  else launch_batch_l<BDEM_LAW_HERTZ_MINDLIN>(L, b, sorted_keys, hp, ap, s);
+}
+
 template <int LAW, bool ORIENT, bool SKIN, bool ENERGY>
 static void launch_step_t(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
                           bool do_kill, bool want_wf, const SkinArgs& sa, uint32_t* dbg_list, uint32_t* dbg_nn,
                           hipStream_t s) {
   hipLaunchKernelGGL((k_step<LAW, ORIENT, SKIN, ENERGY>), dim3((b.n + 255) / 256), dim3(256), 0, s, sa, dbg_list, dbg_nn,
-                     dP, b.n,
-                     b.cells, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
+                     dP, b.cells, b.n, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur, b.vel_cur,
                      b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf, b.meta + META_APPEND + ap,
                      b.hwc_srt, b.hwc_cur, b.hrec[hp], b.hrec[hp ^ 1], b.wrec[hp], b.wrec[hp ^ 1],
                      b.hist_cap, want_wf ? b.wall_force : nullptr, do_kill ? 1 : 0, b.status, b.kill_count);

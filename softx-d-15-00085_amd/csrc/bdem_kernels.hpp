@@ -25,7 +25,6 @@ struct StepBuffers {
   uint32_t *rk[2];                     /* full radix sort ping-pong */
   int *rv[2];
   uint32_t *hist;                      /* full radix digit counters */
-  uint32_t *part;                      /* full radix scan partials (<= 1024) */
   uint2 *cells;                        /* (start, end) per cell, reference format */
   /* tangential history: records indexed by P_ID, double-buffered across
    * steps; their counts (pair | wall << 16) travel with the state in slot
@@ -107,7 +106,6 @@ struct SortArgs {
   uint32_t* rk[2];
   int* rv[2];
   uint32_t* hist;
-  uint32_t* part;
   uint32_t* meta;
   uint2* cells;
   uint32_t* status;
@@ -134,6 +132,29 @@ hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t
 void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sorted_keys, int law,
                  bool orient, int hist_par, int append_par, bool do_kill, bool want_wall_force, bool energy,
                  const SkinArgs& sk, uint32_t* dbg_list, uint32_t* dbg_nn, hipStream_t s);
+/* Many simulations in one persistent launch (k_step_batch): params[s] is
+ * simulation s's parameter block (its key_base offsets its keys and cell
+ * table inside the shared sort), tile_sim[t] the simulation owning sorted
+ * slots [64 t, 64 t + 64).  queue holds 2 x 8 tile counters, set par serving
+ * this step (the persistent launch clears the other).  threads: 64, 128 or
+ * 256 per workgroup; blocks: the persistent grid (0: one tile per wave, no
+ * queue); grab: tiles taken per atomic. */
+struct BatchLaunch {
+  const DevParams* params;
+  const uint32_t* tile_sim;
+  uint32_t ntiles;
+  uint32_t grab;
+  uint32_t* queue;
+  int par;
+  const uint2* cells;
+  int law;
+  int threads;
+  int blocks;
+};
+/* Keys of a batch (n a multiple of 64): tile_sim[i / 64] owns slot i. */
+void launch_hash_batch(const DevParams* Pv, const uint32_t* tile_sim, const StepBuffers& b, hipStream_t s);
+void launch_step_batch(const BatchLaunch& L, const StepBuffers& b, const uint32_t* sorted_keys, int hist_par,
+                       int append_par, hipStream_t s);
 /* Rotate_VolObject_Inc on the device: every lifter attached to the drum in
  * the parameter block turned by the angle whose cosine and sine are c, s
  * about (ax, ay, az), its faces re-derived.  One small launch. */

@@ -34,15 +34,13 @@
  * When L_b exceeds LDS the block sorts it in global scratch instead (slow,
  * correct): the kernel handles any S.
  *
- * Full path: one cooperative kernel does the LSD radix sort (8-bit digits,
- * wave64 ballot ranking), the gather, the table clear and the binning.  The
+ * Full path: an LSD radix sort (8-bit digits, wave64 ballot ranking) by
+ * ordinary launches, then the gather, the table clear and the binning.  The
  * host runs it on the first step after an upload and after a discharge
  * check, when many particles change key at once.
  */
-#include <hip/hip_cooperative_groups.h>
 #include "bdem_kernels.hpp"
 
-namespace cg = cooperative_groups;
 
 namespace bdem {
 
@@ -384,163 +382,128 @@ __device__ __forceinline__ void bin_one(int n, int i, const uint32_t* __restrict
 }
 
 /* ------------------------------------------------------------------------ */
-/* Full path: one cooperative kernel                                          */
+/* Full path: LSD radix sort by ordinary launches (per 8-bit digit: tile     */
+/* histograms, one scan, stable scatter), then gather, table clear and bin.  */
+/* No cooperative launch: those hold the device's global wave-sync resource,  */
+/* which one process at a time can own.                                       */
 /* ------------------------------------------------------------------------ */
-__global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int par) {
-  cg::grid_group grid = cg::this_grid();
-  __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
-  __shared__ uint32_t red[SORT_THREADS];
-  __shared__ uint32_t carry;
+/* Digit histogram of tile blockIdx.x: hist[digit * ntiles + tile].  The
+ * first pass also resets the next step kernel's append counter. */
+__global__ __launch_bounds__(SORT_THREADS) void k_fs_hist(SortArgs a, int par, const uint32_t* __restrict__ kin,
+                                                          int shift, int first) {
+  __shared__ uint32_t h[RADIX];
   const int n = a.n;
   const int ntiles = sort_blocks(n);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  const int m = RADIX * ntiles;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int t = blockIdx.x;
+  if (first && t == 0 && threadIdx.x == 0) {
     a.meta[META_APPEND + (par ^ 1)] = 0u;   /* the next step kernel appends there */
     a.meta[META_COUNT] = (uint32_t)n;
   }
-
-  const uint32_t* kin = a.keys_new;
-  const int* vin = nullptr;
-  for (int p = 0; p < a.passes; ++p) {
-    const int shift = 8 * p;
-    const bool last = p == a.passes - 1;
-    uint32_t* kout = last ? a.keys_sorted : a.rk[p & 1];
-    int* vout = last ? a.idx_sorted : a.rv[p & 1];
-    /* 1. digit histogram per tile: hist[digit * ntiles + tile] */
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      uint32_t* h = &wcnt[0][0];
-      for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) h[d] = 0u;
-      __syncthreads();
-      for (int q = threadIdx.x; q < SORT_TILE; q += SORT_THREADS) {
-        const int i = t * SORT_TILE + q;
-        if (i < n) atomicAdd(&h[(kin[i] >> shift) & (RADIX - 1)], 1u);
-      }
-      __syncthreads();
-      for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) a.hist[d * ntiles + t] = h[d];
-      __syncthreads();
-    }
-    grid.sync();
-    /* 2. exclusive scan of the m counters: chunk sums, one block scans them, chunk rescan */
-    const int chunk = (m + gridDim.x - 1) / gridDim.x;
-    const int c0 = blockIdx.x * chunk, c1 = min(c0 + chunk, m);
-    {
-      uint32_t s = 0u;
-      for (int i = c0 + threadIdx.x; i < c1; i += SORT_THREADS) s += a.hist[i];
-      red[threadIdx.x] = s;
-      __syncthreads();
-      for (int o = SORT_THREADS / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
-    }
-    grid.sync();
-    if (blockIdx.x == 0) {
-      if (threadIdx.x == 0) carry = 0u;
-      __syncthreads();
-      for (int base = 0; base < (int)gridDim.x; base += SORT_THREADS) {
-        const int i = base + threadIdx.x;
-        const uint32_t x = i < (int)gridDim.x ? a.part[i] : 0u;
-        red[threadIdx.x] = x;
-        __syncthreads();
-        for (int o = 1; o < SORT_THREADS; o <<= 1) {
-          const uint32_t y = threadIdx.x >= (unsigned)o ? red[threadIdx.x - o] : 0u;
-          __syncthreads();
-          red[threadIdx.x] += y;
-          __syncthreads();
-        }
-        if (i < (int)gridDim.x) a.part[i] = carry + red[threadIdx.x] - x;
-        __syncthreads();
-        if (threadIdx.x == SORT_THREADS - 1) carry += red[SORT_THREADS - 1];
-        __syncthreads();
-      }
-    }
-    grid.sync();
-    {
-      if (threadIdx.x == 0) carry = a.part[blockIdx.x];
-      __syncthreads();
-      for (int base = c0; base < c1; base += SORT_THREADS) {
-        const int i = base + threadIdx.x;
-        const uint32_t x = i < c1 ? a.hist[i] : 0u;
-        red[threadIdx.x] = x;
-        __syncthreads();
-        for (int o = 1; o < SORT_THREADS; o <<= 1) {
-          const uint32_t y = threadIdx.x >= (unsigned)o ? red[threadIdx.x - o] : 0u;
-          __syncthreads();
-          red[threadIdx.x] += y;
-          __syncthreads();
-        }
-        if (i < c1) a.hist[i] = carry + red[threadIdx.x] - x;
-        __syncthreads();
-        if (threadIdx.x == SORT_THREADS - 1) carry += red[SORT_THREADS - 1];
-        __syncthreads();
-      }
-    }
-    grid.sync();
-    /* 3. scatter each tile (stable wave-ballot ranking, as a one-pass kernel would) */
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS)
-        for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][d] = 0u;
-      __syncthreads();
-      const int base = t * SORT_TILE + wave * (SORT_ITEMS * 64);
-      uint32_t key[SORT_ITEMS];
-      int val[SORT_ITEMS];
-      uint32_t loc[SORT_ITEMS];
-#pragma unroll
-      for (int it = 0; it < SORT_ITEMS; ++it) {
-        const int i = base + it * 64 + lane;
-        const bool valid = i < n;
-        const uint32_t k = valid ? kin[i] : 0u;
-        const int v = valid ? (vin ? vin[i] : i) : 0;
-        const uint32_t d = (k >> shift) & (RADIX - 1);
-        uint64_t eq = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-          const bool set = (d >> bit) & 1u;
-          const uint64_t mm = __ballot(set);
-          eq &= set ? mm : ~mm;
-        }
-        const uint32_t rank = (uint32_t)__popcll(eq & lt);
-        const uint32_t before = wcnt[wave][d];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && (eq & lt) == 0ull) wcnt[wave][d] = before + (uint32_t)__popcll(eq);
-        __builtin_amdgcn_wave_barrier();
-        key[it] = k;
-        val[it] = v;
-        loc[it] = before + rank;
-      }
-      __syncthreads();
-      for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
-        uint32_t s = a.hist[d * ntiles + t];
-        for (int w = 0; w < SORT_THREADS / 64; ++w) {
-          const uint32_t c = wcnt[w][d];
-          wcnt[w][d] = s;
-          s += c;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int it = 0; it < SORT_ITEMS; ++it) {
-        const int i = base + it * 64 + lane;
-        if (i < n) {
-          const uint32_t d = (key[it] >> shift) & (RADIX - 1);
-          const uint32_t pos = wcnt[wave][d] + loc[it];
-          kout[pos] = key[it];
-          vout[pos] = val[it];
-        }
-      }
-      __syncthreads();
-    }
-    grid.sync();
-    kin = kout;
-    vin = vout;
+  for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) h[d] = 0u;
+  __syncthreads();
+  for (int q = threadIdx.x; q < SORT_TILE; q += SORT_THREADS) {
+    const int i = t * SORT_TILE + q;
+    if (i < n) atomicAdd(&h[(kin[i] >> shift) & (RADIX - 1)], 1u);
   }
-  /* gather + table clear, then bin */
+  __syncthreads();
+  for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) a.hist[d * ntiles + t] = h[d];
+}
+
+/* Exclusive scan of the m digit-major counters in one block: each thread
+ * sums a contiguous chunk, the chunk sums are scanned in LDS, and each thread
+ * rewrites its chunk as running offsets. */
+constexpr int SCAN_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_THREADS) void k_fs_scan(uint32_t* __restrict__ hist, int m) {
+  __shared__ uint32_t red[SCAN_THREADS];
+  const int chunk = (m + SCAN_THREADS - 1) / SCAN_THREADS;
+  const int c0 = min((int)threadIdx.x * chunk, m), c1 = min(c0 + chunk, m);
+  uint32_t s = 0u;
+  for (int i = c0; i < c1; ++i) s += hist[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+    const uint32_t y = threadIdx.x >= (unsigned)o ? red[threadIdx.x - o] : 0u;
+    __syncthreads();
+    red[threadIdx.x] += y;
+    __syncthreads();
+  }
+  uint32_t run = red[threadIdx.x] - s;
+  for (int i = c0; i < c1; ++i) {
+    const uint32_t x = hist[i];
+    hist[i] = run;
+    run += x;
+  }
+}
+
+/* Stable scatter of tile blockIdx.x: each wave ranks its 16 x 64 keys by
+ * ballots over the digit bits (equal digits in lane order), the waves'
+ * counts are offset in wave order on top of the tile's scanned base. */
+__global__ __launch_bounds__(SORT_THREADS) void k_fs_scatter(SortArgs a, const uint32_t* __restrict__ kin,
+                                                             const int* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                             int* __restrict__ vout, int shift) {
+  __shared__ uint32_t wcnt[SORT_THREADS / 64][RADIX];
+  const int n = a.n;
+  const int ntiles = sort_blocks(n);
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS)
+    for (int w = 0; w < SORT_THREADS / 64; ++w) wcnt[w][d] = 0u;
+  __syncthreads();
+  const int base = t * SORT_TILE + wave * (SORT_ITEMS * 64);
+  uint32_t key[SORT_ITEMS];
+  int val[SORT_ITEMS];
+  uint32_t loc[SORT_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; ++it) {
+    const int i = base + it * 64 + lane;
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const int v = valid ? (vin ? vin[i] : i) : 0;
+    const uint32_t d = (k >> shift) & (RADIX - 1);
+    uint64_t eq = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool set = (d >> bit) & 1u;
+      const uint64_t mm = __ballot(set);
+      eq &= set ? mm : ~mm;
+    }
+    const uint32_t rank = (uint32_t)__popcll(eq & lt);
+    const uint32_t before = wcnt[wave][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (eq & lt) == 0ull) wcnt[wave][d] = before + (uint32_t)__popcll(eq);
+    __builtin_amdgcn_wave_barrier();
+    key[it] = k;
+    val[it] = v;
+    loc[it] = before + rank;
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) {
+    uint32_t s = a.hist[d * ntiles + t];
+    for (int w = 0; w < SORT_THREADS / 64; ++w) {
+      const uint32_t c = wcnt[w][d];
+      wcnt[w][d] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; ++it) {
+    const int i = base + it * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (key[it] >> shift) & (RADIX - 1);
+      const uint32_t pos = wcnt[wave][d] + loc[it];
+      kout[pos] = key[it];
+      vout[pos] = val[it];
+    }
+  }
+}
+
+/* The sorted order's state gather, and every cell marked empty. */
+__global__ __launch_bounds__(SORT_THREADS) void k_fs_gather(SortArgs a) {
   const int gsize = gridDim.x * SORT_THREADS;
-  for (int i = blockIdx.x * SORT_THREADS + threadIdx.x; i < n; i += gsize) {
+  for (int i = blockIdx.x * SORT_THREADS + threadIdx.x; i < a.n; i += gsize) {
     const int j = a.idx_sorted[i];
     a.pos_srt[i] = a.pos_cur[j];
     a.vel_srt[i] = a.vel_cur[j];
@@ -550,25 +513,34 @@ __global__ __launch_bounds__(SORT_THREADS) void k_full_sort(SortArgs a, int par)
   }
   for (uint32_t c = blockIdx.x * SORT_THREADS + threadIdx.x; c < a.C; c += (uint32_t)gsize)
     a.cells[c] = make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-  grid.sync();
-  for (int i = blockIdx.x * SORT_THREADS + threadIdx.x; i < n; i += gsize)
-    bin_one(n, i, a.keys_sorted, a.cells, a.C);
 }
 
-static int coop_grid = 0;
+__global__ __launch_bounds__(SORT_THREADS) void k_fs_bin(SortArgs a) {
+  const int i = blockIdx.x * SORT_THREADS + threadIdx.x;
+  if (i < a.n) bin_one(a.n, i, a.keys_sorted, a.cells, a.C);
+}
 
-static int full_sort_grid() {
-  if (coop_grid > 0) return coop_grid;
-  int dev = 0, cus = 0, per = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_full_sort),
-                                                     SORT_THREADS, 0);
-  int g = cus * (per > 0 ? per : 1);
-  if (g > 1024) g = 1024;     /* part[] capacity */
+static hipError_t launch_full_sort(const SortArgs& a, int par, hipStream_t s) {
+  const int ntiles = sort_blocks(a.n);
+  const uint32_t* kin = a.keys_new;
+  const int* vin = nullptr;
+  for (int p = 0; p < a.passes; ++p) {
+    const int shift = 8 * p;
+    const bool last = p == a.passes - 1;
+    uint32_t* kout = last ? a.keys_sorted : a.rk[p & 1];
+    int* vout = last ? a.idx_sorted : a.rv[p & 1];
+    hipLaunchKernelGGL(k_fs_hist, dim3(ntiles), dim3(SORT_THREADS), 0, s, a, par, kin, shift, p == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_fs_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a.hist, RADIX * ntiles);
+    hipLaunchKernelGGL(k_fs_scatter, dim3(ntiles), dim3(SORT_THREADS), 0, s, a, kin, vin, kout, vout, shift);
+    kin = kout;
+    vin = vout;
+  }
+  int g = (int)((std::max<uint64_t>((uint64_t)a.n, (uint64_t)a.C) + SORT_THREADS - 1) / SORT_THREADS);
+  if (g > 8192) g = 8192;
   if (g < 1) g = 1;
-  coop_grid = g;
-  return g;
+  hipLaunchKernelGGL(k_fs_gather, dim3(g), dim3(SORT_THREADS), 0, s, a);
+  hipLaunchKernelGGL(k_fs_bin, dim3((a.n + SORT_THREADS - 1) / SORT_THREADS), dim3(SORT_THREADS), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t s) {
@@ -576,11 +548,7 @@ hipError_t launch_sort(const SortArgs& a, bool incremental, int par, hipStream_t
     hipLaunchKernelGGL(k_resort, dim3((a.n + RS_TILE - 1) / RS_TILE), dim3(RS_THREADS), 0, s, a, par);
     return hipGetLastError();
   }
-  SortArgs args = a;
-  int p = par;
-  void* params[] = {&args, &p};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_full_sort), dim3(full_sort_grid()),
-                                    dim3(SORT_THREADS), params, 0, s);
+  return launch_full_sort(a, par, s);
 }
 
 }  // namespace bdem

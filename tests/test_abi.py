@@ -13,7 +13,7 @@ from bdem import scenarios as S
 from bdem import types as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("bdem_abi.h", "bdem_deck.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("bdem_abi.h", "bdem_deck.h", "bdem_batch.h")]
 
 
 def header_functions():
@@ -29,6 +29,7 @@ def test_struct_sizes_match_reference_layout():
     assert T.SimulationInfo.num_NNCells.offset == 68
     assert T.POBJ.PP.offset == 3268
     assert T.WOBJ.cyl_geo.offset == 1296
+    assert ctypes.sizeof(T.BatchTuning) == 20
 
 
 def test_library_exports_every_declared_entry_point():
@@ -163,3 +164,53 @@ def test_binding_rejects_misshapen_state_before_calling_the_library():
         dev.set_radii(np.ones(4), np.ones((4, 1)))
     with pytest.raises(ValueError, match="tallys"):
         dev.set_tallys(np.zeros(2))
+
+
+def test_random_fill_is_seeded_and_non_overlapping():
+    """bdem_fill_random: every sphere inside the box (and cylinder), no pair
+    closer than r_i + r_j + gap (all pairs, brute force), the same seed gives
+    the same fill and another seed a different one."""
+    rad = S.uniform(3, S.STREAM_RADIUS, 600, 0.2, 0.6).astype(np.float32)
+    lo, hi = (0.0, 0.0, 0.0), (12.0, 14.0, 12.0)
+    a = bdem.lib.fill_random(rad, lo, hi, seed=77, gap=0.01, cyl=(6.0, 6.0, 5.9))
+    b = bdem.lib.fill_random(rad, lo, hi, seed=77, gap=0.01, cyl=(6.0, 6.0, 5.9))
+    c = bdem.lib.fill_random(rad, lo, hi, seed=78, gap=0.01, cyl=(6.0, 6.0, 5.9))
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    p = a.astype(np.float64)
+    r = rad.astype(np.float64)
+    assert np.all(p - r[:, None] >= -1e-6) and np.all(p + r[:, None] <= np.array(hi) + 1e-6)
+    assert np.all(np.hypot(p[:, 0] - 6.0, p[:, 2] - 6.0) + r <= 5.9 + 1e-5)
+    d = np.linalg.norm(p[:, None, :] - p[None, :, :], axis=2)
+    need = r[:, None] + r[None, :] + 0.01
+    np.fill_diagonal(d, np.inf)
+    assert np.all(d >= need - 1e-5)
+
+
+def test_random_fill_reports_an_impossible_fill():
+    rad = np.full(50, 1.0, np.float32)
+    with pytest.raises(bdem.BdemError, match="could not be placed"):
+        bdem.lib.fill_random(rad, (0, 0, 0), (4, 4, 4), seed=1, max_tries=200)
+
+
+def test_batch_generator_follows_the_configuration():
+    decks = S.batch_box_decks(count=4, seed=4, nmin=1024, nmax=3000)
+    for d in decks:
+        n = d["sim"].Num_Particles
+        assert 1024 <= n <= 3000
+        r = d["radius"]
+        assert r.min() >= 0.2 and r.max() <= 0.6
+        prm = d["params"]
+        assert 5e4 <= prm["kn"] <= 5e5 and 0.3 <= prm["e"] <= 0.9 and 0.2 <= prm["mu"] <= 0.8
+        vol = (4.0 / 3.0) * np.pi * np.sum(r.astype(np.float64) ** 3)
+        assert 0.2 < vol / prm["L"] ** 3 < 0.4
+        assert d["sim"].cellSize.x == pytest.approx(2.0 * r.max())
+
+
+def test_batch_without_gpu_fails_cleanly():
+    lib = bdem.lib.load()
+    h = ctypes.c_void_p()
+    rc = lib.bdem_batch_create(ctypes.byref(h), 0)
+    if rc == 0:
+        lib.bdem_batch_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc != 0 and not h.value

@@ -70,7 +70,7 @@ def test_contact_lists_bit_exact(pair):
     lg, ng = dev.debug_contacts()
     lo, no = ora.contacts()
     assert np.array_equal(ng, no)
-    assert no.sum() > 0 or name == "box_ref"
+    assert no.sum() > 0 or name.startswith("box")  # the box lattice starts with no contact
     for i in np.nonzero(no)[0]:
         k = min(int(no[i]), 32)
         assert np.array_equal(lg[i, :k], lo[i, :k]), f"particle {i}"
