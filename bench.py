@@ -98,8 +98,12 @@ def main():
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
     ap.add_argument("--revs", type=float, default=1.0, help="mill workload: revolutions to run")
-    ap.add_argument("--workload", choices=("silo", "poly", "mill"), default="silo",
-                    help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box)")
+    ap.add_argument("--workload", choices=("silo", "poly", "mill", "batch"), default="silo",
+                    help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box); "
+                         "batch: 512 independent boxes advanced together vs one after another")
+    ap.add_argument("--instances", type=int, default=512, help="batch workload: number of boxes")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="silo workload: skip the batched-vs-sequential leg reported beside the headline")
     ap.add_argument("--load-state", default=None,
                     help="start from a state written by --save-state instead of settling "
                          "(tangential springs restart at zero)")
@@ -114,6 +118,8 @@ def main():
         return run_poly(args)
     if args.workload == "mill":
         return run_mill(args)
+    if args.workload == "batch":
+        return run_batch(args)
 
     import numpy as np
     import bdem
@@ -221,6 +227,10 @@ def main():
         log(f"[bench] CPU baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
     dev.close()
 
+    batched = None
+    if not args.no_batch:
+        batched = batch_leg(count=args.instances)
+
     out = {
         "metric": "particle-steps/s at 1M spheres (plain step, silo discharge)",
         "value": round(value, 1),
@@ -243,6 +253,88 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
+        "batched": batched,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def batch_leg(count=512, steps=200, warmup=4, tune_steps=5, log_every=64):
+    """Batched independent boxes (512 by default: counts log-uniform in
+    [1024, 65536], r U[0.2, 0.6] cm, packing 0.3, per-box kn, e, mu; LSD +
+    Coulomb + history; dt 1e-5 s) advanced together, against the same boxes
+    run one after another in plain contexts.  The autotuner picks the launch
+    shape and cell factor first; both sides then use the chosen cells.
+    Returns the batched rate, the sequential rate and their ratio."""
+    import numpy as np
+    import bdem
+    from bdem import scenarios as S
+
+    t0 = time.time()
+    decks = S.batch_box_decks(count=count, seed=4)
+    ntot = sum(d["sim"].Num_Particles for d in decks)
+    log(f"[batch] {count} boxes, {ntot} spheres generated in {time.time() - t0:.1f}s")
+    batch = bdem.Batch()
+    for d in decks:
+        dev = make_device(bdem, d)
+        batch.add(dev)
+        dev.close()
+    batch.finalize()
+    best, tuned_ms = batch.autotune(steps=tune_steps)
+    log(f"[batch] autotuner: {best} ({tuned_ms:.3f} ms/step in its trial)")
+    batch.step(warmup)
+    batch.sync()
+    ta = time.perf_counter()
+    batch.step(steps)
+    batch.sync()
+    t_batch = time.perf_counter() - ta
+    status = batch.status()
+    log(f"[batch] batched: {steps} steps in {t_batch:.3f} s ({t_batch * 1e3 / steps:.3f} ms/step), status {status}")
+    sims = [batch.instance_sim(k) for k in range(count)]
+    batch.close()
+    t_seq = 0.0
+    for k, d in enumerate(decks):
+        dd = dict(d)
+        dd["sim"] = sims[k]
+        dev = make_device(bdem, dd)
+        dev.step(warmup)
+        dev.sync()
+        ta = time.perf_counter()
+        dev.step(steps)
+        dev.sync()
+        t_seq += time.perf_counter() - ta
+        dev.close()
+        if (k + 1) % log_every == 0:
+            log(f"[batch] sequential plain runs: {k + 1}/{count} done ({t_seq:.2f} s of stepping)")
+    rate_b = ntot * steps / t_batch
+    rate_s = ntot * steps / t_seq
+    log(f"[batch] sequential: {t_seq:.3f} s; speedup {t_seq / t_batch:.2f}x")
+    return dict(workload=f"{count} independent box fills, {ntot} spheres (counts log-uniform in [1024, 65536]), "
+                         "LSD + Coulomb + history, per-box kn/e/mu, dt 1e-5 s",
+                instances=count, particles=ntot, steps=steps, tuning=best,
+                batched_particle_steps_per_s=round(rate_b, 1), batched_ms_per_step=round(t_batch * 1e3 / steps, 5),
+                sequential_particle_steps_per_s=round(rate_s, 1),
+                sequential_ms_per_step=round(t_seq * 1e3 / steps, 5),
+                speedup_vs_sequential=round(t_seq / t_batch, 3))
+
+
+def run_batch(args):
+    steps = args.steps if args.steps != 1000 else 200
+    b = batch_leg(count=args.instances, steps=steps)
+    out = {
+        "metric": "particle-steps/s, batched independent simulations (vs sequential plain runs)",
+        "value": b["batched_particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 4,
+        "ms_per_step": b["batched_ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": b["workload"], "batched": b},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
