@@ -170,6 +170,13 @@ double bo_time_steps(void* h, int nsteps) {
 
 int bo_set_active_world_objects(void* h, int n) { WITH(h, o.num_wobj_active = n); return 0; }
 int bo_set_drum_rotation(void* h, int on) { WITH(h, o.cylinder_rotation = on != 0); return 0; }
+int bo_set_pair_override(void* h, double kn, double cn, double fk, double rr) {
+  WITH(h, {
+    o.pp_override = true;
+    o.ov_kn = kn; o.ov_cn = cn; o.ov_fk = fk; o.ov_rr = rr;
+  });
+  return 0;
+}
 /* Per-particle lifter contact type kept between steps (Contact_Info::cont_type),
  * in the current (sorted) slot order.  The setter exists for tests that start
  * a particle with a remembered contact. */

@@ -104,6 +104,12 @@ class Oracle {
   bool has_ext = false;
   int num_wobj_active = 0;           /* Num_WorldObjects, decremented by Hatch */
   bool cylinder_rotation = false;    /* is_cylinder_rotation (drum_rotation) */
+  /* Exact (double) pair constants of type pair (0, 0) and rolling
+   * resistance, replacing the float table values when set: finite
+   * differences of the FP64 restatement in these parameters need
+   * perturbations finer than a float's spacing. */
+  bool pp_override = false;
+  double ov_kn = 0, ov_cn = 0, ov_fk = 0, ov_rr = 0;
   uint32_t C = 0;                    /* number of grid cells */
   LawPair<R> law[8][8]{};
   LawPair<R> wlaw[8]{};
@@ -398,9 +404,10 @@ class Oracle {
     const uint32_t ta = ptype[a], tb = ptype[b];
     V3<R> RVel = vel[a] - vel[b];
     const bdem_PP_Parms& pp = pobj[ta].PP[tb];
+    const bool ov = pp_override && ta == 0 && tb == 0;
     V3<R> Vnormal = dot(RVel, N3) * N3;
-    R Fnormal = R(pp.Kn) * dis;
-    V3<R> Fdiss = R(pp.Cn) * Vnormal;
+    R Fnormal = (ov ? R(ov_kn) : R(pp.Kn)) * dis;
+    V3<R> Fdiss = (ov ? R(ov_cn) : R(pp.Cn)) * Vnormal;
     V3<R> Ft = Fnormal * N3 - Fdiss;
     V3<R> V_CPA = vel[a] + cross(avel[a], RA);
     V3<R> V_CPB = vel[b] + cross(avel[b], RB);
@@ -416,7 +423,7 @@ class Oracle {
       R Velf = dot(RVel, RVel);
       Velf = Velf > R(0) ? rsqrt_(Velf) : R(1.0f);
       R tan_f = (((R(0.50f) * (mass[a] + mass[b])) / R(sim.InitalDelta_t)) * R(0.050f)) *
-                R(pp.Fric_kinetic) * (mag / Velf);
+                (ov ? R(ov_fk) : R(pp.Fric_kinetic)) * (mag / Velf);
       Tang = (R(-1.0f) * tan_f) * (Vt / mag);
     }
     if (sim.EnergyCalc) {
@@ -1790,7 +1797,7 @@ class Oracle {
         if (dot(fl, fl) > R(0)) {
           V3<R> aa = poly() ? angular_acc(i, fl) : inv_inertia[i] * fl;
           avel[i] = avel[i] + aa * dt;
-          avel[i] = avel[i] - avel[i] * R(sim.Roll_Res);
+          avel[i] = avel[i] - avel[i] * (pp_override ? R(ov_rr) : R(sim.Roll_Res));
         }
         if (poly() || sim.sphere_orient) {
           V3<R> om = avel[i];

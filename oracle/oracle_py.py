@@ -43,6 +43,7 @@ def load():
     lib.bo_time_steps.argtypes = [P, C.c_int]
     lib.bo_set_active_world_objects.argtypes = [P, C.c_int]
     lib.bo_set_drum_rotation.argtypes = [P, C.c_int]
+    lib.bo_set_pair_override.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double]
     lib.bo_set_dobj.argtypes = [P, C.c_void_p, C.c_int]
     lib.bo_lifter_types.argtypes = [P, ip]
     lib.bo_set_lifter_types.argtypes = [P, ip]
@@ -206,6 +207,10 @@ class Oracle:
     def set_lifter_types(self, t):
         t = np.ascontiguousarray(t, np.int32)
         self.lib.bo_set_lifter_types(self.h, _p(t, C.c_int))
+
+    def set_pair_override(self, kn, cn, fk, rr):
+        """Exact pair constants of type pair (0, 0) and rolling resistance."""
+        self.lib.bo_set_pair_override(self.h, float(kn), float(cn), float(fk), float(rr))
 
     def set_drum_rotation(self, on):
         self.lib.bo_set_drum_rotation(self.h, 1 if on else 0)

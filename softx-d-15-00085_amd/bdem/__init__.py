@@ -5,5 +5,5 @@ lib        the C ABI and a KDevice-style Device wrapper
 scenarios  seeded synthetic decks (box fill, silo, dense random packs)
 deck       the reference project decks and state files
 """
-from . import types, lib, scenarios, deck  # noqa: F401
+from . import types, lib, scenarios, deck, adjoint  # noqa: F401
 from .lib import Batch, BdemError, Device, load  # noqa: F401

@@ -590,6 +590,56 @@ def mill_deck(n=3000, seed=3, R=15.0, L=10.0, n_lifters=8, lifter_w=1.5, lifter_
     return deck
 
 
+def drum_deck(n=20000, seed=5, R=15.0, L=10.0, rpm=30.0, rmin=0.15, rmax=0.25, density=2500.0, kn=1e5,
+              e=0.5, mu=0.4, roll_res=0.001, dt=1e-5, vmax=0.0, spacing=1.05):
+    """Smooth rotating drum (radius R, length L, axis z, centre (R, R), end
+    plates, no lifters) at `rpm`, reference pair law with per-particle radii
+    U[rmin, rmax] on a jittered lattice in the lower part of the drum: the
+    calibration workload.  The drum's wall constants come from (kn, e, mu)
+    and stay fixed; the particle pair constants and rolling resistance are
+    the calibrated parameters.  The lattice pitch is spacing x 2 rmax
+    (below 1: neighbours start in contact)."""
+    f = np.float32
+    cell = 2.0 * rmax
+    d = 2.0 * rmax * spacing
+    pts = []
+    inner = R - rmax * 1.5
+    xs = np.arange(R - inner + rmax, R + inner, d)
+    zs = np.arange(rmax * 1.2, L - rmax * 1.2, d)
+    for y in np.arange(R - inner + rmax, R + inner, d):
+        for x in xs:
+            if (x - R) ** 2 + (y - R) ** 2 > (inner - rmax) ** 2:
+                continue
+            for z in zs:
+                pts.append((x, y, z))
+                if len(pts) == n:
+                    break
+            if len(pts) == n:
+                break
+        if len(pts) == n:
+            break
+    if len(pts) < n:
+        raise ValueError(f"drum holds only {len(pts)} of {n} spheres on this lattice")
+    pos = np.asarray(pts, np.float64) + uniform(seed, STREAM_JITTER, 3 * n, -0.01, 0.01).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3) if vmax > 0 else np.zeros((n, 3))
+    rad = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+    mass = ((4.0 / 3.0) * 2.0 * math.asin(1.0) * rad.astype(np.float64) ** 3 * density * 1e-6).astype(np.float32)
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (2 * R + 1, 2 * R + 1, L + 1), cell,
+                   rotation=True, roll_res=roll_res, sim_type=1)
+    sim.isMillSim = True
+    sim.Rotating_WObject = False
+    omega = rpm * 2.0 * math.pi / 60.0
+    sim.MillRotVel = float(f(-omega))
+    sim.cent = T.f3(R, R, 0.0)
+    pobj = [sphere_pobj(0.5 * (rmin + rmax), density, e, kn, mu)]
+    ext = T.ExtParams()
+    ext.per_particle_radius = 1
+    drum = cylinder_wobj(R, L, (0.0, 0.0, 1.0), (0.0, 0.0, 0.0), top=True, bot=True)
+    deck = _deck(sim, [drum], pobj, pos, vel, ext=ext, radius=rad, mass=mass, name=f"drum{n}")
+    deck["theta"] = (mu, e, kn, roll_res)
+    return deck
+
+
 def place_at_lifters(deck, seed=5, overlap=0.05):
     """Move the first spheres of a mill deck onto every lifter: one on each
     face and one at each top edge, overlapping by `overlap`, at seeded

@@ -496,6 +496,12 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
 
 }  // namespace
 
+namespace bdem {
+/* The parameter block rebuilt from the context's tables and uploaded (for
+ * the adjoint runs, which change the pair constants between runs). */
+int ctx_upload_params(bdem_ctx* c) { return upload_params(c); }
+}  // namespace bdem
+
 extern "C" {
 
 int bdem_create(bdem_ctx** out, int device) {

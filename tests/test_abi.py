@@ -13,7 +13,7 @@ from bdem import scenarios as S
 from bdem import types as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = [os.path.join(ROOT, "include", h) for h in ("bdem_abi.h", "bdem_deck.h", "bdem_batch.h")]
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("bdem_abi.h", "bdem_deck.h", "bdem_batch.h", "bdem_adjoint.h")]
 
 
 def header_functions():
@@ -39,7 +39,8 @@ def test_library_exports_every_declared_entry_point():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     # the Python binding declares exactly the header's entry points
-    assert sorted(bdem.lib.EXPORTED) == names
+    from bdem import adjoint
+    assert sorted(bdem.lib.EXPORTED + adjoint.EXPORTED) == names
 
 
 def test_compat_shim_exports_reference_names():
