@@ -168,6 +168,15 @@ int bdem_get_poly_vertices(bdem_ctx* ctx, int max_vertex, bdem_float3* verts, in
  * bounding radius), NaN for an empty bin.  A centre's bin is
  * (int)((u - lo) * inv_w) with inv_w = (float)bins / (hi - lo), in float. */
 int bdem_get_surface_profile(bdem_ctx* ctx, int axis, float lo, float hi, int bins, float* top);
+/* bdem_get_surface_profile without any allocation, for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments and results; its scratch
+ * (the bin keys and heights, 2 x 4096 words) is a file-scope device array
+ * of the library, one per GPU, whose address is taken on the first call on
+ * each GPU.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (from several threads or contexts) race.  bins above 4096 are
+ * rejected (BDEM_EINVAL). */
+int bdem_get_surface_profile_noalloc(bdem_ctx* ctx, int axis, float lo, float hi, int bins, float* top);
 /* Host only: least-squares line h = intercept + slope * u through the
  * non-empty (non-NaN) bins of a profile, u at the bin centres
  * lo + (k + 0.5)(hi - lo)/bins, in double; angle_deg = atan|slope| in

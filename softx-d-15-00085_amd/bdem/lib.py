@@ -56,6 +56,7 @@ _PROTOS = {
     "bdem_get_particle_stats": (C.c_int, [_P, C.c_float, _f3p, _f3p, C.POINTER(T.ParticleStats)]),
     "bdem_get_kill_count": (C.c_int, [_P, _ip]),
     "bdem_get_surface_profile": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
+    "bdem_get_surface_profile_noalloc": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
     "bdem_get_poly_vertices": (C.c_int, [_P, C.c_int, _f3p, _ip, _ip]),
     "bdem_surface_angle": (C.c_int, [_fp, C.c_int, C.c_float, C.c_float, _dp, _dp, _dp]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
@@ -327,6 +328,13 @@ class Device:
         top = np.zeros(int(bins), np.float32)
         self._check(self.lib.bdem_get_surface_profile(self.h, int(axis), float(lo), float(hi), int(bins),
                                                       _ptr(top, C.c_float)))
+        return top
+
+    def surface_profile_noalloc(self, lo, hi, bins, axis=0):
+        """surface_profile through the allocation-free entry point."""
+        top = np.zeros(int(bins), np.float32)
+        self._check(self.lib.bdem_get_surface_profile_noalloc(self.h, int(axis), float(lo), float(hi), int(bins),
+                                                              _ptr(top, C.c_float)))
         return top
 
     def status(self):

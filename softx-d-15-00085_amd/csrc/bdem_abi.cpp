@@ -930,6 +930,25 @@ int bdem_get_surface_profile(bdem_ctx* c, int axis, float lo, float hi, int bins
   return BDEM_OK;
 }
 
+int bdem_get_surface_profile_noalloc(bdem_ctx* c, int axis, float lo, float hi, int bins, float* top) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!top) return fail(c, BDEM_EINVAL, "null argument");
+  if (axis != 0 && axis != 2) return fail(c, BDEM_EINVAL, "profile axis must be 0 (x) or 2 (z)");
+  if (bins < 1 || bins > PROFILE_MAX_BINS) return fail(c, BDEM_EINVAL, "bins must be 1..4096");
+  if (!(hi > lo)) return fail(c, BDEM_EINVAL, "profile range must have hi > lo");
+  HIPCHK(c, hipSetDevice(c->device));
+  uint32_t* scr = nullptr;
+  HIPCHK(c, noalloc_scratch(c->device, &scr));
+  const float inv_w = (float)bins / (hi - lo);
+  launch_profile_noalloc(c->b, axis, lo, inv_w, bins, scr, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(top, scr + PROFILE_MAX_BINS, sizeof(float) * (size_t)bins, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BDEM_OK;
+}
+
 int bdem_surface_angle(const float* top, int bins, float lo, float hi, double* slope, double* intercept,
                        double* angle_deg) {
   if (!top || bins < 2 || !(hi > lo)) return BDEM_EINVAL;

@@ -193,6 +193,15 @@ void launch_stats(const StepBuffers& b, float min_y, float3 lo, float3 hi, bool 
  * words, bins <= PROFILE_MAX_BINS) must be zeroed first. */
 constexpr int PROFILE_MAX_BINS = 4096;
 void launch_profile(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* keys, hipStream_t s);
+/* The allocation-free profile's scratch: PROFILE_MAX_BINS key words, then
+ * PROFILE_MAX_BINS finished heights (float), in a file-scope device array
+ * of bdem_stats.hip.  noalloc_scratch returns its address on `device`
+ * (which must be current), taken once per device. */
+constexpr int NOALLOC_SCRATCH_WORDS = 2 * PROFILE_MAX_BINS;
+hipError_t noalloc_scratch(int device, uint32_t** out);
+/* Clear the keys, bin, and write the heights to scr + PROFILE_MAX_BINS. */
+void launch_profile_noalloc(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* scr,
+                            hipStream_t s);
 
 /* Debug stages on the sorted scratch state. */
 void launch_contact_list(const DevParams* dP, const StepBuffers& b, uint32_t* list32, uint32_t* nn,

@@ -10,6 +10,7 @@
  * atomic per wave.  Sums are in double, so the result does not depend on
  * the order the waves finish in except in the last bits.
  */
+#include <mutex>
 #include "bdem_kernels.hpp"
 
 namespace bdem {
@@ -113,6 +114,56 @@ __global__ __launch_bounds__(256) void k_profile(int n, const float4* __restrict
 void launch_profile(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* keys, hipStream_t s) {
   const int blocks = b.n <= 0 ? 1 : min((b.n + 255) / 256, 1024);
   hipLaunchKernelGGL(k_profile, dim3(blocks), dim3(256), 0, s, b.n, b.pos_cur, b.omg_cur, axis, lo, inv_w, bins, keys);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Allocation-free profile: the bin keys and the finished heights live in a  */
+/* file-scope device array, one copy per GPU, its address taken once per     */
+/* device with hipGetSymbolAddress.  Every call on a GPU shares that GPU's   */
+/* array (calls that overlap on one GPU race); each call clears the keys it  */
+/* uses with a kernel first.                                                 */
+/* ------------------------------------------------------------------------ */
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCRATCH_WORDS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch[MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, uint32_t** out) {
+  *out = nullptr;
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;   /* the caller makes `device` current */
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
+
+/* keys -> heights (NaN for an empty bin), the inverse of order_key */
+__global__ __launch_bounds__(256) void k_profile_finish(const uint32_t* __restrict__ keys, int bins,
+                                                        float* __restrict__ top) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= bins) return;
+  const uint32_t u = keys[k];
+  top[k] = u == 0u ? __uint_as_float(0x7FC00000u) : __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+void launch_profile_noalloc(const StepBuffers& b, int axis, float lo, float inv_w, int bins, uint32_t* scr,
+                            hipStream_t s) {
+  launch_fill_u32(scr, (size_t)bins, 0u, s);
+  launch_profile(b, axis, lo, inv_w, bins, scr, s);
+  hipLaunchKernelGGL(k_profile_finish, dim3((bins + 255) / 256), dim3(256), 0, s, scr, bins,
+                     reinterpret_cast<float*>(scr + PROFILE_MAX_BINS));
 }
 
 }  // namespace bdem
