@@ -98,10 +98,12 @@ def main():
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
     ap.add_argument("--revs", type=float, default=1.0, help="mill workload: revolutions to run")
-    ap.add_argument("--workload", choices=("silo", "poly", "mill", "batch"), default="silo",
+    ap.add_argument("--workload", choices=("silo", "poly", "mill", "batch", "adjoint"), default="silo",
                     help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box); "
                          "batch: 512 independent boxes advanced together vs one after another")
     ap.add_argument("--instances", type=int, default=512, help="batch workload: number of boxes")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="silo workload: skip the adjoint-vs-forward leg reported beside the headline")
     ap.add_argument("--no-batch", action="store_true",
                     help="silo workload: skip the batched-vs-sequential leg reported beside the headline")
     ap.add_argument("--load-state", default=None,
@@ -120,6 +122,8 @@ def main():
         return run_mill(args)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "adjoint":
+        return run_adjoint(args)
 
     import numpy as np
     import bdem
@@ -230,6 +234,9 @@ def main():
     batched = None
     if not args.no_batch:
         batched = batch_leg(count=args.instances)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg()
 
     out = {
         "metric": "particle-steps/s at 1M spheres (plain step, silo discharge)",
@@ -254,6 +261,7 @@ def main():
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 
@@ -315,6 +323,93 @@ def batch_leg(count=512, steps=200, warmup=4, tune_steps=5, log_every=64):
                 sequential_particle_steps_per_s=round(rate_s, 1),
                 sequential_ms_per_step=round(t_seq * 1e3 / steps, 5),
                 speedup_vs_sequential=round(t_seq / t_batch, 3))
+
+
+def adjoint_leg(n=20000, nsteps=2000, window=100, every=100, settle=10000, iters=1):
+    """Differentiable drum calibration: n spheres (radii U[0.15, 0.25] cm)
+    in a smooth drum (R 15 cm, L 10 cm, 30 rpm), reference pair law, settled
+    (untimed), then nsteps steps at dt 1e-5 s with checkpoints every `every`
+    steps.  The loss is the MSE of the windowed bed centre of mass and
+    inclination against a target run with hidden parameters; `iters`
+    evaluations of loss and gradient (with normalised steps between them).
+    Reports the forward and adjoint times against plain steps.  Over the
+    full horizon the exact gradient of a resting bed is huge: the reference
+    friction law's tangential force has a velocity Jacobian of order
+    mu (m/dt) / |v_rel|, which amplifies the cotangent at every step."""
+    import numpy as np
+    import bdem
+    from bdem import adjoint as ADJ
+    from bdem import scenarios as S
+
+    t0 = time.time()
+    deck = S.drum_deck(n=n, seed=5, R=15.0, L=10.0, rpm=30.0, kn=1e5, e=0.5, mu=0.4, roll_res=0.002)
+    dev = make_device(bdem, deck)
+    dev.drum_rotation(True)
+    dev.step(settle)
+    st = dev.get_state()
+    order = np.argsort(st["pid"])
+    dev.close()
+    d = dict(deck)
+    d["pos"], d["vel"], d["avel"] = st["pos"][order], st["vel"][order], st["avel"][order]
+    log(f"[adjoint] drum n={n} settled {settle} steps ({time.time() - t0:.1f}s)")
+    # plain steps on the same state and parameters, for the cost ratio
+    plain = make_device(bdem, d)
+    plain.drum_rotation(True)
+    plain.step(2)
+    plain.sync()
+    ta = time.perf_counter()
+    plain.step(nsteps)
+    plain.sync()
+    plain_ms = (time.perf_counter() - ta) * 1e3
+    plain.close()
+    dev = make_device(bdem, d)
+    dev.drum_rotation(True)
+    run = ADJ.AdjointRun(dev, nsteps, window, checkpoint_every=every)
+    hidden = np.array([0.55, 0.35, 1.6e5, 0.01])
+    theta = np.array([0.4, 0.5, 1e5, 0.002])
+    target = run.forward(hidden)
+    hist = []
+    fwd_ms = bwd_ms = None
+    for it in range(iters):
+        obs = run.forward(theta)
+        r = obs - target
+        loss = float(np.mean(r * r))
+        g = run.backward(2.0 * r / len(r))
+        fwd_ms, bwd_ms = run.timing()
+        hist.append(dict(iter=it, loss=loss, theta=[float(x) for x in theta], grad=[float(x) for x in g]))
+        log(f"[adjoint] iter {it}: loss {loss:.4e} theta {theta} grad {g} (forward {fwd_ms:.1f} ms, "
+            f"adjoint {bwd_ms:.1f} ms)")
+        # normalised gradient step: each parameter moves by at most 5 % of itself
+        step = -0.05 * theta * np.sign(g) * np.minimum(1.0, np.abs(g * theta) / max(np.max(np.abs(g * theta)), 1e-300))
+        theta = np.clip(theta + step, [0.05, 0.31, 1e4, 0.0], [1.0, 0.89, 1e6, 0.2])
+    run.close()
+    dev.close()
+    return dict(workload=f"drum calibration: {n} spheres r U[0.15, 0.25] cm, R 15 cm, L 10 cm, 30 rpm, "
+                         f"{nsteps} steps at dt 1e-5 s, checkpoints every {every}, windows of {window}",
+                plain_ms=round(plain_ms, 3), forward_ms=round(fwd_ms, 3), adjoint_ms=round(bwd_ms, 3),
+                adjoint_over_forward=round(bwd_ms / fwd_ms, 3), adjoint_over_plain=round(bwd_ms / plain_ms, 3),
+                hidden_theta=[float(x) for x in hidden], calibration=hist)
+
+
+def run_adjoint(args):
+    a = adjoint_leg()
+    out = {
+        "metric": "adjoint step cost vs forward, differentiable drum calibration",
+        "value": a["adjoint_over_forward"],
+        "unit": "x forward",
+        "n_gpus": 1,
+        "steps": 2000,
+        "warmup": 0,
+        "ms_per_step": round(a["adjoint_ms"] / 2000, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": a["workload"], "adjoint": a},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_batch(args):
