@@ -409,29 +409,57 @@ __global__ __launch_bounds__(SORT_THREADS) void k_fs_hist(SortArgs a, int par, c
   for (int d = threadIdx.x; d < RADIX; d += SORT_THREADS) a.hist[d * ntiles + t] = h[d];
 }
 
-/* Exclusive scan of the m digit-major counters in one block: each thread
- * sums a contiguous chunk, the chunk sums are scanned in LDS, and each thread
- * rewrites its chunk as running offsets. */
+/* Exclusive scan of the m digit-major counters in one block, a tile of
+ * 4 x 1024 counters at a time: each thread loads four consecutive counters
+ * (one 16-byte load), the 1024 thread sums are scanned by wave shuffles and
+ * one LDS pass over the 16 wave totals, and the running total carries to
+ * the next tile. */
 constexpr int SCAN_THREADS = 1024;
 __global__ __launch_bounds__(SCAN_THREADS) void k_fs_scan(uint32_t* __restrict__ hist, int m) {
-  __shared__ uint32_t red[SCAN_THREADS];
-  const int chunk = (m + SCAN_THREADS - 1) / SCAN_THREADS;
-  const int c0 = min((int)threadIdx.x * chunk, m), c1 = min(c0 + chunk, m);
-  uint32_t s = 0u;
-  for (int i = c0; i < c1; ++i) s += hist[i];
-  red[threadIdx.x] = s;
+  __shared__ uint32_t wsum[SCAN_THREADS / 64];
+  __shared__ uint32_t carry_s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry_s = 0u;
   __syncthreads();
-  for (int o = 1; o < SCAN_THREADS; o <<= 1) {
-    const uint32_t y = threadIdx.x >= (unsigned)o ? red[threadIdx.x - o] : 0u;
+  for (int base = 0; base < m; base += 4 * SCAN_THREADS) {
+    const int i0 = base + 4 * (int)threadIdx.x;
+    uint32_t x[4];
+    if (i0 + 3 < m) {
+      const uint4 v = *reinterpret_cast<const uint4*>(hist + i0);   /* m is a multiple of 256 */
+      x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+    } else {
+      for (int k = 0; k < 4; ++k) x[k] = i0 + k < m ? hist[i0 + k] : 0u;
+    }
+    const uint32_t t = x[0] + x[1] + x[2] + x[3];
+    uint32_t incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[wave] = incl;
     __syncthreads();
-    red[threadIdx.x] += y;
+    uint32_t before = carry_s;
+    for (int w = 0; w < wave; ++w) before += wsum[w];
+    uint32_t run = before + incl - t;
+    if (i0 + 3 < m) {
+      uint4 o;
+      o.x = run; run += x[0];
+      o.y = run; run += x[1];
+      o.z = run; run += x[2];
+      o.w = run;
+      *reinterpret_cast<uint4*>(hist + i0) = o;
+    } else {
+      for (int k = 0; k < 4; ++k)
+        if (i0 + k < m) { hist[i0 + k] = run; run += x[k]; }
+    }
     __syncthreads();
-  }
-  uint32_t run = red[threadIdx.x] - s;
-  for (int i = c0; i < c1; ++i) {
-    const uint32_t x = hist[i];
-    hist[i] = run;
-    run += x;
+    if (threadIdx.x == SCAN_THREADS - 1) {
+      uint32_t tot = 0u;
+      for (int w = 0; w < SCAN_THREADS / 64; ++w) tot += wsum[w];
+      carry_s += tot;
+    }
+    __syncthreads();
   }
 }
 
