@@ -4,8 +4,9 @@
 Workload: silo discharge, 1,048,576 polydisperse spheres (r ~ U[2, 4] mm,
 rho 2600), Hertz-Mindlin (E 1e7 Pa, nu 0.3, e 0.5, mu 0.5) with tangential
 spring history, cylindrical silo R 25 cm over a 30-degree conical hopper with
-a 6 cm outlet, cell 2*rmax, dt 5e-6 s.  Setup (untimed): a seeded FCC fill,
-settled with the outlet closed at a larger time step, then the outlet opens
+a 6 cm outlet, cell 2*rmax, dt 5e-6 s.  Setup (untimed): the seeded random
+non-overlapping fill, settled with the outlet closed at a larger time step
+until the fastest particle moves below --quiet-vmax, then the outlet opens
 (Device_OpenHatch) and the time step returns to 5e-6.  W warm-up steps, then
 exactly K timed steps of the full step (sort, gather, cell table, fused
 contact/force/wall/integrate kernel), bracketed by stream synchronisation.
@@ -53,8 +54,13 @@ def step_total_bytes(n, k, ncells):
     return n * (320 + 8 * k + 64 * k) + 8 * ncells
 
 
-def cpu_baseline(deck, dev, settled_state, target_s=15.0):
-    """Time the FP32 CPU restatement (OpenMP) on the settled state."""
+def cpu_baseline(deck, dev, settled_state, protocol="bounded", run_s=3.0):
+    """The CPU restatement (OpenMP, all host threads) on the settled state,
+    compiled for float and for double.  protocol "full" follows the plan:
+    100 warm-up steps, then 3 runs of 200 timed steps, median.  "bounded"
+    (the default, a few seconds): 3 warm-up steps, then 3 runs of as many
+    steps as take about run_s seconds, median.  The step loop alone is timed
+    (steady_clock inside the restatement).  value is the FP32 rate."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     from oracle_py import Oracle
@@ -69,28 +75,49 @@ def cpu_baseline(deck, dev, settled_state, target_s=15.0):
     slot = np.clip(settled_state["pid"], 0, None)
     d["radius"] = deck["radius"][slot]
     d["mass"] = deck["mass"][slot]
-    ora = Oracle(d, bits=32)
-    ora.set_active_world_objects(len(deck["wobj"]) - 1)  # hatch open
-    t1 = ora.time_steps(1)
-    steps = max(1, min(20, int(target_s / max(t1, 1e-3))))
-    el = ora.time_steps(steps)
-    ora.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=deck["sim"].Num_Particles * steps / el, unit="particle-steps/s", cores=cores,
-                kind="port",
-                sample=f"{steps} steps (after 1 untimed) of the settled {deck['sim'].Num_Particles}-sphere "
-                       f"silo, hatch open, FP32 OpenMP restatement, tangential springs reset")
+    n = deck["sim"].Num_Particles
+    out = {}
+    for bits in (32, 64):
+        ora = Oracle(d, bits=bits)
+        ora.set_active_world_objects(len(deck["wobj"]) - 1)  # hatch open
+        if protocol == "full":
+            ora.time_steps(100)
+            steps = 200
+        else:
+            t1 = ora.time_steps(3) / 3.0
+            steps = max(1, min(200, int(run_s / max(t1, 1e-4))))
+        runs = [ora.time_steps(steps) for _ in range(3)]
+        ora.close()
+        el = float(np.median(runs))
+        out[bits] = dict(rate=n * steps / el, steps=steps, runs_s=[round(x, 4) for x in runs])
+        log(f"[bench] CPU FP{bits}: {out[bits]['rate']:.3e} particle-steps/s ({steps} steps x 3, median)")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    warm = "100 warm-up steps, 3 runs of 200 timed steps" if protocol == "full" else \
+        f"3 warm-up steps, 3 runs of {out[32]['steps']} (FP32) / {out[64]['steps']} (FP64) timed steps"
+    return dict(value=out[32]["rate"], unit="particle-steps/s", cores=threads, kind="port",
+                fp64_value=out[64]["rate"], nproc=os.cpu_count(), omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                protocol=protocol, runs_s={"fp32": out[32]["runs_s"], "fp64": out[64]["runs_s"]},
+                sample=f"{warm}, median, of the settled {n}-sphere silo with the hatch open (tangential springs "
+                       f"start at zero); FP32 and FP64 OpenMP restatement on {threads} threads")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--settle", type=int, default=24000, help="untimed settle steps, outlet closed")
+    ap.add_argument("--settle", type=int, default=300000,
+                    help="untimed settle steps, outlet closed: the most to run (it stops once quiet)")
+    ap.add_argument("--quiet-vmax", type=float, default=10.0,
+                    help="settle until the fastest particle moves below this speed (cm/s)")
+    ap.add_argument("--fill", choices=("random", "fcc"), default="random",
+                    help="initial state: seeded random non-overlapping fill (the configuration) or an FCC lattice")
     ap.add_argument("--settle-dt", type=float, default=2e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-protocol", choices=("bounded", "full"), default="bounded",
+                    help="CPU baseline: bounded (a few seconds per precision) or full (100 warm-up steps, "
+                         "3 x 200 timed steps, median)")
     ap.add_argument("--skin", type=float, default=None,
                     help="skin masks of the step's stencil scan in cm (default: off, every candidate tested)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -130,7 +157,7 @@ def main():
     from bdem import scenarios as S
 
     t0 = time.time()
-    deck = S.silo_deck(args.n)
+    deck = S.silo_deck(args.n, fill=args.fill)
     sim = deck["sim"]
     dt = sim.InitalDelta_t
     dev = make_device(bdem, deck)
@@ -150,13 +177,17 @@ def main():
         sim.InitalDelta_t = args.settle_dt
         dev.update_sim_info(sim)
         done = 0
-        chunk = 2000
+        chunk = 5000
         while done < args.settle:
             m = min(chunk, args.settle - done)
             dev.step(m)
-            dev.sync()
             done += m
-            log(f"[bench] settle {done}/{args.settle} steps ({time.time() - t0:.1f}s)")
+            s_ = dev.particle_stats()
+            log(f"[bench] settle {done} steps ({time.time() - t0:.1f}s): max |v| {s_['max_speed'] * 100:.1f} cm/s, "
+                f"mean {s_['mean_speed'] * 100:.2f} cm/s")
+            if s_["max_speed"] * 100.0 < args.quiet_vmax:
+                break
+        args.settle = done
     st = dev.get_state()
     if args.save_state:
         slot = np.clip(st["pid"], 0, None)
@@ -203,10 +234,17 @@ def main():
         for line in open(tfile):
             if line.startswith("traffic_bytes_per_launch:"):
                 traffic = float(line.split(":")[1])
+    valu_frac = None
+    vfile = os.path.join(ROOT, "profiles", "k_step_silo1m_valu.txt")
+    if os.path.exists(vfile):
+        for line in open(vfile):
+            if line.startswith("valu_frac:"):
+                valu_frac = float(line.split(":")[1])
     tot_b = step_total_bytes(n_live, k, ncells)
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_step<HERTZ_MINDLIN> (fused contact+force+wall+integrate)",
+                measured_bound="valu issue", valu_frac=valu_frac,
                 algorithmic_bytes_per_launch=int(kb), avg_launch_ms=round(step_ms, 5))
     step_roof = dict(bytes_per_step=int(tot_b),
                      achieved=round(tot_b / (ms_per_step * 1e-3) / 1e9, 1),
@@ -227,8 +265,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (FP32 restatement)...")
-        cpu = cpu_baseline(deck, dev, dev.get_state())
-        log(f"[bench] CPU baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(deck, dev, dev.get_state(), protocol=args.cpu_protocol)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} (FP32) / {cpu['fp64_value']:.3e} (FP64) particle-steps/s "
+            f"on {cpu['cores']} threads")
     dev.close()
 
     batched = None

@@ -227,8 +227,14 @@ int bdem_zero_force(bdem_ctx* ctx);
 /* Device_DEM_UpdateSim, Device/DeviceInterface.h:95 (defined
  * Device/DeviceInterface.cu:1825-2513): one step, asynchronous. */
 int bdem_update_sim(bdem_ctx* ctx);
-/* nsteps consecutive bdem_update_sim calls. */
+/* nsteps consecutive bdem_update_sim calls.  Steady-state pairs of steps
+ * (incremental sort, no discharge check, skin masks, stage timing or
+ * recording among them; spheres) are replayed from a HIP graph captured
+ * once per parity: the same kernels with the same arguments, fewer launch
+ * costs.  No reference counterpart as an entry point. */
 int bdem_step(bdem_ctx* ctx, int nsteps);
+/* Turn the graph replay of bdem_step on (default) or off. */
+int bdem_set_graphs(bdem_ctx* ctx, int on);
 /* Wait for the context's stream. */
 int bdem_sync(bdem_ctx* ctx);
 

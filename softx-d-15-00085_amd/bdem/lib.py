@@ -74,6 +74,7 @@ _PROTOS = {
     "bdem_zero_force": (C.c_int, [_P]),
     "bdem_update_sim": (C.c_int, [_P]),
     "bdem_step": (C.c_int, [_P, C.c_int]),
+    "bdem_set_graphs": (C.c_int, [_P, C.c_int]),
     "bdem_sync": (C.c_int, [_P]),
     "bdem_timing_begin": (C.c_int, [_P, C.c_int]),
     "bdem_timing_end": (C.c_int, [_P, C.POINTER(C.c_double), _ip]),
@@ -402,6 +403,10 @@ class Device:
 
     def sync(self):
         self._check(self.lib.bdem_sync(self.h))
+
+    def set_graphs(self, on):
+        """Replay steady-state step pairs from a captured HIP graph (default on)."""
+        self._check(self.lib.bdem_set_graphs(self.h, 1 if on else 0))
 
     def open_hatch(self):
         self._check(self.lib.bdem_open_hatch(self.h, None, 0))

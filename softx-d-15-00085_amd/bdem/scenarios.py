@@ -377,17 +377,29 @@ def silo_geometry():
 
 
 def silo_deck(n=1048576, seed=2, rmin=0.2, rmax=0.4, density=2600.0, E=1e5, nu=0.3, e=0.5,
-              mu=0.5, dt=5e-6, fill_spacing=None, law="hm"):
+              mu=0.5, dt=5e-6, fill_spacing=None, law="hm", fill="fcc", packing=0.25):
     """Silo discharge deck.  Cylinder R 25 cm (axis y) over a 30-degree conical
     hopper down to a 6 cm outlet closed by a hatch plane (the last world
-    object, so bdem_open_hatch removes it like Device_OpenHatch).  Initial
-    state: an FCC lattice of spacing 2*rmax + gap filling the hopper and the
-    cylinder bottom-up (a non-overlapping fill), radii U[rmin, rmax] from the
-    seed, zero velocity.  Hertz-Mindlin E in kg cm^-1 s^-2 (1e7 Pa = 1e5)."""
+    object, so bdem_open_hatch removes it like Device_OpenHatch).  Radii
+    U[rmin, rmax] from the seed, zero velocity.  Initial state (fill):
+    "fcc", an FCC lattice of spacing 2*rmax + gap filling the hopper and the
+    cylinder bottom-up; "random", the seeded random non-overlapping fill
+    (bdem_fill_random, gap 0.01 cm) of a column of the cylinder above the
+    hopper tall enough for a solid fraction `packing`.  Hertz-Mindlin E in
+    kg cm^-1 s^-2 (1e7 Pa = 1e5)."""
     g = silo_geometry()
-    d = 2.0 * rmax + 0.01 if fill_spacing is None else fill_spacing
-    pos = _fcc_fill(n, d, rmax, g)
     rad = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float32)
+    if fill == "random":
+        from . import lib as L
+        vol = float(np.sum((4.0 / 3.0) * math.pi * rad.astype(np.float64) ** 3))
+        H = vol / (packing * math.pi * (SILO_R - rmax) ** 2) * 1.02 + 2.0 * rmax
+        lo = (SILO_AXIS_XZ - SILO_R, g["y1"] + 0.01, SILO_AXIS_XZ - SILO_R)
+        hi = (SILO_AXIS_XZ + SILO_R, g["y1"] + 0.01 + H, SILO_AXIS_XZ + SILO_R)
+        pos = L.fill_random(rad, lo, hi, seed=seed * 64 + STREAM_POS, gap=0.01,
+                            cyl=(SILO_AXIS_XZ, SILO_AXIS_XZ, SILO_R - 0.01))
+    else:
+        d = 2.0 * rmax + 0.01 if fill_spacing is None else fill_spacing
+        pos = _fcc_fill(n, d, rmax, g)
     mass = np.array((4.0 / 3.0) * math.pi * rad.astype(np.float64) ** 3 * density * 1e-6, np.float32)
     top = float(pos[:, 1].max()) + 2.0 * rmax
     cell = 2.0 * rmax

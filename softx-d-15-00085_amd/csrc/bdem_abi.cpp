@@ -54,7 +54,16 @@ int dalloc(bdem_ctx* c, T** p, size_t count) {
   return BDEM_OK;
 }
 
+void drop_graphs(bdem_ctx* c) {
+  if ((c->graph[0] || c->graph[1]) && c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+}
+
 void free_all(bdem_ctx* c) {
+  drop_graphs(c);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   c->b = StepBuffers{};
@@ -186,6 +195,7 @@ int pull_dobj(bdem_ctx* c) {
 }
 
 int upload_params(bdem_ctx* c) {
+  drop_graphs(c);   /* captured launches may depend on what changes here */
   int rc0 = pull_dobj(c);
   if (rc0 != BDEM_OK) return rc0;
   build_params(c);
@@ -491,6 +501,61 @@ int prepare_sorted(bdem_ctx* c, hipEvent_t* ev = nullptr) {
   c->keys_valid = false;
   c->flags_valid = false;
   HIPCHK(c, hipGetLastError());
+  return BDEM_OK;
+}
+
+/* Two steps whose launches are the same every time: the incremental path
+ * of a sphere context with no discharge check, skin masks, stage timing or
+ * recording in them.  Every per-step alternation (sorted keys, append
+ * counters, spring buffers) has period two, so such a pair is captured
+ * once per parity and replayed. */
+bool graph_pair_ok(const bdem_ctx* c) {
+  if (!c->graphs_on || c->poly || c->tev.size() || c->dbg_list || c->dbg_nn) return false;
+  if (!(c->keys_valid && c->flags_valid && !c->full_next)) return false;
+  if (c->b.skin > 0.0f) return false;
+  if (c->sim.Kill_Particles) {
+    const int interval = (c->has_ext && c->ext.kill_check_interval > 0) ? c->ext.kill_check_interval : 1000;
+    if (c->stepc + 2 >= interval) return false;   /* a check falls in the pair */
+  }
+  return true;
+}
+
+int graph_pair(bdem_ctx* c) {
+  const int key = c->append_par;
+  hipGraphExec_t& g = c->graph[key];
+  if (g && (c->graph_hist[key] != c->hist_par || c->graph_sorted[key] != c->sorted_par)) {
+    (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  if (!g) {
+    const int hp = c->hist_par, sp = c->sorted_par, ap = c->append_par, st = c->stepc;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = bdem_update_sim(c);
+    if (rc == BDEM_OK) rc = bdem_update_sim(c);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc != BDEM_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, BDEM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    if (c->hist_par != hp || c->sorted_par != sp || c->append_par != ap) {
+      (void)hipGraphDestroy(graph);
+      return fail(c, BDEM_ESTATE, "internal: a step pair did not return to its parity");
+    }
+    const hipError_t ei = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      g = nullptr;
+      return fail(c, BDEM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    c->graph_hist[key] = hp;
+    c->graph_sorted[key] = sp;
+    c->stepc = st;   /* captured, not run: the replay below runs the pair */
+  }
+  HIPCHK(c, hipGraphLaunch(g, c->stream));
+  if (c->sim.Kill_Particles) c->stepc += 2;
+  if (c->lift_on && !c->dobj.empty()) c->dobj_dirty = true;
   return BDEM_OK;
 }
 
@@ -807,6 +872,7 @@ int bdem_get_velocity(bdem_ctx* c, bdem_float3* vel, bdem_float3* avel, int flag
 int bdem_record_wall_forces(bdem_ctx* c, int on) {
   int rc = require(c);
   if (rc) return rc;
+  drop_graphs(c);   /* the step's launch arguments change */
   c->record_wf = on != 0;
   return BDEM_OK;
 }
@@ -1094,6 +1160,7 @@ int bdem_set_lifter_rotation(bdem_ctx* c, double rad_per_step, float ax, float a
     return fail(c, BDEM_EINVAL, "rotation angle and axis point must be finite");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->lift_on = rad_per_step != 0.0;
   c->lift_c = std::cos(rad_per_step);
   c->lift_s = std::sin(rad_per_step);
@@ -1170,10 +1237,29 @@ int bdem_update_sim(bdem_ctx* c) {
 }
 
 int bdem_step(bdem_ctx* c, int nsteps) {
-  for (int s = 0; s < nsteps; ++s) {
-    int rc = bdem_update_sim(c);
-    if (rc) return rc;
+  int rc = require(c);
+  if (rc) return rc;
+  int s = 0;
+  while (s < nsteps) {
+    if (nsteps - s >= 2 && graph_pair_ok(c)) {
+      HIPCHK(c, hipSetDevice(c->device));
+      if ((rc = graph_pair(c)) != BDEM_OK) return rc;
+      s += 2;
+    } else {
+      if ((rc = bdem_update_sim(c)) != BDEM_OK) return rc;
+      s += 1;
+    }
   }
+  return BDEM_OK;
+}
+
+int bdem_set_graphs(bdem_ctx* c, int on) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->graphs_on = on != 0;
+  if (!c->graphs_on) drop_graphs(c);
   return BDEM_OK;
 }
 
@@ -1327,6 +1413,7 @@ int bdem_set_skin(bdem_ctx* c, float skin) {
   if (!(skin == skin)) return fail(c, BDEM_EINVAL, "skin must be a number");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);
   c->skin_req = skin;
   return apply_skin(c);
 }

@@ -62,6 +62,11 @@ struct bdem_ctx {
   uint32_t* dbg_list = nullptr;  /* debug: the contacts the step kernel used */
   uint32_t* dbg_nn = nullptr;
   int stepc = 0;
+  /* two steady-state steps captured as a HIP graph, one per parity of the
+   * per-step alternations (see bdem_step) */
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_hist[2] = {0, 0}, graph_sorted[2] = {0, 0};
+  bool graphs_on = true;
   /* stage timing: one event per stage boundary per recorded step */
   std::vector<hipEvent_t> tev;
   std::vector<char> t_lists;   /* per recorded step: the lists boundary was recorded */

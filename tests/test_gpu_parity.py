@@ -11,7 +11,7 @@ import pytest
 
 import bdem
 from bdem import scenarios as S
-from helpers import make_pair, rel_err, by_pid
+from helpers import make_device, make_pair, rel_err, by_pid
 from oracle_py import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -227,3 +227,34 @@ def test_mill_with_lifters_matches(law):
         assert err <= TOL, f"{key}: {err}"
     dev.close()
     ora.close()
+
+
+@pytest.mark.parametrize("law", ["ref", "hm"])
+def test_graph_replay_equals_eager_steps(law):
+    """bdem_step replays steady-state step pairs from a captured HIP graph;
+    the states equal eager steps bit for bit, across odd step counts,
+    a lifter-turning mill and a re-upload between runs (the graphs are
+    re-captured); replays alternate with the program's own synchronisation."""
+    deck = S.mill_deck(1500, seed=6, law=law)
+    print(f"[case] graph replay: mill n=1500 law={law}", flush=True)
+    a, b = make_device(deck), make_device(deck)
+    for dev in (a, b):
+        dev.drum_rotation(True)
+        dev.set_lifter_rotation(deck["rad_per_step"], deck["lifter_about"])
+    b.set_graphs(False)
+    for n in (1, 2, 7, 40, 3, 64):
+        a.step(n)
+        b.step(n)
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("pos", "vel", "avel", "pid"):
+            assert np.array_equal(sa[k], sb[k]), (n, k)
+    st = a.get_state()
+    for dev in (a, b):
+        dev.set_state(st["pos"], st["ptype"], st["pid"], vel=st["vel"], avel=st["avel"])
+    a.step(30)
+    b.step(30)
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("pos", "vel", "avel", "pid"):
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
