@@ -125,8 +125,9 @@ def main():
                          "256 MiB read + 256 MiB written) three times, to scale HBM byte counters")
     ap.add_argument("--save-state", default=None, help="write the settled state (npz) for profiling runs")
     ap.add_argument("--revs", type=float, default=1.0, help="mill workload: revolutions to run")
-    ap.add_argument("--workload", choices=("silo", "poly", "mill", "batch", "adjoint"), default="silo",
+    ap.add_argument("--workload", choices=("silo", "poly", "hullmill", "mill", "batch", "adjoint"), default="silo",
                     help="silo: the 1M-sphere headline; poly: 200k convex polyhedra (reference law, box); "
+                         "hullmill: 200k per-particle convex hulls in a lifter drum; "
                          "batch: 512 independent boxes advanced together vs one after another")
     ap.add_argument("--instances", type=int, default=512, help="batch workload: number of boxes")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -147,6 +148,8 @@ def main():
         return run_poly(args)
     if args.workload == "mill":
         return run_mill(args)
+    if args.workload == "hullmill":
+        return run_hullmill(args)
     if args.workload == "batch":
         return run_batch(args)
     if args.workload == "adjoint":
@@ -545,6 +548,124 @@ def run_poly(args):
         "config": {"workload": f"{n} convex hulls of 12 seeded points (20 faces, 30 edges), jittered "
                                f"lattice {spacing} cm in a closed {L:.1f} cm box, dt 1e-5 s",
                    "particles": n_live, "cells": ncells, "candidates_per_particle": round(k, 3),
+                   "stage_ms_per_step": {s_: round(v / nrec, 5) for s_, v in stages.items()}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_hullmill(args):
+    """Polyhedral ball mill: 200,000 per-particle convex hulls (8 to 16
+    seeded points on a sphere, bounding radius U[0.3, 0.6] cm, 2700 kg/m^3),
+    each its own shape-table row, in a 30 cm-radius, 20 cm-long drum with 8
+    lifters turning at 60 % of the critical speed; reference pair law kn
+    2e5, e 0.4, mu 0.6, dt 2e-6 s.  The hulls start in contact (random fill
+    of half-radius spheres) and spread for --settle untimed steps with the
+    drum turning, then --steps steps are timed."""
+    import numpy as np
+    import bdem
+    from bdem import scenarios as S
+
+    n = args.n if args.n != 1048576 else 200000
+    if args.steps == 5000:        # the silo's default: this configuration runs 20,000 steps
+        args.steps = 20000
+    if args.settle == 300000:     # the silo's default: settle this charge for 20,000 steps
+        args.settle = 20000
+    if args.settle_dt == 2e-5:    # the silo's default: hulls this small need the configuration's dt
+        args.settle_dt = 2e-6
+    t0 = time.time()
+    deck = S.hull_mill_deck(n)
+    dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"], dobj=deck["dobj"])
+    dev.set_poly_shapes(deck["shapes"])
+    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"], quat=deck["quat"])
+    dev.drum_rotation(True)
+    ncells = dev.num_cells()
+    log(f"[bench] hullmill n={n} cells={ncells} rpm {deck['rpm']:.2f} setup {time.time() - t0:.1f}s")
+    dt = deck["sim"].InitalDelta_t
+    if args.settle > 0:
+        sim = deck["sim"]
+        sim.InitalDelta_t = float(args.settle_dt)
+        dev.update_sim_info(sim)
+        dev.set_lifter_rotation(deck["rad_per_step"] * args.settle_dt / dt, deck["lifter_about"])
+        done = 0
+        while done < args.settle:
+            k = min(5000, args.settle - done)
+            dev.step(k)
+            done += k
+            st = dev.particle_stats()
+            log(f"[bench] settle {done} steps at dt {args.settle_dt:g}: mean speed {st['mean_speed']:.3f} m/s, "
+                f"max {st['max_speed']:.3f} m/s, status {dev.status()}")
+        sim.InitalDelta_t = float(dt)
+        dev.update_sim_info(sim)
+    dev.set_lifter_rotation(deck["rad_per_step"], deck["lifter_about"])
+    _, nn = dev.debug_contacts()
+    k = float(nn.mean())
+    dev.step(args.warmup)
+    dev.sync()
+    dev.timing_begin(args.steps)
+    ta = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        kk = min(5000, args.steps - done)
+        dev.step(kk)
+        done += kk
+        if done < args.steps:
+            dev.sync()
+            log(f"[bench] timed {done} of {args.steps} steps")
+    dev.sync()
+    tb = time.perf_counter()
+    stages, nrec = dev.timing_end()
+    elapsed = tb - ta
+    ms_per_step = elapsed * 1e3 / args.steps
+    st = dev.get_state()
+    n_live = int(np.count_nonzero(st["pid"] >= 0))
+    finite = bool(np.all(np.isfinite(st["pos"])))
+    status = dev.status()
+    queued, narrow = dev.debug_poly_queues()
+    log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step, candidates/particle {k:.2f}; stage "
+        "ms/step " + ", ".join(f"{s_}={v / nrec:.4f}" for s_, v in stages.items())
+        + f"; status {status}; finite {finite}; last step: {queued} listed pairs, {narrow} to the wave-per-pair kernel")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_py import Oracle
+        com = np.array([[s_.COM.x, s_.COM.y, s_.COM.z] for s_ in deck["shapes"]], np.float32)
+        live = np.maximum(st["pid"], 0)
+        state = dict(deck, pos=st["pos"] + com[live], vel=st["vel"], avel=st["avel"], quat=st["quat"],
+                     ptype=st["ptype"], pid=st["pid"])
+        ora = Oracle(state, bits=32)
+        ora.set_drum_rotation(True)
+        ora.run(1, deck["rad_per_step"], deck["lifter_about"])
+        steps = 3
+        tc = time.perf_counter()
+        ora.run(steps, deck["rad_per_step"], deck["lifter_about"])
+        el = time.perf_counter() - tc
+        ora.close()
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = dict(value=n * steps / el, unit="particle-steps/s", cores=cores, kind="port",
+                   sample=f"{steps} steps (after 1 untimed) from the state the timed run ended in, lifters "
+                          f"turning; FP32 OpenMP restatement")
+        log(f"[bench] CPU baseline {cpu['value']:.3e} particle-steps/s on {cores} threads")
+    dev.close()
+    out = {
+        "metric": "particle-steps/s, per-particle convex hulls in a lifter drum (reference law)",
+        "value": round(n_live * args.steps / elapsed, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n} per-particle convex hulls of 8-16 seeded points (bounding radius "
+                               "U[0.3, 0.6] cm), drum R 30 cm L 20 cm, 8 lifters 2 x 3 cm, "
+                               f"{deck['rpm']:.2f} rpm (60% critical), kn 2e5, e 0.4, mu 0.6, dt 2e-6 s",
+                   "settle": {"steps": args.settle, "dt": args.settle_dt},
+                   "particles": n_live, "cells": ncells, "candidates_per_particle": round(k, 3),
+                   "last_step_pairs": {"listed": queued, "wave_per_pair": narrow},
                    "stage_ms_per_step": {s_: round(v / nrec, 5) for s_, v in stages.items()}},
         "cpu_baseline": cpu,
     }

@@ -160,6 +160,22 @@ int bdem_get_particle_stats(bdem_ctx* ctx, float min_y, const bdem_float3* box_l
  * Input/FileIO.h:253-263).  BDEM_EINVAL for a sphere context or a
  * max_vertex below the largest vertex count. */
 int bdem_get_poly_vertices(bdem_ctx* ctx, int max_vertex, bdem_float3* verts, int* num_vertex, int* pid);
+/* Per-particle polyhedron shapes (no reference counterpart: the reference
+ * takes every polyhedron's geometry from its type's POBJ row,
+ * KSimulationObjects.h:272-316, read by ReadPObject_Poly,
+ * Input/KSimulationData.cpp:1180-1530).  After this call the particle with
+ * P_ID p has the geometry, bounding radius, mass and inverse inertia of
+ * shapes[p]; its type still selects the pair, wall and lifter constants
+ * (POBJ::PP and the surface constants).  count rows, each checked as a POBJ
+ * row is but with the larger bdem_poly_shape limits; vbound is recomputed.
+ * Every live P_ID of the current state and of later uploads must be below
+ * count; the current state's radius and mass are rewritten from the rows.
+ * Synchronous.  BDEM_EINVAL for a sphere context or a bad table (the
+ * context then keeps its previous table). */
+int bdem_set_poly_shapes(bdem_ctx* ctx, const bdem_poly_shape* shapes, int count);
+/* Rows of the shape table the polyhedra path reads (the POBJ count, or the
+ * count of bdem_set_poly_shapes) and whether rows are P_IDs (1) or types. */
+int bdem_get_num_poly_shapes(bdem_ctx* ctx, int* count, int* by_pid);
 /* Free-surface profile of the current state (synchronous), for the angle
  * of repose; no reference counterpart (the reference computes no angle).
  * [lo, hi) along axis 0 (x) or 2 (z) is cut into `bins` (1..4096) equal
@@ -284,14 +300,21 @@ int bdem_debug_cells(bdem_ctx* ctx, unsigned int* cell_start, unsigned int* cell
 /* Broad_List (32 per particle) and NN_NUM of the sorted state
  * (Kernel_BroadCollisionDetection_NonSymmetry). */
 int bdem_debug_contacts(bdem_ctx* ctx, unsigned int* list32, unsigned int* nn);
-/* Pair force (with gravity), pair torque, wall force, wall torque of the
- * sorted state, N x 3 floats each; sorted_pid receives the P_ID order. */
 /* One bdem_update_sim that also reports the contacts the step kernel used
  * (the reference's Broad_List / NN_NUM format: 32 slots per particle in
  * sorted order, full count), for parity checks of the skin lists.  No
  * reference counterpart. */
 int bdem_debug_step_contacts(bdem_ctx* ctx, unsigned int* list32, unsigned int* nn);
+/* Pair force (with gravity), pair torque, wall force, wall torque of the
+ * sorted state, N x 3 floats each; sorted_pid receives the P_ID order. */
 int bdem_debug_forces(bdem_ctx* ctx, float* fpp, float* tpp, float* fw, float* tw, int* sorted_pid);
+/* Work of the last polyhedra step (synchronous; no reference counterpart):
+ * counts[0] = listed pairs (bounding spheres overlapping, at most 32 per
+ * particle), counts[1] = pairs the two facing faces did not separate, on
+ * which the wave-per-pair kernel ran the full separating-plane test and,
+ * where no face separated them, the vertex-face and edge-edge searches.
+ * BDEM_EINVAL for a sphere context. */
+int bdem_debug_poly_queues(bdem_ctx* ctx, unsigned int* counts);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -146,6 +146,25 @@ int bdem_deck_initial_state(const bdem_deck* deck, bdem_float3* pos, bdem_Quater
 int bdem_fill_random(int n, const float* radius, bdem_float3 lo, bdem_float3 hi, const float* cyl, float gap,
                      unsigned long long seed, int max_tries, bdem_float3* pos, int* failed);
 
+/* Convex polyhedron shapes from point sets (no reference counterpart: the
+ * reference reads each polyhedron's vertices, faces and inertia from its
+ * POBJ file, Input/KSimulationData.cpp:1180-1530).  Shape s is the convex
+ * hull of the npts[s] (4..max_pts, max_pts <= 32) points
+ * pts[s * max_pts ...]: hull faces (coplanar points merged into one polygon
+ * of at most 8 vertices) ordered counter-clockwise from outside, points
+ * inside the hull dropped.  Derived as the reference's reader derives a
+ * POBJ: COM = vertex mean, Local_Vertex = vertex - COM, radius = largest
+ * |Local_Vertex|, face centroid = vertex mean, face normal =
+ * normalise(cross(v1 - v2, v2 - v0)) (triangles) or
+ * normalise(cross(v2 - v3, v3 - v0)) (polygons), area = centroid fan, all
+ * in float; edges in face order; mass = density (kg/m^3) x volume (cm^3)
+ * x 1e-6 and InertiaT = inverse of the solid's inertia tensor about COM,
+ * from tetrahedra in double.  Every face gets Tol = face_tol.  Host only.
+ * BDEM_EINVAL with the shape index in *failed (may be NULL) for a
+ * degenerate (flat) point set or a hull beyond the table's limits. */
+int bdem_poly_shapes_from_points(int count, const int* npts, const bdem_float3* pts, int max_pts, float density,
+                                 float face_tol, bdem_poly_shape* out, int* failed);
+
 /* Read a state file of this deck (path NULL: <project_dir>/<start_file>, or
  * the grid fill of bdem_deck_initial_state through bdem_set_pos when the
  * deck has no start file)

@@ -232,6 +232,33 @@ typedef struct bdem_ext_params {
   float pad[2];
 } bdem_ext_params;
 
+/* The geometry of one convex polyhedron, for per-particle shapes (no
+ * reference counterpart: the reference takes every polyhedron's geometry
+ * from its type's POBJ, at most 8 types of at most 32 vertices, faces and
+ * edges, KSimulationObjects.h:272-316).  The fields the polyhedra path
+ * reads, named as in POBJ; a triangulated hull of V vertices has 2V - 4
+ * faces and 3V - 6 edges, so every hull of up to 18 vertices fits.  vbound
+ * is the largest |Local_Vertex| times 1.0001; bdem_set_poly_shapes
+ * recomputes it and plane_slack. */
+#define BDEM_SHAPE_MAX_VERTEX 32
+#define BDEM_SHAPE_MAX_FACE 32
+#define BDEM_SHAPE_MAX_EDGE 64
+typedef struct bdem_poly_shape {
+  float         radius;           /* bounding radius about COM (cm) */
+  float         mass;             /* kg */
+  bdem_float3   COM;
+  int           num_vertex;
+  bdem_float3   Local_Vertex[BDEM_SHAPE_MAX_VERTEX];
+  int           num_edges;
+  bdem_Edge_Def edge[BDEM_SHAPE_MAX_EDGE];
+  int           num_faces;
+  bdem_Surface  face[BDEM_SHAPE_MAX_FACE];
+  float         InertiaT[9];      /* inverse inertia tensor about COM (body frame) */
+  float         vbound;
+  float         plane_slack;      /* largest height of a vertex above a face plane of its own
+                                     shape (0 for a convex hull up to rounding), cm */
+} bdem_poly_shape;
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
@@ -266,6 +293,9 @@ BDEM_SA(sizeof(bdem_DOBJ) == 392, "DOBJ size");
 BDEM_SA(sizeof(bdem_Macro_Cylinder) == 84, "Macro_Cylinder size");
 BDEM_SA(offsetof(bdem_WOBJ, cyl_geo) == 1296, "WOBJ cyl_geo");
 BDEM_SA(sizeof(bdem_WOBJ) == 1380, "WOBJ size");
+BDEM_SA(offsetof(bdem_poly_shape, face) == 928, "poly_shape face");
+BDEM_SA(sizeof(bdem_poly_shape) == 3148, "poly_shape size");
+
 #undef BDEM_SA
 
 #endif /* BDEM_TYPES_H_ */

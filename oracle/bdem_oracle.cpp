@@ -177,6 +177,11 @@ int bo_set_pair_override(void* h, double kn, double cn, double fk, double rr) {
   });
   return 0;
 }
+/* Per-particle polyhedron shapes: row p for P_ID p (bdem_set_poly_shapes). */
+int bo_set_poly_shapes(void* h, const bdem_poly_shape* t, int count) {
+  WITH(h, o.set_poly_shapes(t, count));
+  return 0;
+}
 /* Per-particle lifter contact type kept between steps (Contact_Info::cont_type),
  * in the current (sorted) slot order.  The setter exists for tests that start
  * a particle with a remembered contact. */

@@ -92,6 +92,23 @@ inline float damping_ratio_t(float estar, float gstar) {
   return estar > 0.0f ? float(std::sqrt(4.0 * double(gstar) / double(estar))) : 0.0f;
 }
 
+/* The geometric fields of a POBJ row as a shape row (the per-type table). */
+inline bdem_poly_shape shape_of_pobj(const bdem_POBJ& po) {
+  bdem_poly_shape s;
+  std::memset(&s, 0, sizeof(s));
+  s.radius = po.radius;
+  s.mass = po.mass;
+  s.COM = po.COM;
+  s.num_vertex = po.num_vertex;
+  std::memcpy(s.Local_Vertex, po.Local_Vertex, sizeof(po.Local_Vertex));
+  s.num_edges = po.num_edges;
+  std::memcpy(s.edge, po.edge, sizeof(po.edge));
+  s.num_faces = po.num_faces;
+  std::memcpy(s.face, po.face, sizeof(po.face));
+  std::memcpy(s.InertiaT, po.InertiaT, sizeof(po.InertiaT));
+  return s;
+}
+
 template <class R>
 class Oracle {
  public:
@@ -99,6 +116,10 @@ class Oracle {
   bdem_SimulationInfo sim{};
   std::vector<bdem_WOBJ> wobj;
   std::vector<bdem_POBJ> pobj;
+  /* polyhedra: geometry rows, one per type (the POBJ rows) or, after
+   * set_poly_shapes, one per P_ID; the type still selects the constants */
+  std::vector<bdem_poly_shape> shapes;
+  bool shape_by_pid = false;
   std::vector<bdem_DOBJ> dobj;      /* lifters (DynamicObject[32]), host-rotated */
   bdem_ext_params ext{};
   bool has_ext = false;
@@ -152,6 +173,9 @@ class Oracle {
     sim = *s;
     wobj.assign(w, w + nw);
     pobj.assign(p, p + np);
+    shapes.clear();
+    for (int t = 0; t < np; ++t) shapes.push_back(shape_of_pobj(p[t]));
+    shape_by_pid = false;
     num_wobj_active = nw;
     has_ext = e != nullptr;
     if (e) ext = *e; else std::memset(&ext, 0, sizeof(ext));
@@ -160,6 +184,21 @@ class Oracle {
   }
 
   int law_kind() const { return has_ext ? ext.contact_law : BDEM_LAW_REFERENCE; }
+
+  /* The geometry row of particle i (a dead particle, P_ID -1, takes row 0). */
+  const bdem_poly_shape& shp(int i) const {
+    return shapes[shape_by_pid ? (pid[i] > 0 ? pid[i] : 0) : int(ptype[i])];
+  }
+  /* Per-particle shapes: row p for P_ID p.  Rewrites the radius and mass
+   * of the current state's particles from their rows. */
+  void set_poly_shapes(const bdem_poly_shape* t, int count) {
+    shapes.assign(t, t + count);
+    shape_by_pid = true;
+    for (int i = 0; i < N; ++i) {
+      radius[i] = R(shp(i).radius);
+      mass[i] = R(shp(i).mass);
+    }
+  }
 
   /* Pair and wall constants of the history laws.  Restated from the
    * textbook forms (no reference counterpart):
@@ -232,6 +271,11 @@ class Oracle {
         /* per-particle sphere inverse inertia, same float formula as
          * main.cpp:615-616 evaluated per particle */
         inv_inertia[i] = R(1) / (R(0.4f) * radius[i] * radius[i] * mass[i]);
+      } else if (shape_by_pid) {
+        const bdem_poly_shape& sh = shapes[id[i] > 0 ? id[i] : 0];
+        radius[i] = R(sh.radius);
+        mass[i] = R(sh.mass);
+        inv_inertia[i] = R(sh.InertiaT[0]);
       } else {
         radius[i] = R(po.radius);
         mass[i] = R(po.mass);
@@ -800,7 +844,7 @@ class Oracle {
   /* RotatePolyhedra (Device/Utilities.cuh:420-479) */
   PolyRot rotate_poly(int i) const {
     PolyRot P;
-    const bdem_POBJ& po = pobj[ptype[i]];
+    const bdem_poly_shape& po = shp(i);
     P.nv = po.num_vertex;
     P.nf = po.num_faces;
     const V3<R> c = pos[i];
@@ -900,7 +944,7 @@ class Oracle {
   V3<R> fcent(const PolyRot& B, int f) const { return f >= 0 ? B.Fc[f] : mk<R>(R(0), R(0), R(0)); }
 
   /* The ray from PosA through `through` meets face f of B inside its polygon. */
-  bool ray_hits_face(V3<R> PosA, V3<R> through, const PolyRot& B, const bdem_POBJ& pb, int f, R tol) const {
+  bool ray_hits_face(V3<R> PosA, V3<R> through, const PolyRot& B, const bdem_poly_shape& pb, int f, R tol) const {
     if (f < 0) return false;
     V3<R> Ray = through - PosA;
     Ray = Ray / length(Ray);
@@ -914,7 +958,7 @@ class Oracle {
    * (Collision_Detection_Polyhedra.cuh:1154-1630): vertices of A inside B;
    * one vertex -> vertex-face, two -> edge-face, more -> face-face, with a
    * ray test choosing among the (up to) three closest faces. */
-  PColl vertex_faces(V3<R> PosA, const PolyRot& A, V3<R> PosB, const PolyRot& B, const bdem_POBJ& pb) const {
+  PColl vertex_faces(V3<R> PosA, const PolyRot& A, V3<R> PosB, const PolyRot& B, const bdem_poly_shape& pb) const {
     PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
     int np = 0;
     int valid[8];
@@ -983,7 +1027,7 @@ class Oracle {
   }
 
   /* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707) */
-  PColl edges(const PolyRot& A, const bdem_POBJ& pa, const PolyRot& B, const bdem_POBJ& pb, V3<R> RelPos,
+  PColl edges(const PolyRot& A, const bdem_poly_shape& pa, const PolyRot& B, const bdem_poly_shape& pb, V3<R> RelPos,
               V3<R> PosB) const {
     PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_SP};
     R min_edgedis = R(5.0f);
@@ -1052,8 +1096,8 @@ class Oracle {
    * the contact of a with listed partner b (A's rotated shape given), after
    * the reference's depth clamps. */
   PColl poly_pair_contact(int a, int b, const PolyRot& A) const {
-    const bdem_POBJ& pa = pobj[ptype[a]];
-    const bdem_POBJ& pb = pobj[ptype[b]];
+    const bdem_poly_shape& pa = shp(a);
+    const bdem_poly_shape& pb = shp(b);
     const V3<R> PosA = pos[a], PosB = pos[b];
     const V3<R> RVel = vel[a] - vel[b];
     const PolyRot B = rotate_poly(b);
@@ -1137,7 +1181,7 @@ class Oracle {
     PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
     const V3<R> n = cvt<R>(S.normal), c = cvt<R>(S.centroid);
     if (dot(n, (pos[i] - c)) > R(0.000000f)) {
-      if (dot(n, (pos[i] - c)) <= R(pobj[ptype[i]].radius)) {
+      if (dot(n, (pos[i] - c)) <= R(shp(i).radius)) {
         res = vertices_surface(A, S, svtx);
         if (res.cls == C_FF) res.cp = pos[i];   /* no moment for face-face */
         return res;
@@ -1173,7 +1217,7 @@ class Oracle {
     PColl res{false, R(0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), C_UNDEF};
     const V3<R> AX = cvt<R>(cyl.AXIS), cb = cvt<R>(cyl.center_bot_cap);
     const V3<R> Pvec = pos[i] * AX - cb;
-    const R Pvec_dis = length(Pvec) + R(pobj[ptype[i]].radius);
+    const R Pvec_dis = length(Pvec) + R(shp(i).radius);
     if (!(Pvec_dis > R(cyl.radius))) return res;
     R dis = R(0.0);
     int cnt = 0;
@@ -1271,7 +1315,7 @@ class Oracle {
    * the world-frame torque, as written. */
   V3<R> angular_acc(int i, V3<R> tau) const {
     const Q4<R> Q = quat[i];
-    const float* I0 = pobj[ptype[i]].InertiaT;
+    const float* I0 = shp(i).InertiaT;
     R RQ[9];
     RQ[0] = (Q.w * Q.w + Q.x * Q.x - Q.y * Q.y - Q.z * Q.z);
     RQ[1] = (R(2.0f) * Q.x * Q.y - R(2.0f) * Q.w * Q.z);
@@ -1555,7 +1599,7 @@ class Oracle {
    * lifter's point is nearer the particle's centre.  The reference leaves
    * the normal unset when no such crossing is within 0.1 cm of the other
    * point; that case is no contact here. */
-  PColl edges_dobj(const bdem_DOBJ& D, const PolyRot& B, const bdem_POBJ& pb, V3<R> PosB, V3<R> RelPos) const {
+  PColl edges_dobj(const bdem_DOBJ& D, const PolyRot& B, const bdem_poly_shape& pb, V3<R> PosB, V3<R> RelPos) const {
     PColl res{false, R(0), mk<R>(R(0.0f), R(0.0f), R(0.0f)), mk<R>(0, 0, 0), C_SP};
     R min_edgedis = R(0.10f);
     V3<R> selA = mk<R>(0, 0, 0), selB = selA, nsel = selA, cpsum = selA;
@@ -1623,7 +1667,7 @@ class Oracle {
   void lifter_forces_poly(int i, const PolyRot& A, V3<R>& F, V3<R>& T) const {
     F = mk<R>(0, 0, 0);
     T = mk<R>(0, 0, 0);
-    const bdem_POBJ& pa = pobj[ptype[i]];
+    const bdem_poly_shape& pa = shp(i);
     const R r = R(pa.radius);
     const V3<R> P = pos[i];
     const bool check = dobj[0].is_translating || !sim.isMillSim || sim.Rotating_WObject ||

@@ -44,6 +44,7 @@ def load():
     lib.bo_set_active_world_objects.argtypes = [P, C.c_int]
     lib.bo_set_drum_rotation.argtypes = [P, C.c_int]
     lib.bo_set_pair_override.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.bo_set_poly_shapes.argtypes = [P, C.c_void_p, C.c_int]
     lib.bo_set_dobj.argtypes = [P, C.c_void_p, C.c_int]
     lib.bo_lifter_types.argtypes = [P, ip]
     lib.bo_set_lifter_types.argtypes = [P, ip]
@@ -86,6 +87,8 @@ class Oracle:
         self.bits = bits
         if deck.get("dobj"):
             self.set_dobj(deck["dobj"])
+        if deck.get("shapes") is not None:
+            self.set_poly_shapes(deck["shapes"])
         self.set_state(deck)
 
     def set_state(self, deck):
@@ -211,6 +214,12 @@ class Oracle:
     def set_pair_override(self, kn, cn, fk, rr):
         """Exact pair constants of type pair (0, 0) and rolling resistance."""
         self.lib.bo_set_pair_override(self.h, float(kn), float(cn), float(fk), float(rr))
+
+    def set_poly_shapes(self, shapes):
+        """Per-particle polyhedron shapes (a ctypes array of bdem_poly_shape):
+        row p for P_ID p."""
+        self._shapes = shapes
+        self.lib.bo_set_poly_shapes(self.h, C.cast(shapes, C.c_void_p), len(shapes))
 
     def set_drum_rotation(self, on):
         self.lib.bo_set_drum_rotation(self.h, 1 if on else 0)

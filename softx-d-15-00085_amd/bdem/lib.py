@@ -58,6 +58,8 @@ _PROTOS = {
     "bdem_get_surface_profile": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
     "bdem_get_surface_profile_noalloc": (C.c_int, [_P, C.c_int, C.c_float, C.c_float, C.c_int, _fp]),
     "bdem_get_poly_vertices": (C.c_int, [_P, C.c_int, _f3p, _ip, _ip]),
+    "bdem_set_poly_shapes": (C.c_int, [_P, C.POINTER(T.PolyShape), C.c_int]),
+    "bdem_get_num_poly_shapes": (C.c_int, [_P, _ip, _ip]),
     "bdem_surface_angle": (C.c_int, [_fp, C.c_int, C.c_float, C.c_float, _dp, _dp, _dp]),
     "bdem_sort_changed": (C.c_int, [_P, _up]),
     "bdem_calibration_copy": (C.c_int, [_P, C.c_ulonglong, C.c_int]),
@@ -84,6 +86,7 @@ _PROTOS = {
     "bdem_debug_contacts": (C.c_int, [_P, _up, _up]),
     "bdem_debug_forces": (C.c_int, [_P, _fp, _fp, _fp, _fp, _ip]),
     "bdem_debug_step_contacts": (C.c_int, [_P, _up, _up]),
+    "bdem_debug_poly_queues": (C.c_int, [_P, _up]),
     "bdem_set_lifter_rotation": (C.c_int, [_P, C.c_double, C.c_float, C.c_float, C.c_float]),
     "bdem_get_dobjects": (C.c_int, [_P, C.POINTER(T.DOBJ), _ip]),
     "bdem_set_skin": (C.c_int, [_P, C.c_float]),
@@ -106,6 +109,8 @@ _PROTOS = {
     "bdem_deck_initial_state": (C.c_int, [_P, _f3p, _qp, _up, _ip]),
     "bdem_fill_random": (C.c_int, [C.c_int, _fp, T.Float3, T.Float3, _fp, C.c_float, C.c_ulonglong, C.c_int,
                                    _f3p, _ip]),
+    "bdem_poly_shapes_from_points": (C.c_int, [C.c_int, _ip, _f3p, C.c_int, C.c_float, C.c_float,
+                                               C.POINTER(T.PolyShape), _ip]),
     # include/bdem_batch.h
     "bdem_batch_create": (C.c_int, [C.POINTER(_P), C.c_int]),
     "bdem_batch_destroy": (C.c_int, [_P]),
@@ -157,6 +162,33 @@ def fill_random(radius, lo, hi, seed, gap=0.01, cyl=None, max_tries=100000):
     if rc != 0:
         raise BdemError(f"bdem_fill_random: status {rc}, sphere {failed.value} could not be placed")
     return pos
+
+
+def poly_shapes_from_points(points, density, face_tol=0.1):
+    """bdem_poly_shapes_from_points (host only): one convex-hull shape per
+    point set.  points: a list of (k, 3) arrays (4 <= k <= 32) or one
+    (count, max_pts, 3) array with counts in a second element, i.e.
+    (pts, npts).  Returns a ctypes array of PolyShape."""
+    lib = load()
+    if isinstance(points, tuple):
+        pts, npts = points
+        pts = np.ascontiguousarray(pts, np.float32)
+        npts = np.ascontiguousarray(npts, np.int32)
+    else:
+        m = max(len(p) for p in points)
+        pts = np.zeros((len(points), m, 3), np.float32)
+        npts = np.zeros(len(points), np.int32)
+        for k, p in enumerate(points):
+            pts[k, :len(p)] = p
+            npts[k] = len(p)
+    count, max_pts = pts.shape[0], pts.shape[1]
+    out = (T.PolyShape * count)()
+    failed = C.c_int(-1)
+    rc = lib.bdem_poly_shapes_from_points(count, _ptr(npts, C.c_int), _ptr(pts, T.Float3), max_pts,
+                                          float(density), float(face_tol), out, C.byref(failed))
+    if rc != 0:
+        raise BdemError(f"bdem_poly_shapes_from_points: status {rc}, shape {failed.value} is degenerate or too large")
+    return out
 
 
 def surface_angle(top, lo, hi):
@@ -315,13 +347,26 @@ class Device:
         num_vertex [n], pid [n]) in the library's order."""
         n = self.n
         if max_vertex is None:
-            max_vertex = max(p.num_vertex for p in self._pobj)
+            rows = self._shapes if getattr(self, "_shapes", None) is not None else self._pobj
+            max_vertex = max(p.num_vertex for p in rows)
         verts = np.zeros((n, int(max_vertex), 3), np.float32)
         nv = np.zeros(n, np.int32)
         pid = np.zeros(n, np.int32)
         self._check(self.lib.bdem_get_poly_vertices(self.h, int(max_vertex), _ptr(verts, T.Float3),
                                                     _ptr(nv, C.c_int), _ptr(pid, C.c_int)))
         return verts, nv, pid
+
+    def set_poly_shapes(self, shapes):
+        """bdem_set_poly_shapes: per-particle shapes, row p for P_ID p (a
+        ctypes array of PolyShape, e.g. from poly_shapes_from_points)."""
+        self._check(self.lib.bdem_set_poly_shapes(self.h, shapes, len(shapes)))
+        self._shapes = shapes
+
+    def num_poly_shapes(self):
+        """(rows, by_pid) of the shape table the polyhedra path reads."""
+        cnt, by = C.c_int(), C.c_int()
+        self._check(self.lib.bdem_get_num_poly_shapes(self.h, C.byref(cnt), C.byref(by)))
+        return cnt.value, bool(by.value)
 
     def surface_profile(self, lo, hi, bins, axis=0):
         """Highest particle top y + r per bin of [lo, hi) along x (axis 0) or
@@ -445,6 +490,12 @@ class Device:
         e = np.zeros(nc, np.uint32)
         self._check(self.lib.bdem_debug_cells(self.h, _ptr(s, C.c_uint), _ptr(e, C.c_uint)))
         return s, e
+
+    def debug_poly_queues(self):
+        """(listed pairs, pairs left for the narrow search) of the last polyhedra step."""
+        out = np.zeros(2, np.uint32)
+        self._check(self.lib.bdem_debug_poly_queues(self.h, _ptr(out, C.c_uint)))
+        return int(out[0]), int(out[1])
 
     def debug_contacts(self):
         lst = np.zeros(self.n * 32, np.uint32)

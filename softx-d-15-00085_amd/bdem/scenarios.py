@@ -936,6 +936,148 @@ def poly_pair_deck(pa, pb, qa, qb, shape_a=(CUBE_VERTS, CUBE_FACES), shape_b=Non
     return d
 
 
+def hull_points(seed, n, kmin=8, kmax=16, rmin=0.15, rmax=0.30):
+    """Point sets for per-particle hulls: k_p ~ U{kmin..kmax} points uniform
+    on the unit sphere (seeded), scaled so that the largest distance from
+    their mean (the body's COM as the reference's reader takes it) is
+    r_p ~ U[rmin, rmax].  Returns (pts [n, kmax, 3] float32, k [n] int32,
+    r [n])."""
+    k = (kmin + np.floor(uniform(seed, STREAM_SHAPE, n, 0.0, 1.0) * (kmax - kmin + 1))).astype(np.int32)
+    k = np.minimum(k, kmax)
+    u = uniform(seed, STREAM_SHAPE + 32, 2 * n * kmax, 0.0, 1.0).reshape(n, kmax, 2).astype(np.float64)
+    z = 1.0 - 2.0 * u[..., 0]
+    ph = 2.0 * math.pi * u[..., 1]
+    rho = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    P = np.stack([rho * np.cos(ph), rho * np.sin(ph), z], -1)
+    live = np.arange(kmax)[None, :] < k[:, None]
+    P = np.where(live[..., None], P, 0.0)
+    mean = P.sum(1) / k[:, None]
+    d = np.where(live, np.linalg.norm(P - mean[:, None, :], axis=-1), 0.0).max(1)
+    r = uniform(seed, STREAM_RADIUS, n, rmin, rmax).astype(np.float64)
+    P = np.where(live[..., None], P * (r / d)[:, None, None], 0.0)
+    return P.astype(np.float32), k, r
+
+
+def shape_from_pobj(p):
+    """The geometry of a POBJ row as a PolyShape row (what the library makes
+    of a per-type table)."""
+    s = T.PolyShape()
+    s.radius, s.mass, s.COM = p.radius, p.mass, p.COM
+    s.num_vertex, s.num_edges, s.num_faces = p.num_vertex, p.num_edges, p.num_faces
+    for k in range(32):
+        s.Local_Vertex[k] = p.Local_Vertex[k]
+        s.edge[k] = p.edge[k]
+        s.face[k] = p.face[k]
+    for g in range(9):
+        s.InertiaT[g] = p.InertiaT[g]
+    return s
+
+
+def _poly_type_for_shapes(shapes, density, kn, e, mu):
+    """Type 0 of a per-particle-shape deck: the constants every pair and wall
+    contact uses (POBJ::PP and the surface constants), damping from the
+    restitution at the mean particle mass (the reference keeps one constant
+    per type pair, Set_SpringConstants_KN, Input/KSimulationData.cpp:950-1045).
+    Its own geometry (a cube of the mean bounding radius) is not read once
+    the shape table is set."""
+    rbar = float(np.mean([sh.radius for sh in shapes]))
+    p = poly_pobj(CUBE_VERTS * (rbar / math.sqrt(3.0)), CUBE_FACES, density, e, kn, mu)
+    m = float(np.float32(np.mean([sh.mass for sh in shapes])))
+    p.PP[0].Cn = cn_from_cor(float(np.float32(kn)), 0.5 * m, e)
+    p.surface_Cn = cn_from_cor(float(np.float32(kn)), m, e)
+    p.Dsurface_Cn = p.surface_Cn
+    return p
+
+
+def hull_mill_deck(n=200000, seed=21, R=30.0, L=20.0, n_lifters=8, lifter_w=2.0, lifter_h=3.0,
+                   speed=0.6, rmin=0.3, rmax=0.6, kmin=8, kmax=16, density=2700.0, kn=2e5, e=0.4, mu=0.6,
+                   dt=2e-6, gap=0.02, vmax=0.0, fill_scale=0.5):
+    """Per-particle convex hulls in a lifter drum: n hulls of kmin..kmax
+    seeded points on a sphere (hull_points, bounding radius U[rmin, rmax]
+    about the COM), each its own row of the shape table (P_ID p -> row p),
+    in a drum of radius R and length L (axis z, centre (R, R), end plates)
+    with n_lifters rectangular lifters (lifter_w wide, lifter_h high)
+    turning at `speed` x the critical speed sqrt(g / R).  Start: random
+    non-overlapping bounding spheres (bdem_fill_random, largest first, gap
+    apart) filling the drum inside the lifter tips, random orientations;
+    the spheres placed have fill_scale x the bounding radius.  200,000
+    bounding spheres of U[0.3, 0.6] cm would need 1.9 times the drum's
+    volume, so the default places half-radius spheres (23 % of the space
+    inside the lifter tips): the hulls then start in contact with about 13
+    neighbours each and the charge spreads from there."""
+    from .lib import fill_random, poly_shapes_from_points
+    f = np.float32
+    pts, k, _ = hull_points(seed, n, kmin, kmax, rmin, rmax)
+    shapes = poly_shapes_from_points((pts, k), density)
+    rad = np.array([sh.radius for sh in shapes], np.float32)
+    x0, x1 = R - lifter_w / 2, R + lifter_w / 2
+    base = lifter_dobj([(x0, 0, 0), (x0, 0, L), (x1, 0, L), (x1, 0, 0),
+                        (x0, lifter_h, 0), (x0, lifter_h, L), (x1, lifter_h, L), (x1, lifter_h, 0)],
+                       (R, lifter_h / 2, L / 2), max(lifter_w, lifter_h))
+    about = (R, R, 0.0)
+    dobj = [base] + [rotate_dobj(base, about, 2.0 * math.pi * q / n_lifters) for q in range(1, n_lifters)]
+    # fill in a frame whose vertical axis is the drum axis: (X, Y, Z) = (x, z, y)
+    inner = R - lifter_h - 0.05
+    order = np.argsort(-rad, kind="stable")
+    placed = fill_random(rad[order] * np.float32(fill_scale), (R - inner, 0.05, R - inner), (R + inner, L - 0.05, R + inner), seed,
+                         gap=gap, cyl=(R, R, inner), max_tries=1000000)
+    pos = np.zeros((n, 3), np.float32)
+    pos[order] = placed[:, [0, 2, 1]]
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3) if vmax > 0 else np.zeros((n, 3))
+    cell = 2.0 * float(rad.max())
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (2 * R + 1, 2 * R + 1, L + 1), cell,
+                   rotation=True, roll_res=0.0, sim_type=1)
+    sim.particle_type = 1
+    sim.Num_ParticleObjects = 1
+    sim.isMillSim = True
+    sim.Rotating_WObject = False
+    omega = speed * math.sqrt(981.0 / R)
+    sim.MillRotVel = float(f(-omega))
+    sim.DObject_CylinderBoundR = float(f(f(R) - f(2.0) * f(base.COM.y)))
+    sim.cent = T.f3(R, R, 0.0)
+    pobj = [_poly_type_for_shapes(shapes, density, kn, e, mu)]
+    drum = cylinder_wobj(R, L, (0.0, 0.0, 1.0), (0.0, 0.0, 0.0), top=True, bot=True)
+    deck = _deck(sim, [drum], pobj, pos, vel, name=f"hullmill{n}")
+    deck["shapes"] = shapes
+    deck["dobj"] = dobj
+    deck["lifter_about"] = about
+    deck["rad_per_step"] = -omega * dt
+    deck["rpm"] = omega * 60.0 / (2.0 * math.pi)
+    deck["quat"] = random_quats(seed, n)
+    deck["avel"] = np.zeros((n, 3), np.float32)
+    return deck
+
+
+def hull_box_deck(n=600, seed=41, L=14.0, rmin=0.5, rmax=0.9, kmin=8, kmax=16, density=2500.0, kn=1e5,
+                  e=0.6, mu=0.4, dt=1e-5, vmax=20.0, wmax=10.0, overlap=0.25, jitter=0.05):
+    """Per-particle hulls (hull_points) on a jittered lattice in a closed box
+    with random orientations, velocities and spins; the lattice pitch is
+    2 rmax (1 - overlap), so bounding spheres overlap and vertex, edge and
+    face contacts of every kind occur from the first step."""
+    from .lib import poly_shapes_from_points
+    pts, k, _ = hull_points(seed, n, kmin, kmax, rmin, rmax)
+    shapes = poly_shapes_from_points((pts, k), density)
+    rb = max(sh.radius for sh in shapes)
+    d = 2.0 * rb * (1.0 - overlap)
+    m = int(math.floor((L - 2 * rb - 0.2) / d)) + 1
+    assert m ** 3 >= n, "box too small for the lattice"
+    gi = np.indices((m, m, m)).reshape(3, -1).T[:n].astype(np.float64)
+    pos = rb + 0.1 + gi * d
+    pos += uniform(seed, STREAM_JITTER, 3 * n, -jitter * d, jitter * d).reshape(n, 3)
+    vel = uniform(seed, STREAM_VEL, 3 * n, -vmax, vmax).reshape(n, 3)
+    avel = uniform(seed, STREAM_VEL + 8, 3 * n, -wmax, wmax).reshape(n, 3)
+    sim = sim_info(n, dt, (0.0, -981.0, 0.0), (-1.0, -1.0, -1.0), (L, L, L), 2.0 * rb, rotation=True,
+                   roll_res=0.0)
+    sim.particle_type = 1
+    sim.Num_ParticleObjects = 1
+    pobj = [_poly_type_for_shapes(shapes, density, kn, e, mu)]
+    out = _deck(sim, [box_walls(L)], pobj, pos, vel, name=f"hullbox{n}")
+    out["shapes"] = shapes
+    out["avel"] = avel.astype(np.float32)
+    out["quat"] = random_quats(seed, n)
+    return out
+
+
 def poly_mill_deck(n=300, seed=11, size=1.0, gap=0.6, R=15.0, L=10.0, n_lifters=8, lifter_w=1.5,
                    lifter_h=1.5, rpm=30.0, density=2700.0, kn=2e5, e=0.4, mu=0.6, dt=1e-5, vmax=10.0):
     """Cubes of side `size` in the drum of mill_deck (8 attached lifters),

@@ -99,6 +99,27 @@ class PPParms(C.Structure):
                 ("Fric_kinetic", C.c_float)]
 
 
+SHAPE_MAX_VERTEX, SHAPE_MAX_FACE, SHAPE_MAX_EDGE = 32, 32, 64
+
+
+class PolyShape(C.Structure):
+    """bdem_poly_shape: one convex polyhedron's geometry (per-particle shapes)."""
+    _fields_ = [
+        ("radius", C.c_float),
+        ("mass", C.c_float),
+        ("COM", Float3),
+        ("num_vertex", C.c_int),
+        ("Local_Vertex", Float3 * SHAPE_MAX_VERTEX),
+        ("num_edges", C.c_int),
+        ("edge", EdgeDef * SHAPE_MAX_EDGE),
+        ("num_faces", C.c_int),
+        ("face", Surface * SHAPE_MAX_FACE),
+        ("InertiaT", C.c_float * 9),
+        ("vbound", C.c_float),
+        ("plane_slack", C.c_float),
+    ]
+
+
 class POBJ(C.Structure):
     _fields_ = [
         ("radius", C.c_float),

@@ -54,6 +54,16 @@ int dalloc(bdem_ctx* c, T** p, size_t count) {
   return BDEM_OK;
 }
 
+/* Frees one allocation made by dalloc. */
+template <class T>
+void dfree(bdem_ctx* c, T*& p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), (void*)p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree((void*)p);
+  p = nullptr;
+}
+
 void drop_graphs(bdem_ctx* c) {
   if ((c->graph[0] || c->graph[1]) && c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& g : c->graph) {
@@ -68,7 +78,9 @@ void free_all(bdem_ctx* c) {
   c->allocs.clear();
   c->b = StepBuffers{};
   c->pb = PolyBuffers{};
-  c->dpobj = nullptr;
+  c->dshape = nullptr;
+  c->shapes.clear();
+  c->shape_by_pid = false;
   c->dp = nullptr;
   c->configured = false;
 }
@@ -140,15 +152,11 @@ void build_params(bdem_ctx* c) {
   P.max_size = s.max_size;
   P.kill_enabled = s.Kill_Particles ? 1 : 0;
   P.one_type = c->pobj.size() == 1 ? 1 : 0;   /* uploads reject types >= num_pobj */
+  P.vbound_scale = float(c->vbound_scale);
+  P.shape_by_pid = c->shape_by_pid ? 1 : 0;
   for (size_t t = 0; t < c->pobj.size() && t < size_t(MAX_TYPES); ++t) {
     const bdem_POBJ& po = c->pobj[t];
     P.t_radius[t] = po.radius;
-    double vmax = 0.0;
-    for (int v = 0; v < po.num_vertex && v < 32; ++v) {
-      const double x = po.Local_Vertex[v].x, y = po.Local_Vertex[v].y, z = po.Local_Vertex[v].z;
-      vmax = std::fmax(vmax, std::sqrt(x * x + y * y + z * z));
-    }
-    P.t_vbound[t] = float(vmax * 1.0001 * c->vbound_scale);
     P.t_mass[t] = po.mass;
     P.t_inertia[t] = po.InertiaT[0];
     for (size_t u = 0; u < c->pobj.size() && u < size_t(MAX_TYPES); ++u) {
@@ -200,8 +208,8 @@ int upload_params(bdem_ctx* c) {
   if (rc0 != BDEM_OK) return rc0;
   build_params(c);
   HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
-  if (c->dpobj)
-    HIPCHK(c, hipMemcpyAsync(c->dpobj, c->pobj.data(), c->pobj.size() * sizeof(bdem_POBJ),
+  if (c->dshape)
+    HIPCHK(c, hipMemcpyAsync(c->dshape, c->shapes.data(), c->shapes.size() * sizeof(bdem_poly_shape),
                              hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BDEM_OK;
@@ -269,14 +277,15 @@ int check_wobj(bdem_ctx* c, const bdem_WOBJ* wobj, int num_wobj) {
   return BDEM_OK;
 }
 
-/* POBJ polyhedron tables (KSimulationObjects.h:272-300): the index ranges
- * the polyhedra kernels read. */
-int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
+/* POBJ polyhedron tables (KSimulationObjects.h:272-300) and per-particle
+ * shape tables: the index ranges the polyhedra kernels read. */
+template <class T>
+int check_poly(bdem_ctx* c, const T* p, int np, int max_vertex, int max_face, int max_edge) {
   for (int t = 0; t < np; ++t) {
-    const bdem_POBJ& P = p[t];
-    if (P.num_vertex < 1 || P.num_vertex > 32 || P.num_faces < 1 || P.num_faces > 32 || P.num_edges < 0 ||
-        P.num_edges > 32)
-      return fail(c, BDEM_EINVAL, "polyhedron vertex/face/edge count out of range (type " + std::to_string(t) + ")");
+    const T& P = p[t];
+    if (P.num_vertex < 1 || P.num_vertex > max_vertex || P.num_faces < 1 || P.num_faces > max_face ||
+        P.num_edges < 0 || P.num_edges > max_edge)
+      return fail(c, BDEM_EINVAL, "polyhedron vertex/face/edge count out of range (row " + std::to_string(t) + ")");
     if (!(P.radius > 0.0f) || !(P.mass > 0.0f))
       return fail(c, BDEM_EINVAL, "polyhedron radius and mass must be positive");
     for (int f = 0; f < P.num_faces; ++f) {
@@ -287,7 +296,7 @@ int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
       const double nn = std::sqrt(double(S.normal.x) * S.normal.x + double(S.normal.y) * S.normal.y +
                                   double(S.normal.z) * S.normal.z);
       if (!(std::fabs(nn - 1.0) <= 1e-5))
-        return fail(c, BDEM_EINVAL, "polyhedron face normal is not unit length (type " + std::to_string(t) +
+        return fail(c, BDEM_EINVAL, "polyhedron face normal is not unit length (row " + std::to_string(t) +
                                         ", face " + std::to_string(f) + ")");
       for (int q = 0; q < S.num_vertex; ++q)
         if (S.vertex_Order[q] < 0 || S.vertex_Order[q] >= P.num_vertex)
@@ -298,6 +307,71 @@ int check_poly(bdem_ctx* c, const bdem_POBJ* p, int np) {
         return fail(c, BDEM_EINVAL, "polyhedron edge vertex index out of range");
   }
   return BDEM_OK;
+}
+
+/* Largest |Local_Vertex| x 1.0001 (cm), in double: a bound on the distance
+ * of any turned vertex from the centre (unit quaternion). */
+float vertex_bound(const bdem_float3* lv, int nv) {
+  double vmax = 0.0;
+  for (int v = 0; v < nv; ++v) {
+    const double x = lv[v].x, y = lv[v].y, z = lv[v].z;
+    vmax = std::fmax(vmax, std::sqrt(x * x + y * y + z * z));
+  }
+  return float(vmax * 1.0001);
+}
+
+/* Largest height of a vertex above a face plane of the same body, in
+ * double (0 when the faces support the body: a convex hull up to rounding).
+ * The edge search uses the face planes as bounds only up to this slack. */
+float plane_slack(const bdem_float3* lv, int nv, const bdem_Surface* face, int nf, bdem_float3 com) {
+  double worst = 0.0;
+  for (int f = 0; f < nf; ++f) {
+    const bdem_Surface& S = face[f];
+    for (int v = 0; v < nv; ++v) {
+      const double h = double(S.normal.x) * (double(lv[v].x) + com.x - S.centroid.x) +
+                       double(S.normal.y) * (double(lv[v].y) + com.y - S.centroid.y) +
+                       double(S.normal.z) * (double(lv[v].z) + com.z - S.centroid.z);
+      worst = std::fmax(worst, h);
+    }
+  }
+  return float(worst * 1.0001 + 1e-6);
+}
+
+/* The geometric part of a POBJ row as a shape-table row. */
+bdem_poly_shape shape_of_pobj(const bdem_POBJ& po) {
+  bdem_poly_shape s;
+  std::memset(&s, 0, sizeof(s));
+  s.radius = po.radius;
+  s.mass = po.mass;
+  s.COM = po.COM;
+  s.num_vertex = po.num_vertex;
+  std::memcpy(s.Local_Vertex, po.Local_Vertex, sizeof(po.Local_Vertex));
+  s.num_edges = po.num_edges;
+  std::memcpy(s.edge, po.edge, sizeof(po.edge));
+  s.num_faces = po.num_faces;
+  std::memcpy(s.face, po.face, sizeof(po.face));
+  std::memcpy(s.InertiaT, po.InertiaT, sizeof(po.InertiaT));
+  s.vbound = vertex_bound(po.Local_Vertex, po.num_vertex);
+  s.plane_slack = plane_slack(po.Local_Vertex, po.num_vertex, po.face, po.num_faces, po.COM);
+  return s;
+}
+
+/* The shape-table row of a packed P_ID * 8 + type word (the device's
+ * shape_index), or -1 when the table has no such row. */
+int shape_row(const bdem_ctx* c, int packed) {
+  int row = packed & 7;
+  if (c->shape_by_pid) row = (packed >> 3) > 0 ? (packed >> 3) : 0;
+  return row < (int)c->shapes.size() ? row : -1;
+}
+
+/* rot's row layout from the largest vertex and face counts of the table. */
+PolyLayout poly_layout(const std::vector<bdem_poly_shape>& t) {
+  int pv = 0, pf = 0;
+  for (const bdem_poly_shape& s : t) {
+    pv = s.num_vertex > pv ? s.num_vertex : pv;
+    pf = s.num_faces > pf ? s.num_faces : pf;
+  }
+  return PolyLayout{pv, pf, pv + 2 * pf};
 }
 
 /* Skin masks for the sphere step: allowed when the hash is injective
@@ -364,6 +438,8 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
     /* P_ID * 8 + type is packed into one int; -1 marks a dead particle */
     if (pid[i] < -1 || pid[i] >= (1 << 28))
       return fail(c, BDEM_EINVAL, "P_ID out of range [-1, 2^28) at index " + std::to_string(i));
+    if (c->shape_by_pid && pid[i] >= (int)c->shapes.size())
+      return fail(c, BDEM_EINVAL, "P_ID beyond the per-particle shape table at index " + std::to_string(i));
     if (hist && pid[i] >= c->b.hist_cap)
       return fail(c, BDEM_EINVAL, "history laws need P_ID < Num_Particles (index " + std::to_string(i) + ")");
     if (check_q && pid[i] >= 0) {
@@ -396,8 +472,13 @@ int upload_state(bdem_ctx* c, const bdem_float3* pos, const bdem_Quaterion* ornt
   }
   for (int i = 0; i < n; ++i) {
     const unsigned t = ptype[i];
-    const float r = c->hp.per_particle ? oldP[i].w : c->pobj[t].radius;
-    const float m = c->hp.per_particle ? oldV[i].w : c->pobj[t].mass;
+    float r = c->hp.per_particle ? oldP[i].w : c->pobj[t].radius;
+    float m = c->hp.per_particle ? oldV[i].w : c->pobj[t].mass;
+    if (c->shape_by_pid) {   /* per-particle shapes carry their own radius and mass */
+      const bdem_poly_shape& sh = c->shapes[pid[i] > 0 ? pid[i] : 0];
+      r = sh.radius;
+      m = sh.mass;
+    }
     P[i] = make_float4(pos[i].x, pos[i].y, pos[i].z, r);
     if (vel) V[i] = make_float4(vel[i].x, vel[i].y, vel[i].z, m);
     else V[i] = make_float4(oldV[i].x, oldV[i].y, oldV[i].z, m);
@@ -622,7 +703,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
       return fail(c, BDEM_ENOTSUP,
                   "polyhedra contact-type history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) is "
                   "disabled in the reference (main.cpp:830) and undefined as written; not built");
-    if ((rc0 = check_poly(c, pobj, num_pobj)) != BDEM_OK) return rc0;
+    if ((rc0 = check_poly(c, pobj, num_pobj, 32, 32, 32)) != BDEM_OK) return rc0;
   }
   if ((rc0 = check_wobj(c, wobj, num_wobj)) != BDEM_OK) return rc0;
 
@@ -670,13 +751,9 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   ALLOC(b.wall_force, (size_t)n * 3);
   if (c->poly) {
     PolyBuffers& pb = c->pb;
-    int pv = 0, pf = 0;
-    for (int t = 0; t < num_pobj; ++t) {
-      pv = pobj[t].num_vertex > pv ? pobj[t].num_vertex : pv;
-      pf = pobj[t].num_faces > pf ? pobj[t].num_faces : pf;
-    }
-    pb.L = PolyLayout{pv, pf, pv + 2 * pf};
-    ALLOC(c->dpobj, (size_t)num_pobj);
+    for (int t = 0; t < num_pobj; ++t) c->shapes.push_back(shape_of_pobj(pobj[t]));
+    pb.L = poly_layout(c->shapes);
+    ALLOC(c->dshape, c->shapes.size());
     ALLOC(pb.rot, (size_t)n * pb.L.stride);
     ALLOC(pb.list, (size_t)n * LIST_STRIDE);
     ALLOC(pb.nn, (size_t)n);
@@ -837,13 +914,13 @@ int bdem_get_system_state(bdem_ctx* c, bdem_float3* pos, bdem_Quaterion* ornt, b
   std::vector<float4> P, V, O, Q;
   if ((rc = download(c, P, V, O, Q)) != BDEM_OK) return rc;
   for (int i = 0; i < c->b.n; ++i) {
-    const int packed_t = f_bits(O[i].w) & 7;
     if (pos) {
       pos[i] = bdem_float3{P[i].x, P[i].y, P[i].z};
-      if (c->poly && packed_t < int(c->pobj.size())) {
+      const int row = shape_row(c, f_bits(O[i].w));
+      if (c->poly && row >= 0) {
         /* Get_SystemState / Get_Particle_PosArray return position - COM for
          * polyhedra (GPU_Memory_Transfer.cuh:461, 504) */
-        const bdem_float3 cm = c->pobj[packed_t].COM;
+        const bdem_float3 cm = c->shapes[row].COM;
         pos[i] = bdem_float3{P[i].x - cm.x, P[i].y - cm.y, P[i].z - cm.z};
       }
     }
@@ -940,6 +1017,70 @@ int bdem_get_particle_stats(bdem_ctx* c, float min_y, const bdem_float3* box_lo,
   return BDEM_OK;
 }
 
+int bdem_set_poly_shapes(bdem_ctx* c, const bdem_poly_shape* shapes, int count) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!c->poly) return fail(c, BDEM_EINVAL, "not a polyhedra context (particle_type 1)");
+  if (!shapes || count < 1 || count > (1 << 28))
+    return fail(c, BDEM_EINVAL, "shape count must be in [1, 2^28] with a non-null table");
+  if ((rc = check_poly(c, shapes, count, BDEM_SHAPE_MAX_VERTEX, BDEM_SHAPE_MAX_FACE, BDEM_SHAPE_MAX_EDGE)) !=
+      BDEM_OK)
+    return rc;
+  for (int k = 0; k < count; ++k)
+    if (!(shapes[k].radius > 0.0f) || !(shapes[k].mass > 0.0f))
+      return fail(c, BDEM_EINVAL, "shape radius and mass must be positive (row " + std::to_string(k) + ")");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<float4> P, V, O, Q;
+  if ((rc = download(c, P, V, O, Q)) != BDEM_OK) return rc;
+  const int n = c->b.n;
+  for (int i = 0; i < n; ++i)
+    if ((f_bits(O[i].w) >> 3) >= count)
+      return fail(c, BDEM_EINVAL, "a particle's P_ID has no row in the shape table (index " + std::to_string(i) + ")");
+  std::vector<bdem_poly_shape> t(shapes, shapes + count);
+  for (bdem_poly_shape& s : t) {
+    s.vbound = vertex_bound(s.Local_Vertex, s.num_vertex);
+    s.plane_slack = plane_slack(s.Local_Vertex, s.num_vertex, s.face, s.num_faces, s.COM);
+  }
+  const PolyLayout L = poly_layout(t);
+  drop_graphs(c);
+  bdem_poly_shape* d = nullptr;
+  if ((rc = dalloc(c, &d, t.size())) != BDEM_OK) return rc;
+  if (L.stride != c->pb.L.stride) {
+    float4* nr = nullptr;
+    if ((rc = dalloc(c, &nr, (size_t)n * L.stride)) != BDEM_OK) {
+      dfree(c, d);
+      return rc;
+    }
+    dfree(c, c->pb.rot);
+    c->pb.rot = nr;
+  }
+  dfree(c, c->dshape);
+  c->dshape = d;
+  c->pb.L = L;
+  c->shapes.swap(t);
+  c->shape_by_pid = true;
+  /* the current state's radius and mass from its rows */
+  for (int i = 0; i < n; ++i) {
+    const int row = shape_row(c, f_bits(O[i].w));
+    P[i].w = c->shapes[row].radius;
+    V[i].w = c->shapes[row].mass;
+  }
+  HIPCHK(c, hipMemcpy(c->b.pos_cur, P.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->b.vel_cur, V.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  c->keys_valid = false;
+  c->flags_valid = false;
+  c->lists_full = true;
+  return upload_params(c);
+}
+
+int bdem_get_num_poly_shapes(bdem_ctx* c, int* count, int* by_pid) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (count) *count = (int)c->shapes.size();
+  if (by_pid) *by_pid = c->shape_by_pid ? 1 : 0;
+  return BDEM_OK;
+}
+
 int bdem_get_poly_vertices(bdem_ctx* c, int max_vertex, bdem_float3* verts, int* num_vertex, int* pid) {
   int rc = require(c);
   if (rc) return rc;
@@ -949,7 +1090,7 @@ int bdem_get_poly_vertices(bdem_ctx* c, int max_vertex, bdem_float3* verts, int*
   if (max_vertex < L.pv) return fail(c, BDEM_EINVAL, "max_vertex is below the largest vertex count of the types");
   HIPCHK(c, hipSetDevice(c->device));
   const int n = c->b.n;
-  launch_poly_world_shapes(c->dp, c->dpobj, c->b, c->pb, c->stream);
+  launch_poly_world_shapes(c->dp, c->dshape, c->b, c->pb, c->stream);
   HIPCHK(c, hipGetLastError());
   std::vector<float4> R((size_t)n * L.stride), O((size_t)n);
   HIPCHK(c, hipMemcpyAsync(R.data(), c->pb.rot, R.size() * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
@@ -957,8 +1098,8 @@ int bdem_get_poly_vertices(bdem_ctx* c, int max_vertex, bdem_float3* verts, int*
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < n; ++i) {
     const int packed = f_bits(O[i].w);
-    const int t = packed & 7;
-    const int nv = t < (int)c->pobj.size() ? c->pobj[t].num_vertex : 0;
+    const int row = shape_row(c, packed);
+    const int nv = row >= 0 ? c->shapes[row].num_vertex : 0;
     if (num_vertex) num_vertex[i] = nv;
     if (pid) pid[i] = packed >> 3;
     for (int v = 0; v < max_vertex; ++v) {
@@ -1218,7 +1359,7 @@ int bdem_update_sim(bdem_ctx* c) {
   if (ev && sa.on) (void)hipEventRecord(ev[2], c->stream);
   if (ev) c->t_lists[(size_t)c->t_used] = sa.on ? 1 : 0;
   if (c->poly)
-    launch_poly_step(c->dp, c->dpobj, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
+    launch_poly_step(c->dp, c->dshape, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
                      c->record_wf, !c->dobj.empty(), c->stream);
   else
     launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
@@ -1384,7 +1525,7 @@ int bdem_debug_forces(bdem_ctx* c, float* fpp, float* tpp, float* fw, float* tw,
   float* d = nullptr;
   HIPCHK(c, hipMalloc((void**)&d, (size_t)n * 12 * sizeof(float)));
   if (c->poly)
-    launch_poly_forces(c->dp, c->dpobj, c->b, c->pb, d, d + 3 * (size_t)n, d + 6 * (size_t)n, d + 9 * (size_t)n,
+    launch_poly_forces(c->dp, c->dshape, c->b, c->pb, d, d + 3 * (size_t)n, d + 6 * (size_t)n, d + 9 * (size_t)n,
                        c->stream);
   else
     launch_forces(c->dp, c->b, c->hp.law, c->hist_par, d, d + 3 * (size_t)n, d + 6 * (size_t)n,
@@ -1419,6 +1560,17 @@ int bdem_set_skin(bdem_ctx* c, float skin) {
 }
 
 float bdem_get_skin(const bdem_ctx* c) { return (c && c->configured) ? c->b.skin : 0.0f; }
+
+int bdem_debug_poly_queues(bdem_ctx* c, unsigned int* counts) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!counts) return fail(c, BDEM_EINVAL, "null output");
+  if (!c->poly) return fail(c, BDEM_EINVAL, "not a polyhedra context (particle_type 1)");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(counts, c->pb.qcount, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return BDEM_OK;
+}
 
 int bdem_debug_step_contacts(bdem_ctx* c, unsigned int* list32, unsigned int* nn) {
   int rc = require(c);

@@ -48,7 +48,11 @@ struct bdem_ctx {
   bool orient = false;
   double vbound_scale = 1.0;  /* largest |q|^2 (>= 1) of the last orientation upload */
   bool poly = false;         /* particle_type 1 */
-  bdem_POBJ* dpobj = nullptr;  /* device copy of the POBJ table (polyhedra) */
+  /* polyhedra: the shape table the kernels read, one row per type (the
+   * POBJ table's geometry) or, after bdem_set_poly_shapes, one per P_ID */
+  std::vector<bdem_poly_shape> shapes;
+  bool shape_by_pid = false;
+  bdem_poly_shape* dshape = nullptr;
   PolyBuffers pb{};
   bool keys_valid = false;   /* keys_new holds the keys of the current state */
   bool flags_valid = false;  /* ... and the step kernel's changed flags for them */

@@ -108,7 +108,8 @@ struct DevParams {
   float s_kn[MAX_TYPES], s_cn[MAX_TYPES], s_fs[MAX_TYPES], s_fk[MAX_TYPES];
   LawPair law_pp[MAX_TYPES][MAX_TYPES];
   int one_type;               /* one particle type: every pair uses law_pp[0][0] */
-  float t_vbound[MAX_TYPES];  /* polyhedra: largest |Local_Vertex| x 1.0001 (cm) */
+  float vbound_scale;         /* polyhedra: largest |q|^2 (>= 1) of the last orientation upload */
+  int shape_by_pid;           /* polyhedra: shape-table rows are P_IDs (else types) */
   LawPair law_w[MAX_TYPES];
   bdem_cone cone[BDEM_MAX_CONES];
   bdem_WOBJ wobj[MAX_WOBJ];

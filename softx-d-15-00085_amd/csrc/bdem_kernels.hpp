@@ -167,13 +167,13 @@ void launch_skin(const DevParams* dP, const StepBuffers& b, const uint32_t* old_
 
 /* Polyhedra step (rotate -> list + queue -> pairs -> finish), srt -> cur,
  * with the same outputs as launch_step; and the force stage alone. */
-void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+void launch_poly_step(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sorted_keys, int append_par, bool do_kill, bool want_wall_force,
                       bool lifters, hipStream_t s);
 /* rot <- world-frame vertices, normals and centroids of the current state */
-void launch_poly_world_shapes(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+void launch_poly_world_shapes(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                               const PolyBuffers& pb, hipStream_t s);
-void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+void launch_poly_forces(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s);
 
 /* Ensemble statistics of the current state (bdem_stats.hip): live

@@ -12,10 +12,12 @@
  *                   sphere broad phase), plus a queue of (particle, slot)
  *                   work items appended one atomic per wave
  *   k_poly_pairs    one thread per queued pair, grid-stride over the queue:
- *                   the separating-plane test; the pairs it does not
- *                   separate go to a second queue
- *   k_poly_narrow   one wave per pair of that queue: vertex-face both ways
- *                   (a lane per vertex), edge-edge (lanes over edge pairs),
+ *                   the separating-plane test of the two facing faces; the
+ *                   pairs it does not separate go to a second queue
+ *   k_poly_narrow   one wave per pair of that queue, both bodies staged in
+ *                   LDS: the full separating-plane test (a lane per face),
+ *                   vertex-face both ways (a lane per vertex), edge-edge
+ *                   (lanes over the edge pairs a distance bound keeps),
  *                   depth clamps, pair law.  Results go to the pair's own
  *                   slot, so the queue orders do not matter
  *   k_poly_finish   one thread per particle: gravity, then the slots in list
@@ -44,6 +46,15 @@ __device__ __forceinline__ f3 qrot(QuatF Q, QuatF QT, f3 v) {
   return mk3(r.x, r.y, r.z);
 }
 
+/* The shape-table row of a particle: its type (the POBJ table of the deck)
+ * or, with per-particle shapes, its P_ID.  A dead particle (P_ID -1) takes
+ * row 0; nothing reads its shape. */
+__device__ __forceinline__ int shape_index(const DevParams& P, float w) {
+  if (!P.shape_by_pid) return packed_type(w);
+  const int pid = packed_pid(w);
+  return pid > 0 ? pid : 0;
+}
+
 /* A rotated shape in `rot`: vertices, then face normals, then face centroids. */
 struct Shape {
   const float4* V;
@@ -52,7 +63,7 @@ struct Shape {
   int nv, nf;
 };
 __device__ __forceinline__ Shape shape_of(const float4* __restrict__ rot, const PolyLayout& L, int i,
-                                          const bdem_POBJ& po) {
+                                          const bdem_poly_shape& po) {
   const float4* base = rot + (size_t)i * L.stride;
   return Shape{base, base + L.pv, base + L.pv + L.pf, po.num_vertex, po.num_faces};
 }
@@ -89,24 +100,6 @@ __device__ __forceinline__ bool point_on_face(const bdem_Surface& f, const float
 __device__ __forceinline__ bool beyond_by_sphere(float dist, f3 c, f3 PosB, float vb) {
   const float size = fabsf(PosB.x) + fabsf(PosB.y) + fabsf(PosB.z) + fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
   return dist > vb + 0.001f + 4.0e-6f * size;
-}
-
-/* is_SP_Poly_Poly (Utilities.cuh:105-156); vb: B's vertex bound (the sphere
- * shortcut changes which face is found first, never whether one is) */
-__device__ __forceinline__ bool sp_poly_poly(const Shape& A, f3 PosB, const Shape& B, float vb) {
-  for (int f = 0; f < A.nf; ++f) {
-    const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
-    const float dist = dot(n, PosB - c);
-    if (dist > 0.0f) {
-      if (beyond_by_sphere(dist, c, PosB, vb)) return true;
-      bool contact = false;
-      for (int v = 0; v < B.nv; ++v) {
-        if (dot(n, ld3(B.V + v) - c) <= 0.00000f) { contact = true; break; }
-      }
-      if (!contact) return true;
-    }
-  }
-  return false;
 }
 
 /* isPointinPoly (Utilities.cuh:63-98) */
@@ -146,7 +139,7 @@ __device__ __forceinline__ bool vertex_in_poly(f3 PosP, f3 v, const Shape& B, In
 }
 
 /* The ray from PosA through `through` meets face f of B inside its polygon. */
-__device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& B, const bdem_POBJ& pb,
+__device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& B, const bdem_poly_shape& pb,
                                               int f, float tol) {
   if (f < 0) return false;
   f3 Ray = through - PosA;
@@ -170,22 +163,21 @@ __device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ?
  * taken in ascending order, as the serial loop appends them, so the first
  * two vertices, their closest faces and the ordered sum of the first eight
  * are the serial ones, and the selection runs on wave-uniform values. */
-__device__ __forceinline__ PColl vertex_faces_wave(f3 PosA, const Shape& A, const Shape& B, const bdem_POBJ& pb) {
-  const int lane = threadIdx.x & 63;
+/* The selection among the inside vertices of A (bit v of mask: vertex v),
+ * whose closest faces lane off + v holds in t. */
+__device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, const Shape& B,
+                                                     const bdem_poly_shape& pb, uint64_t mask, int off,
+                                                     const InPoly& t) {
   PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
-  InPoly t{-1, -1, -1, 10.0f, 10.0f, 10.0f};
-  bool inside = false;
-  if (lane < A.nv) inside = vertex_in_poly(PosA, ld3(A.V + lane), B, t);
-  uint64_t mask = __ballot(inside);
   const int np_all = __popcll(mask);
   if (np_all == 0) return res;
   const int v0 = __ffsll((unsigned long long)mask) - 1;
   const uint64_t rest = mask & (mask - 1ull);
   const int v1 = rest ? __ffsll((unsigned long long)rest) - 1 : v0;
   InPoly s0;
-  s0.f0 = __shfl(t.f0, v0, 64); s0.f1 = __shfl(t.f1, v0, 64); s0.f2 = __shfl(t.f2, v0, 64);
-  s0.d0 = __shfl(t.d0, v0, 64); s0.d1 = __shfl(t.d1, v0, 64); s0.d2 = __shfl(t.d2, v0, 64);
-  const int s1f1 = __shfl(t.f1, v1, 64);
+  s0.f0 = __shfl(t.f0, v0 + off, 64); s0.f1 = __shfl(t.f1, v0 + off, 64); s0.f2 = __shfl(t.f2, v0 + off, 64);
+  s0.d0 = __shfl(t.d0, v0 + off, 64); s0.d1 = __shfl(t.d1, v0 + off, 64); s0.d2 = __shfl(t.d2, v0 + off, 64);
+  const int s1f1 = __shfl(t.f1, v1 + off, 64);
   const int np = np_all < 8 ? np_all : 8;
   res.hit = true;
   if (np == 1) {
@@ -245,66 +237,170 @@ __device__ __forceinline__ PColl vertex_faces_wave(f3 PosA, const Shape& A, cons
   return res;   /* face logic fall-through: depth 0, zero normal */
 }
 
-/* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707),
- * with a whole wave on one pair: lane l takes the
- * edge pairs t = l, l + 64, ... of the sequence t = i * E_B + j that the
- * serial loop walks.  The serial loop keeps the first strictly smaller
- * distance below 5, i.e. the smallest distance and among equal distances
- * the smallest t: a wave minimum over (distance, t) selects the same pair,
- * whose points are then recomputed by every lane with the same arithmetic.
- * The result is wave-uniform. */
-__device__ __forceinline__ PColl edges_wave(const Shape& A, const bdem_POBJ& pa, const Shape& B, const bdem_POBJ& pb,
-                                            f3 RelPos, f3 PosB) {
+/* Vertices of A in B, and if none, vertices of B in A (normal reversed):
+ * both inside tests run at once, lanes 0..31 on A's vertices, lanes 32..63
+ * on B's (at most 32 vertices each). */
+__device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, const bdem_poly_shape& pa, f3 PosB,
+                                                   const Shape& B, const bdem_poly_shape& pb) {
   const int lane = threadIdx.x & 63;
-  const int ea = pa.num_edges, eb = pb.num_edges;
-  const int total = ea * eb;
+  const bool hb = lane >= 32;
+  const int v = lane & 31;
+  InPoly t{-1, -1, -1, 10.0f, 10.0f, 10.0f};
+  bool inside = false;
+  if (v < (hb ? B.nv : A.nv)) {
+    const Shape& own = hb ? B : A;
+    const Shape& other = hb ? A : B;
+    inside = vertex_in_poly(hb ? PosB : PosA, ld3(own.V + v), other, t);
+  }
+  const uint64_t mask = __ballot(inside);
+  PColl c = vertex_faces_select(PosA, A, B, pb, mask & 0xFFFFFFFFull, 0, t);
+  if (!c.hit) {
+    c = vertex_faces_select(PosB, B, A, pa, mask >> 32, 32, t);
+    c.n = c.n * -1.0f;   /* B into A: normal reversed */
+  }
+  return c;
+}
+
+/* One edge pair of Collision_Detection_Polyhedra_Polyhedra_Edges
+ * (Collision_Detection_Polyhedra.cuh:537-707): true, with the distance of
+ * the closest points, when the closest points of the two lines fall inside
+ * both edges (1 % slack) and apart. */
+__device__ __forceinline__ bool edge_pair_dis(const Shape& A, uint2 ea, const Shape& B, uint2 eb, float& dis) {
+  const f3 PA = ld3(A.V + ea.x);
+  const f3 DA = ld3(A.V + ea.y) - PA;
+  const float DEAD = dot(DA, DA);
+  const f3 PB = ld3(B.V + eb.x);
+  const f3 DB = ld3(B.V + eb.y) - PB;
+  const f3 dir = cross(DA, DB);
+  const float mag = dot(dir, dir);
+  if (!(mag > 0.00000f)) return false;
+  const f3 DV = PA - PB;
+  const float DEBD = dot(DB, DB);
+  const float DEABD = dot(DA, DB);
+  const float DEADAB = dot(DA, DV);
+  const float DEBDAB = dot(DB, DV);
+  const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
+  const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
+  if (!(dA >= -0.010f && dA <= 1.0100f)) return false;
+  const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
+  if (!(dB >= -0.0100f && dB <= 1.010f)) return false;
+  const f3 pA = PA + DA * dA;
+  const f3 pB = PB + DB * dB;
+  const f3 D = (pA - pB);
+  dis = dot(D, D);
+  if (!(dis > 0.00000f)) return false;
+  dis = __builtin_sqrtf(dis);
+  return true;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+/* The face of S whose normal points most along d (lane f tests face f). */
+__device__ __forceinline__ int facing_face_wave(const Shape& S, f3 d) {
+  const int lane = threadIdx.x & 63;
+  const float v = lane < S.nf ? dot(ld3(S.Fn + lane), d) : -3.0e38f;
+  float m = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  const uint64_t b = __ballot(lane < S.nf && v == m);
+  return b ? __ffsll((unsigned long long)b) - 1 : 0;
+}
+
+/* A lower bound of |pA - pB| over every pA of the edge [p0, p1] extended by
+ * 1 % at both ends (the range the search accepts) and every pB the search
+ * can pick on body Q: centre c, vertex bound vb, face plane (n, fc) with
+ * slack sl (no vertex of Q lies more than sl in front of it).  A point of
+ * Q's extended edges is within 1.02 vb of c and at most sl + 0.02 vb in
+ * front of the plane; n may be longer than 1 (a non-unit orientation
+ * quaternion turns it so) and is divided out. */
+__device__ __forceinline__ float edge_bound(f3 p0, f3 p1, f3 c, float vb, f3 n, f3 fc, float sl) {
+  const f3 d = p1 - p0;
+  const f3 e0 = p0 - 0.01f * d, e1 = p1 + 0.01f * d;
+  const float ln = length(n);
+  const float hp = ln > 0.0f ? fminf(dot(n, e0 - fc), dot(n, e1 - fc)) / ln - sl - 0.02f * vb : -3.0e38f;
+  const f3 de = e1 - e0;
+  const float dd = dot(de, de);
+  float u = dd > 0.0f ? dot(c - e0, de) / dd : 0.0f;
+  u = fminf(fmaxf(u, 0.0f), 1.0f);
+  const float hs = length(c - (e0 + de * u)) - 1.02f * vb;
+  return fmaxf(hp, hs);
+}
+
+/* Collision_Detection_Polyhedra_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:537-707),
+ * with a whole wave on one pair.  The serial loop walks t = i * E_B + j and
+ * keeps the first strictly smaller distance below 5: the smallest distance
+ * and among equal distances the smallest t.  Here every edge of A gets a
+ * lower bound la_i of the distance any of its pairs can have (edge_bound
+ * against B's bounding sphere and the face of B turned most towards A), and
+ * every edge of B one against A, lb_j.  Pass 1 evaluates the pairs whose
+ * edges both come within w1 of the other body (the closest edges); its best
+ * distance U then bounds the answer, and pass 2 evaluates every remaining
+ * pair with la_i and lb_j below U plus a rounding margin.  A pair left out
+ * has a distance above U, so it can be neither the minimum nor tie with it;
+ * the lanes keep the smallest (distance, t) of what they evaluate and a wave
+ * minimum selects the serial loop's pair, whose points every lane then
+ * recomputes with the same arithmetic.  The result is wave-uniform. */
+__device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int ea, const Shape& B, const uint2* EB,
+                                            int eb, f3 RelPos, f3 PosA, f3 PosB, float vbA, float vbB, float slA,
+                                            float slB, unsigned char* ia, unsigned char* ib) {
+  const int lane = threadIdx.x & 63;
+  const int fB = facing_face_wave(B, PosA - PosB);
+  const int fA = facing_face_wave(A, PosB - PosA);
+  float la = 3.0e38f, lb = 3.0e38f;
+  if (lane < ea)
+    la = edge_bound(ld3(A.V + EA[lane].x), ld3(A.V + EA[lane].y), PosB, vbB, ld3(B.Fn + fB), ld3(B.Fc + fB), slB);
+  if (lane < eb)
+    lb = edge_bound(ld3(B.V + EB[lane].x), ld3(B.V + EB[lane].y), PosA, vbA, ld3(A.Fn + fA), ld3(A.Fc + fA), slA);
+  const float size = fabsf(PosA.x) + fabsf(PosA.y) + fabsf(PosA.z) + fabsf(PosB.x) + fabsf(PosB.y) + fabsf(PosB.z);
+  const float margin = 0.001f + 4.0e-6f * size;
   float best = 5.0f;
   int bt = 0x7FFFFFFF;
-  for (int t = lane; t < total; t += 64) {
-    const int i = t / eb, j = t - (t / eb) * eb;
-    const f3 PA = ld3(A.V + pa.edge[i].point_vindex[0]);
-    const f3 DA = ld3(A.V + pa.edge[i].point_vindex[1]) - PA;
-    const float DEAD = dot(DA, DA);
-    const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
-    const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
-    const f3 dir = cross(DA, DB);
-    const float mag = dot(dir, dir);
-    if (!(mag > 0.00000f)) continue;
-    const f3 DV = PA - PB;
-    const float DEBD = dot(DB, DB);
-    const float DEABD = dot(DA, DB);
-    const float DEADAB = dot(DA, DV);
-    const float DEBDAB = dot(DB, DV);
-    const float detJinv = 1.00000f / (DEABD * DEABD - DEAD * DEBD);
-    const float dA = detJinv * (DEBD * DEADAB - DEABD * DEBDAB);
-    if (!(dA >= -0.010f && dA <= 1.0100f)) continue;
-    const float dB = detJinv * (DEABD * DEADAB - DEAD * DEBDAB);
-    if (!(dB >= -0.0100f && dB <= 1.010f)) continue;
-    const f3 pA = PA + DA * dA;
-    const f3 pB = PB + DB * dB;
-    const f3 D = (pA - pB);
-    float dis = dot(D, D);
-    if (dis > 0.00000f) {
-      dis = __builtin_sqrtf(dis);
-      if (dis < best) { best = dis; bt = t; }   /* t grows along the lane's walk: first wins */
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  /* the pairs of SA x SB not in XA x XB, from compacted index lists */
+  auto sweep = [&](uint64_t SA, uint64_t SB, uint64_t XA, uint64_t XB) {
+    __builtin_amdgcn_wave_barrier();
+    if ((SA >> lane) & 1ull) ia[__popcll(SA & lt)] = (unsigned char)lane;
+    if ((SB >> lane) & 1ull) ib[__popcll(SB & lt)] = (unsigned char)lane;
+    __builtin_amdgcn_wave_barrier();
+    const int na = __popcll(SA), nb = __popcll(SB);
+    const int total = na * nb;
+    int a = nb > 0 ? lane / nb : 0, b = lane - a * nb;
+    for (int u = lane; u < total; u += 64) {
+      const int i = ia[a], j = ib[b];
+      b += 64;
+      while (b >= nb) { b -= nb; ++a; }
+      if (((XA >> i) & 1ull) && ((XB >> j) & 1ull)) continue;   /* evaluated in pass 1 */
+      float dis;
+      if (edge_pair_dis(A, EA[i], B, EB[j], dis) && dis < 5.0f) {
+        const int t = i * eb + j;
+        if (dis < best || (dis == best && t < bt)) { best = dis; bt = t; }
+      }
     }
-  }
+  };
+  const float t0 = fmaxf(wave_min(la), wave_min(lb));
+  const float w1 = fmaxf(t0, 0.0f) + 0.05f * fminf(vbA, vbB) + margin;
+  const uint64_t SA1 = __ballot(lane < ea && la <= w1), SB1 = __ballot(lane < eb && lb <= w1);
+  sweep(SA1, SB1, 0ull, 0ull);
+  const float w2 = wave_min(best) + margin;
+  sweep(__ballot(lane < ea && la <= w2), __ballot(lane < eb && lb <= w2), SA1, SB1);
   /* wave minimum of (distance, t) */
-  float m = best;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+  const float m = wave_min(best);
   int tm = (best == m) ? bt : 0x7FFFFFFF;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) tm = min(tm, __shfl_xor(tm, o, 64));
   PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
   if (!(m < 5.0f) || tm == 0x7FFFFFFF) return res;
   /* the selected pair, recomputed by every lane */
-  const int i = tm / eb, j = tm - (tm / eb) * eb;
-  const f3 PA = ld3(A.V + pa.edge[i].point_vindex[0]);
-  const f3 DA = ld3(A.V + pa.edge[i].point_vindex[1]) - PA;
+  const int si = tm / eb, sj = tm - (tm / eb) * eb;
+  const f3 PA = ld3(A.V + EA[si].x);
+  const f3 DA = ld3(A.V + EA[si].y) - PA;
   const float DEAD = dot(DA, DA);
-  const f3 PB = ld3(B.V + pb.edge[j].point_vindex[0]);
-  const f3 DB = ld3(B.V + pb.edge[j].point_vindex[1]) - PB;
+  const f3 PB = ld3(B.V + EB[sj].x);
+  const f3 DB = ld3(B.V + EB[sj].y) - PB;
   const f3 DV = PA - PB;
   const float DEBD = dot(DB, DB);
   const float DEABD = dot(DA, DB);
@@ -336,7 +432,9 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const bdem_POBJ& pa,
 }
 
 /* Face f of A separates B: B's centre and every vertex of B strictly in
- * front (the per-face test of is_SP_Poly_Poly). */
+ * front (the per-face test of is_SP_Poly_Poly, Utilities.cuh:105-156; vb is
+ * B's vertex bound: the sphere shortcut changes which face is found first,
+ * never whether one is). */
 __device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb) {
   const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
   const float dist = dot(n, PosB - c);
@@ -362,16 +460,30 @@ __device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
  * starts with the separating-plane test both ways (is_SP_Poly_Poly): true
  * when some face of one body separates them.  Which face is found does not
  * matter, only whether one exists, so each body's face turned most towards
- * the other is tried first: it separates most separated pairs, and the
- * full loops run only when it does not. */
-__device__ __forceinline__ bool pair_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
-                                               float vbB) {
+ * the other is tried first (quick_separated, one thread per pair): it
+ * separates most separated pairs.  The rest go to the wave-per-pair kernel,
+ * where separated_wave tests every face at once. */
+__device__ __forceinline__ bool quick_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
+                                                float vbB) {
   const f3 dAB = PosB - PosA;
   if (face_separates(A, facing_face(A, dAB), PosB, B, vbB)) return true;
-  if (face_separates(B, facing_face(B, -1.0f * dAB), PosA, A, vbA)) return true;
-  bool sp = sp_poly_poly(A, PosB, B, vbB);
-  if (!sp) sp = sp_poly_poly(B, PosA, A, vbA);
-  return sp;
+  return face_separates(B, facing_face(B, -1.0f * dAB), PosA, A, vbA);
+}
+
+/* is_SP_Poly_Poly both ways with a lane per face: lanes 0..31 test A's
+ * faces against B, lanes 32..63 B's faces against A (at most 32 faces). */
+__device__ __forceinline__ bool separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
+                                               float vbB) {
+  const int lane = threadIdx.x & 63;
+  const bool hb = lane >= 32;
+  const int f = lane & 31;
+  bool sep = false;
+  if (f < (hb ? B.nf : A.nf)) {
+    const Shape& own = hb ? B : A;
+    const Shape& other = hb ? A : B;
+    sep = face_separates(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB);
+  }
+  return __ballot(sep) != 0ull;
 }
 
 /* The depth clamps (DEM_Kernels_Compute.cuh:918-950). */
@@ -397,11 +509,11 @@ __device__ __forceinline__ void depth_clamps(PColl& res, float rA, f3 PosA, f3 P
 
 /* A queued pair's operands. */
 struct PairOps {
-  int a, b, ta, tb;
+  int a, b, ta, tb, sa, sb;   /* types (materials) and shape-table rows */
   f3 PosA, PosB, va, vb, wa, wb;
   float ma, mb;
 };
-__device__ __forceinline__ PairOps pair_ops(uint32_t slot, const uint32_t* __restrict__ list32,
+__device__ __forceinline__ PairOps pair_ops(const DevParams& P, uint32_t slot, const uint32_t* __restrict__ list32,
                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
                                             const float4* __restrict__ omg) {
   PairOps o;
@@ -410,6 +522,8 @@ __device__ __forceinline__ PairOps pair_ops(uint32_t slot, const uint32_t* __res
   const float4 oa = omg[o.a], ob = omg[o.b];
   o.ta = packed_type(oa.w);
   o.tb = packed_type(ob.w);
+  o.sa = shape_index(P, oa.w);
+  o.sb = shape_index(P, ob.w);
   const float4 va4 = vel[o.a], vb4 = vel[o.b];
   o.PosA = xyz(pos[o.a]);
   o.PosB = xyz(pos[o.b]);
@@ -420,7 +534,7 @@ __device__ __forceinline__ PairOps pair_ops(uint32_t slot, const uint32_t* __res
 }
 
 /* Clamps, then the reference pair law into the pair's slot. */
-__device__ __forceinline__ void pair_store(const DevParams& P, const bdem_POBJ& pa, const PairOps& o, PColl c,
+__device__ __forceinline__ void pair_store(const DevParams& P, const bdem_poly_shape& pa, const PairOps& o, PColl c,
                                            uint32_t slot, float4* __restrict__ pf, float4* __restrict__ pt) {
   float4 F4 = make_float4(0.f, 0.f, 0.f, 0.f), T4 = F4;
   if (c.hit) {
@@ -766,7 +880,7 @@ __device__ __forceinline__ PColl vertex_faces_dobj(f3 PosP, const Shape& A, cons
 /* Collision_Detection_DObject_Polyhedra_Edges (Collision_Detection_Polyhedra.cuh:716-890);
  * no crossing within 0.1 cm of the other point (the reference then reads an
  * unset normal) is no contact. */
-__device__ __forceinline__ PColl edges_dobj(const bdem_DOBJ& D, const Shape& B, const bdem_POBJ& pb, f3 PosB,
+__device__ __forceinline__ PColl edges_dobj(const bdem_DOBJ& D, const Shape& B, const bdem_poly_shape& pb, f3 PosB,
                                             f3 RelPos) {
   PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
   float min_edgedis = 0.10f;
@@ -832,7 +946,7 @@ __device__ __forceinline__ PColl edges_dobj(const bdem_DOBJ& D, const Shape& B, 
 
 /* Lifter force and torque on a polyhedron: the first lifter in contact acts. */
 __device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
-                                                   const Shape& A, const bdem_POBJ& pa, f3& F, f3& T) {
+                                                   const Shape& A, const bdem_poly_shape& pa, f3& F, f3& T) {
   F = mk3(0.f, 0.f, 0.f);
   T = F;
   const float r = pa.radius;
@@ -872,7 +986,7 @@ __device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3
 
 /* ------------------------------------------------------------------ kernels */
 __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict__ Pp,
-                                                     const bdem_POBJ* __restrict__ pobj, int n,
+                                                     const bdem_poly_shape* __restrict__ shapes, int n,
                                                      PolyLayout L, const float4* __restrict__ pos,
                                                      const float4* __restrict__ omg,
                                                      const float4* __restrict__ quat,
@@ -883,7 +997,7 @@ __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict
   const size_t i = t / (size_t)L.stride;
   const int e = (int)(t - i * (size_t)L.stride);
   if (i >= (size_t)n) return;
-  const bdem_POBJ& po = pobj[packed_type(omg[i].w)];
+  const bdem_poly_shape& po = shapes[shape_index(P, omg[i].w)];
   const f3 c = xyz(pos[i]);
   const float4 q4 = quat[i];
   const QuatF Q{q4.x, q4.y, q4.z, q4.w};
@@ -946,12 +1060,13 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
 }
 
 /* One queued pair per thread, grid-stride so that every wave exits once the
- * queue (at most n * 32 entries) is drained: the separating-plane test.
- * The few pairs it does not separate (about 1 in 60 in a dense pack) go to
- * a second queue, where a wave per pair runs the long vertex-face and
- * edge-edge searches. */
+ * queue (at most n * 32 entries) is drained: the separating-plane test of
+ * each body's face turned most towards the other.  The pairs it does not
+ * separate (about 1 in 5 in a dense pack of hulls) go to a second queue,
+ * where a wave per pair runs the full separating-plane test, then the
+ * vertex-face and edge-edge searches. */
 __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict__ Pp,
-                                                    const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+                                                    const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
                                                     const float4* __restrict__ rot,
                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                     const float4* __restrict__ omg,
@@ -973,12 +1088,12 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
     uint32_t slot = 0u;
     if (q < cnt) {
       slot = queue[q];
-      const PairOps o = pair_ops(slot, list32, pos, vel, omg);
-      const bdem_POBJ& pa = pobj[o.ta];
-      const bdem_POBJ& pb = pobj[o.tb];
+      const PairOps o = pair_ops(P, slot, list32, pos, vel, omg);
+      const bdem_poly_shape& pa = shapes[o.sa];
+      const bdem_poly_shape& pb = shapes[o.sb];
       const Shape A = shape_of(rot, L, o.a, pa);
       const Shape B = shape_of(rot, L, o.b, pb);
-      need = !pair_separated(A, B, o.PosA, o.PosB, P.t_vbound[o.ta], P.t_vbound[o.tb]);
+      need = !quick_separated(A, B, o.PosA, o.PosB, pa.vbound * P.vbound_scale, pb.vbound * P.vbound_scale);
       if (!need) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
     }
     const uint64_t mask = __ballot(need);
@@ -994,12 +1109,24 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
   }
 }
 
+/* One pair's bodies staged in LDS by its wave: turned vertices, face
+ * normals and centroids, and edge lists of A (row 0) and B (row 1). */
+struct PairStage {
+  float4 v[2][BDEM_SHAPE_MAX_VERTEX];
+  float4 fn[2][BDEM_SHAPE_MAX_FACE];
+  float4 fc[2][BDEM_SHAPE_MAX_FACE];
+  uint2 e[2][BDEM_SHAPE_MAX_EDGE];
+  unsigned char ia[BDEM_SHAPE_MAX_EDGE], ib[BDEM_SHAPE_MAX_EDGE];   /* edge lists of the pruned search */
+};
+
 /* The pairs the first pass could not separate, one wave per pair,
  * grid-stride over the queue by waves: vertices of A in B, vertices of B in
  * A (normal reversed), then edge-edge with its depth limited to 0.025 rA
- * (DEM_Kernels_Compute.cuh:852-904). */
+ * (DEM_Kernels_Compute.cuh:852-904).  The wave first copies both bodies
+ * into its LDS stage with one load per lane and row, so the searches read
+ * LDS instead of chains of dependent global loads. */
 __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict__ Pp,
-                                                     const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+                                                     const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
                                                      const float4* __restrict__ rot,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const float4* __restrict__ omg,
@@ -1008,28 +1135,45 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
                                                      const uint32_t* __restrict__ equeue,
                                                      float4* __restrict__ pf, float4* __restrict__ pt) {
   const DevParams& P = *Pp;
+  __shared__ PairStage stage[4];
+  PairStage& S = stage[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
   const uint32_t cnt = qcount[1] < cap ? qcount[1] : cap;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t q = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); q < cnt; q += waves) {
     const uint32_t slot = equeue[q];
     if (slot >= cap) continue;   /* wave-uniform */
-    const PairOps o = pair_ops(slot, list32, pos, vel, omg);
-    const bdem_POBJ& pa = pobj[o.ta];
-    const bdem_POBJ& pb = pobj[o.tb];
-    const Shape A = shape_of(rot, L, o.a, pa);
-    const Shape B = shape_of(rot, L, o.b, pb);
-    PColl c = vertex_faces_wave(o.PosA, A, B, pb);
-    if (!c.hit) {
-      c = vertex_faces_wave(o.PosB, B, A, pa);
-      c.n = c.n * -1.0f;   /* B into A: normal reversed */
+    const PairOps o = pair_ops(P, slot, list32, pos, vel, omg);
+    const bdem_poly_shape& pa = shapes[o.sa];
+    const bdem_poly_shape& pb = shapes[o.sb];
+    const Shape Ag = shape_of(rot, L, o.a, pa);
+    const Shape Bg = shape_of(rot, L, o.b, pb);
+    const int ea = pa.num_edges, eb = pb.num_edges;
+    /* the previous pair's reads are done before the stage is overwritten */
+    __builtin_amdgcn_wave_barrier();
+    if (lane < Ag.nv) S.v[0][lane] = Ag.V[lane];
+    if (lane < Bg.nv) S.v[1][lane] = Bg.V[lane];
+    if (lane < Ag.nf) { S.fn[0][lane] = Ag.Fn[lane]; S.fc[0][lane] = Ag.Fc[lane]; }
+    if (lane < Bg.nf) { S.fn[1][lane] = Bg.Fn[lane]; S.fc[1][lane] = Bg.Fc[lane]; }
+    if (lane < ea) S.e[0][lane] = make_uint2(pa.edge[lane].point_vindex[0], pa.edge[lane].point_vindex[1]);
+    if (lane < eb) S.e[1][lane] = make_uint2(pb.edge[lane].point_vindex[0], pb.edge[lane].point_vindex[1]);
+    __builtin_amdgcn_wave_barrier();
+    const Shape A{S.v[0], S.fn[0], S.fc[0], Ag.nv, Ag.nf};
+    const Shape B{S.v[1], S.fn[1], S.fc[1], Bg.nv, Bg.nf};
+    const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
+    if (separated_wave(A, B, o.PosA, o.PosB, vbA, vbB)) {
+      if (lane == 0) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
+      continue;
     }
+    PColl c = vertex_faces_pair(o.PosA, A, pa, o.PosB, B, pb);
     if (!c.hit) {
       const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
-      c = edges_wave(A, pa, B, pb, RelPos, o.PosB);
+      c = edges_wave(A, S.e[0], ea, B, S.e[1], eb, RelPos, o.PosA, o.PosB, vbA, vbB,
+                     pa.plane_slack * P.vbound_scale, pb.plane_slack * P.vbound_scale, S.ia, S.ib);
       if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
     }
-    if ((threadIdx.x & 63) == 0) pair_store(P, pa, o, c, slot, pf, pt);
+    if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt);
   }
 }
 
@@ -1038,7 +1182,7 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
  * FORCES_ONLY stops after the forces and writes them (debug stage). */
 template <bool FORCES_ONLY, bool LIFTERS>
 __global__ __launch_bounds__(256) void k_poly_finish(
-    const DevParams* __restrict__ Pp, const bdem_POBJ* __restrict__ pobj, int n, PolyLayout L,
+    const DevParams* __restrict__ Pp, const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
     const float4* __restrict__ rot, const uint32_t* __restrict__ nn, const float4* __restrict__ pf,
     const float4* __restrict__ pt, const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
     const float4* __restrict__ omg_in, const float4* __restrict__ quat_in, float4* __restrict__ pos_out,
@@ -1061,7 +1205,7 @@ __global__ __launch_bounds__(256) void k_poly_finish(
     const float r = pr.w, m = vm.w;
     QuatF q{q4.x, q4.y, q4.z, q4.w};
     if (pid >= 0) {
-      const bdem_POBJ& po = pobj[ta];
+      const bdem_poly_shape& po = shapes[shape_index(P, ot.w)];
       f3 Fpp = cvt(P.gravity) * m, Tpp = mk3(0.f, 0.f, 0.f);
       const uint32_t c = nn[i];
       const int cnt = c < (uint32_t)LIST_STRIDE ? (int)c : LIST_STRIDE;
@@ -1135,46 +1279,46 @@ __global__ __launch_bounds__(256) void k_poly_finish(
 }
 
 /* ------------------------------------------------------------------ launchers */
-static void launch_poly_forces_common(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                                       const PolyBuffers& pb, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
-  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dpobj,
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dshape,
                      b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
   hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
-  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dpobj, b.n, pb.L, pb.rot,
+  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt);
-  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dpobj, b.n, pb.L, pb.rot,
+  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.qcount, pb.equeue, pb.pf, pb.pt);
 }
 
 /* World-frame shapes of the current (unsorted) state into pb.rot: the
  * point-cloud output, between steps (the next step rewrites pb.rot). */
-void launch_poly_world_shapes(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b,
+void launch_poly_world_shapes(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                               const PolyBuffers& pb, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
   if (rot_threads == 0) return;
-  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dpobj,
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dshape,
                      b.n, pb.L, b.pos_cur, b.omg_cur, b.quat_cur, pb.rot, pb.qcount);
 }
 
-void launch_poly_step(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+void launch_poly_step(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sk, int ap, bool do_kill, bool want_wf, bool lifters, hipStream_t s) {
-  launch_poly_forces_common(dP, dpobj, b, pb, s);
+  launch_poly_forces_common(dP, dshape, b, pb, s);
   /* the lifter search needs private arrays (scratch): only its own instantiation carries it */
   auto k = lifters ? k_poly_finish<false, true> : k_poly_finish<false, false>;
-  hipLaunchKernelGGL(k, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+  hipLaunchKernelGGL(k, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dshape, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur,
                      b.vel_cur, b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf,
                      b.meta + META_APPEND + ap, b.hwc_srt, b.hwc_cur, want_wf ? b.wall_force : nullptr,
                      do_kill ? 1 : 0, b.status, b.kill_count, nullptr, nullptr, nullptr, nullptr);
 }
 
-void launch_poly_forces(const DevParams* dP, const bdem_POBJ* dpobj, const StepBuffers& b, const PolyBuffers& pb,
+void launch_poly_forces(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s) {
-  launch_poly_forces_common(dP, dpobj, b, pb, s);
-  hipLaunchKernelGGL((k_poly_finish<true, false>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dpobj, b.n, pb.L,
+  launch_poly_forces_common(dP, dshape, b, pb, s);
+  hipLaunchKernelGGL((k_poly_finish<true, false>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dshape, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, nullptr, nullptr,
                      nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                      b.status, b.kill_count, fpp, tpp, fw, tw);

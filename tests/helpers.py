@@ -11,6 +11,8 @@ def make_device(deck):
         dev.set_ext(deck["ext"])
     if deck.get("radius") is not None:
         dev.set_radii(deck["radius"], deck["mass"])
+    if deck.get("shapes") is not None:
+        dev.set_poly_shapes(deck["shapes"])
     dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"],
                   quat=deck.get("quat"))
     return dev
@@ -35,11 +37,14 @@ def by_pid(state):
 
 def oracle_positions_out(deck, state):
     """The positions the library returns for the oracle's state: for
-    polyhedra, position - COM of the type in float (Get_SystemState,
-    GPU_Memory_Transfer.cuh:504)."""
+    polyhedra, position - COM of the type (or of the particle's own shape) in
+    float (Get_SystemState, GPU_Memory_Transfer.cuh:504)."""
     pos = np.asarray(state["pos"], np.float32)
     if deck["sim"].particle_type != 1:
         return pos.astype(np.float64)
+    if deck.get("shapes") is not None:   # per-particle shapes: the COM of each P_ID's row
+        com = np.array([[s.COM.x, s.COM.y, s.COM.z] for s in deck["shapes"]], np.float32)
+        return (pos - com[np.maximum(state["pid"], 0)]).astype(np.float64)
     com = np.array([[p.COM.x, p.COM.y, p.COM.z] for p in deck["pobj"]], np.float32)
     return (pos - com[state["ptype"]]).astype(np.float64)
 

@@ -30,6 +30,8 @@ def test_struct_sizes_match_reference_layout():
     assert T.POBJ.PP.offset == 3268
     assert T.WOBJ.cyl_geo.offset == 1296
     assert ctypes.sizeof(T.BatchTuning) == 20
+    assert ctypes.sizeof(T.PolyShape) == 3148
+    assert T.PolyShape.face.offset == 928
 
 
 def test_library_exports_every_declared_entry_point():
