@@ -577,7 +577,15 @@ def run_hullmill(args):
     deck = S.hull_mill_deck(n)
     dev = bdem.Device(deck["wobj"], deck["pobj"], deck["sim"], dobj=deck["dobj"])
     dev.set_poly_shapes(deck["shapes"])
-    dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"], quat=deck["quat"])
+    com = np.array([[s_.COM.x, s_.COM.y, s_.COM.z] for s_ in deck["shapes"]], np.float32)
+    if args.load_state:   # a state written by --save-state (positions as returned: centre - COM)
+        z = np.load(args.load_state)
+        dev.set_state(z["pos"] + com[np.maximum(z["pid"], 0)], z["ptype"], z["pid"], vel=z["vel"], avel=z["avel"],
+                      quat=z["quat"])
+        log(f"[bench] loaded state {args.load_state}")
+    else:
+        dev.set_state(deck["pos"], deck["ptype"], deck["pid"], vel=deck["vel"], avel=deck["avel"],
+                      quat=deck["quat"])
     dev.drum_rotation(True)
     ncells = dev.num_cells()
     log(f"[bench] hullmill n={n} cells={ncells} rpm {deck['rpm']:.2f} setup {time.time() - t0:.1f}s")
@@ -597,6 +605,11 @@ def run_hullmill(args):
                 f"max {st['max_speed']:.3f} m/s, status {dev.status()}")
         sim.InitalDelta_t = float(dt)
         dev.update_sim_info(sim)
+    if args.save_state:
+        st = dev.get_state()
+        np.savez(args.save_state, pos=st["pos"], vel=st["vel"], avel=st["avel"], quat=st["quat"], ptype=st["ptype"],
+                 pid=st["pid"])
+        log(f"[bench] saved state to {args.save_state}")
     dev.set_lifter_rotation(deck["rad_per_step"], deck["lifter_about"])
     _, nn = dev.debug_contacts()
     k = float(nn.mean())

@@ -68,7 +68,7 @@ struct PolyBuffers {
   uint32_t* list;                      /* n * 32 partner slots */
   uint32_t* nn;                        /* full contact count per particle */
   uint32_t* queue;                     /* (particle * 32 + k) work items */
-  uint32_t* equeue;                    /* the items that need the edge-edge search */
+  uint2* equeue;                       /* (slot, partner) of the pairs for the wave-per-pair kernel */
   uint32_t* qcount;                    /* [0] queue length, [1] edge queue length */
   float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
 };

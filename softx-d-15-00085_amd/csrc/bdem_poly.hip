@@ -55,12 +55,17 @@ __device__ __forceinline__ int shape_index(const DevParams& P, float w) {
   return pid > 0 ? pid : 0;
 }
 
-/* A rotated shape in `rot`: vertices, then face normals, then face centroids. */
+/* A rotated shape in `rot`: vertices, then face normals, then face centroids.
+ * The wave-per-pair kernel adds the face polygons from its LDS stage (vertex
+ * orders packed four to a word, areas, vertex counts). */
 struct Shape {
   const float4* V;
   const float4* Fn;
   const float4* Fc;
   int nv, nf;
+  const uint2* vo = nullptr;
+  const float* area = nullptr;
+  const int* fnv = nullptr;
 };
 __device__ __forceinline__ Shape shape_of(const float4* __restrict__ rot, const PolyLayout& L, int i,
                                           const bdem_poly_shape& po) {
@@ -75,20 +80,26 @@ __device__ __forceinline__ f3 fcent(const Shape& B, int f) { return f >= 0 ? ld3
 enum { C_UNDEF = 0, C_VF = 1, C_EF = 2, C_FF = 3, C_EE = 4, C_SP = 5 };
 struct PColl { bool hit; int cls; float dis; f3 n; f3 cp; };
 
-/* is_PointOnSurface over a polyhedron's rotated vertices (Utilities.cuh:20-53) */
-__device__ __forceinline__ bool point_on_face(const bdem_Surface& f, const float4* __restrict__ V, f3 point,
+/* Vertex q of a face's packed vertex order. */
+__device__ __forceinline__ int vo_at(uint2 vo, int q) {
+  const uint32_t w = q < 4 ? vo.x : vo.y;
+  return (int)((w >> (8 * (q & 3))) & 0xFFu);
+}
+
+/* is_PointOnSurface over a polyhedron's rotated vertices (Utilities.cuh:20-53):
+ * face of nv vertices in order vo, area farea. */
+__device__ __forceinline__ bool point_on_face(int nv, uint2 vo, float farea, const float4* __restrict__ V, f3 point,
                                               float tol) {
   float area = 0.000000f;
-  const int nv = f.num_vertex;
   for (int q = 0; q < nv - 1; ++q) {
-    const f3 V1 = point - ld3(V + f.vertex_Order[q]);
-    const f3 V2 = point - ld3(V + f.vertex_Order[q + 1]);
+    const f3 V1 = point - ld3(V + vo_at(vo, q));
+    const f3 V2 = point - ld3(V + vo_at(vo, q + 1));
     area = area + 0.500000f * length(cross(V1, V2));
   }
-  const f3 V1 = point - ld3(V + f.vertex_Order[nv - 1]);
-  const f3 V2 = point - ld3(V + f.vertex_Order[0]);
+  const f3 V1 = point - ld3(V + vo_at(vo, nv - 1));
+  const f3 V2 = point - ld3(V + vo_at(vo, 0));
   area = area + 0.500000f * length(cross(V1, V2));
-  return (fabsf(area - f.area) / f.area) * 100.0000f <= tol;
+  return (fabsf(area - farea) / farea) * 100.0000f <= tol;
 }
 
 /* B's centre farther beyond a face plane of A than B's vertex bound vb (any
@@ -139,8 +150,7 @@ __device__ __forceinline__ bool vertex_in_poly(f3 PosP, f3 v, const Shape& B, In
 }
 
 /* The ray from PosA through `through` meets face f of B inside its polygon. */
-__device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& B, const bdem_poly_shape& pb,
-                                              int f, float tol) {
+__device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& B, int f, float tol) {
   if (f < 0) return false;
   f3 Ray = through - PosA;
   Ray = Ray / length(Ray);
@@ -148,7 +158,7 @@ __device__ __forceinline__ bool ray_hits_face(f3 PosA, f3 through, const Shape& 
   const float dir = dot(Ray, n);
   if (!(fabsf(dir) > 0.0f)) return false;
   const float d = dot((ld3(B.Fc + f) - PosA), n) / dir;
-  return point_on_face(pb.face[f], B.V, PosA + d * Ray, tol);
+  return point_on_face(B.fnv[f], B.vo[f], B.area[f], B.V, PosA + d * Ray, tol);
 }
 
 __device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ? s.f0 : (q == 1 ? s.f1 : s.f2); }
@@ -165,8 +175,7 @@ __device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ?
  * are the serial ones, and the selection runs on wave-uniform values. */
 /* The selection among the inside vertices of A (bit v of mask: vertex v),
  * whose closest faces lane off + v holds in t. */
-__device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, const Shape& B,
-                                                     const bdem_poly_shape& pb, uint64_t mask, int off,
+__device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, const Shape& B, uint64_t mask, int off,
                                                      const InPoly& t) {
   PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
   const int np_all = __popcll(mask);
@@ -186,13 +195,13 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
     res.n = fnorm(B, s0.f0);
     res.cp = ld3(A.V + v0);
     if (s0.f1 == -1) return res;
-    if (ray_hits_face(PosA, res.cp, B, pb, s0.f0, 0.10f)) return res;
-    if (s0.f1 > -1 && ray_hits_face(PosA, res.cp, B, pb, s0.f1, 0.10f)) {
+    if (ray_hits_face(PosA, res.cp, B, s0.f0, 0.10f)) return res;
+    if (s0.f1 > -1 && ray_hits_face(PosA, res.cp, B, s0.f1, 0.10f)) {
       res.n = fnorm(B, s0.f1);
       res.dis = s0.d1;
       return res;
     }
-    if (s0.f2 > -1 && ray_hits_face(PosA, res.cp, B, pb, s0.f2, 0.10f)) {
+    if (s0.f2 > -1 && ray_hits_face(PosA, res.cp, B, s0.f2, 0.10f)) {
       res.n = fnorm(B, s0.f2);
       res.dis = s0.d2;
       return res;
@@ -208,7 +217,7 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
     for (int q = 0; q < 3; ++q) {
       const int f = face_at(s0, q);
       if (q > 0 && !(f > -1)) continue;
-      if (ray_hits_face(PosA, res.cp, B, pb, f, 0.10f)) {
+      if (ray_hits_face(PosA, res.cp, B, f, 0.10f)) {
         res.dis = fabsf(dot(fnorm(B, f), (res.cp - fcent(B, f))));
         res.n = fnorm(B, f);
         return res;
@@ -228,7 +237,7 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
   for (int q = 0; q < 3; ++q) {
     const int f = face_at(s0, q);
     if (q > 0 && !(f > -1)) continue;
-    if (ray_hits_face(PosA, res.cp, B, pb, f, q == 1 ? 0.010f : 0.10f)) {
+    if (ray_hits_face(PosA, res.cp, B, f, q == 1 ? 0.010f : 0.10f)) {
       res.dis = fabsf(dot(fnorm(B, f), (res.cp - fcent(B, f))));
       res.n = fnorm(B, f);
       return res;
@@ -240,8 +249,7 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
 /* Vertices of A in B, and if none, vertices of B in A (normal reversed):
  * both inside tests run at once, lanes 0..31 on A's vertices, lanes 32..63
  * on B's (at most 32 vertices each). */
-__device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, const bdem_poly_shape& pa, f3 PosB,
-                                                   const Shape& B, const bdem_poly_shape& pb) {
+__device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, f3 PosB, const Shape& B) {
   const int lane = threadIdx.x & 63;
   const bool hb = lane >= 32;
   const int v = lane & 31;
@@ -253,9 +261,9 @@ __device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, cons
     inside = vertex_in_poly(hb ? PosB : PosA, ld3(own.V + v), other, t);
   }
   const uint64_t mask = __ballot(inside);
-  PColl c = vertex_faces_select(PosA, A, B, pb, mask & 0xFFFFFFFFull, 0, t);
+  PColl c = vertex_faces_select(PosA, A, B, mask & 0xFFFFFFFFull, 0, t);
   if (!c.hit) {
-    c = vertex_faces_select(PosB, B, A, pa, mask >> 32, 32, t);
+    c = vertex_faces_select(PosB, B, A, mask >> 32, 32, t);
     c.n = c.n * -1.0f;   /* B into A: normal reversed */
   }
   return c;
@@ -435,13 +443,28 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
  * front (the per-face test of is_SP_Poly_Poly, Utilities.cuh:105-156; vb is
  * B's vertex bound: the sphere shortcut changes which face is found first,
  * never whether one is). */
+template <bool BATCHED = true>
 __device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb) {
   const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
   const float dist = dot(n, PosB - c);
   if (!(dist > 0.0f)) return false;
   if (beyond_by_sphere(dist, c, PosB, vb)) return true;
-  for (int v = 0; v < B.nv; ++v)
-    if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
+  if (!BATCHED) {   /* shapes staged in LDS: one vertex at a time costs fewer instructions */
+    for (int v = 0; v < B.nv; ++v)
+      if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
+    return true;
+  }
+  /* four vertices per round, their loads in flight together (indices past
+   * the count re-read the last vertex and are not tested) */
+  const int last = B.nv - 1;
+  for (int v = 0; v < B.nv; v += 4) {
+    const f3 p0 = ld3(B.V + v), p1 = ld3(B.V + min(v + 1, last));
+    const f3 p2 = ld3(B.V + min(v + 2, last)), p3 = ld3(B.V + min(v + 3, last));
+    if (dot(n, p0 - c) <= 0.00000f) return false;
+    if (v + 1 < B.nv && dot(n, p1 - c) <= 0.00000f) return false;
+    if (v + 2 < B.nv && dot(n, p2 - c) <= 0.00000f) return false;
+    if (v + 3 < B.nv && dot(n, p3 - c) <= 0.00000f) return false;
+  }
   return true;
 }
 
@@ -449,9 +472,16 @@ __device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, c
 __device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
   int best = 0;
   float bv = -3.0e38f;
-  for (int f = 0; f < A.nf; ++f) {
-    const float d = dot(ld3(A.Fn + f), dirAB);
-    if (d > bv) { bv = d; best = f; }
+  const int last = A.nf - 1;
+  for (int f = 0; f < A.nf; f += 4) {   /* four normals per round, as above */
+    const float d0 = dot(ld3(A.Fn + f), dirAB);
+    const float d1 = dot(ld3(A.Fn + min(f + 1, last)), dirAB);
+    const float d2 = dot(ld3(A.Fn + min(f + 2, last)), dirAB);
+    const float d3 = dot(ld3(A.Fn + min(f + 3, last)), dirAB);
+    if (d0 > bv) { bv = d0; best = f; }
+    if (f + 1 < A.nf && d1 > bv) { bv = d1; best = f + 1; }
+    if (f + 2 < A.nf && d2 > bv) { bv = d2; best = f + 2; }
+    if (f + 3 < A.nf && d3 > bv) { bv = d3; best = f + 3; }
   }
   return best;
 }
@@ -481,7 +511,7 @@ __device__ __forceinline__ bool separated_wave(const Shape& A, const Shape& B, f
   if (f < (hb ? B.nf : A.nf)) {
     const Shape& own = hb ? B : A;
     const Shape& other = hb ? A : B;
-    sep = face_separates(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB);
+    sep = face_separates<false>(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB);
   }
   return __ballot(sep) != 0ull;
 }
@@ -519,6 +549,27 @@ __device__ __forceinline__ PairOps pair_ops(const DevParams& P, uint32_t slot, c
   PairOps o;
   o.a = (int)(slot / LIST_STRIDE);
   o.b = (int)list32[slot];
+  const float4 oa = omg[o.a], ob = omg[o.b];
+  o.ta = packed_type(oa.w);
+  o.tb = packed_type(ob.w);
+  o.sa = shape_index(P, oa.w);
+  o.sb = shape_index(P, ob.w);
+  const float4 va4 = vel[o.a], vb4 = vel[o.b];
+  o.PosA = xyz(pos[o.a]);
+  o.PosB = xyz(pos[o.b]);
+  o.va = xyz(va4); o.vb = xyz(vb4);
+  o.ma = va4.w; o.mb = vb4.w;
+  o.wa = xyz(oa); o.wb = xyz(ob);
+  return o;
+}
+
+/* The operands of a queued pair whose partner is known. */
+__device__ __forceinline__ PairOps pair_ops_ab(const DevParams& P, uint32_t slot, uint32_t b,
+                                               const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                               const float4* __restrict__ omg) {
+  PairOps o;
+  o.a = (int)(slot / LIST_STRIDE);
+  o.b = (int)b;
   const float4 oa = omg[o.a], ob = omg[o.b];
   o.ta = packed_type(oa.w);
   o.tb = packed_type(ob.w);
@@ -1072,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
                                                     const float4* __restrict__ omg,
                                                     const uint32_t* __restrict__ list32,
                                                     const uint32_t* __restrict__ queue,
-                                                    uint32_t* __restrict__ qcount, uint32_t* __restrict__ equeue,
+                                                    uint32_t* __restrict__ qcount, uint2* __restrict__ equeue,
                                                     float4* __restrict__ pf, float4* __restrict__ pt) {
   const DevParams& P = *Pp;
   const uint32_t total = qcount[0];
@@ -1085,10 +1136,11 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
   const uint32_t cnt_w = (cnt + 63u) & ~63u;
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt_w; q += stride) {
     bool need = false;
-    uint32_t slot = 0u;
+    uint32_t slot = 0u, partner = 0u;
     if (q < cnt) {
       slot = queue[q];
       const PairOps o = pair_ops(P, slot, list32, pos, vel, omg);
+      partner = (uint32_t)o.b;
       const bdem_poly_shape& pa = shapes[o.sa];
       const bdem_poly_shape& pb = shapes[o.sb];
       const Shape A = shape_of(rot, L, o.a, pa);
@@ -1104,7 +1156,7 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
       base = __shfl(base, leader, 64);
       const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
       const uint32_t at = base + (uint32_t)__popcll(mask & lt);
-      if (need && at < cap) equeue[at] = slot;
+      if (need && at < cap) equeue[at] = make_uint2(slot, partner);
     }
   }
 }
@@ -1116,8 +1168,24 @@ struct PairStage {
   float4 fn[2][BDEM_SHAPE_MAX_FACE];
   float4 fc[2][BDEM_SHAPE_MAX_FACE];
   uint2 e[2][BDEM_SHAPE_MAX_EDGE];
+  uint2 vo[2][BDEM_SHAPE_MAX_FACE];      /* face vertex orders, four to a word */
+  float area[2][BDEM_SHAPE_MAX_FACE];
+  int fnv[2][BDEM_SHAPE_MAX_FACE];
   unsigned char ia[BDEM_SHAPE_MAX_EDGE], ib[BDEM_SHAPE_MAX_EDGE];   /* edge lists of the pruned search */
 };
+
+/* Face f's polygon of a shape row into the stage (vertex indices < 32). */
+__device__ __forceinline__ void stage_face(PairStage& S, int r, int f, const bdem_Surface& F) {
+  uint32_t w0 = 0u, w1 = 0u;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    w0 |= ((uint32_t)F.vertex_Order[q] & 0xFFu) << (8 * q);
+    w1 |= ((uint32_t)F.vertex_Order[q + 4] & 0xFFu) << (8 * q);
+  }
+  S.vo[r][f] = make_uint2(w0, w1);
+  S.area[r][f] = F.area;
+  S.fnv[r][f] = F.num_vertex;
+}
 
 /* The pairs the first pass could not separate, one wave per pair,
  * grid-stride over the queue by waves: vertices of A in B, vertices of B in
@@ -1130,9 +1198,8 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
                                                      const float4* __restrict__ rot,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const float4* __restrict__ omg,
-                                                     const uint32_t* __restrict__ list32,
                                                      const uint32_t* __restrict__ qcount,
-                                                     const uint32_t* __restrict__ equeue,
+                                                     const uint2* __restrict__ equeue,
                                                      float4* __restrict__ pf, float4* __restrict__ pt) {
   const DevParams& P = *Pp;
   __shared__ PairStage stage[4];
@@ -1142,31 +1209,36 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
   const uint32_t cnt = qcount[1] < cap ? qcount[1] : cap;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
   for (uint32_t q = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); q < cnt; q += waves) {
-    const uint32_t slot = equeue[q];
-    if (slot >= cap) continue;   /* wave-uniform */
-    const PairOps o = pair_ops(P, slot, list32, pos, vel, omg);
-    const bdem_poly_shape& pa = shapes[o.sa];
-    const bdem_poly_shape& pb = shapes[o.sb];
-    const Shape Ag = shape_of(rot, L, o.a, pa);
-    const Shape Bg = shape_of(rot, L, o.b, pb);
-    const int ea = pa.num_edges, eb = pb.num_edges;
+    const uint2 item = equeue[q];   /* (slot, partner) */
+    const uint32_t slot = item.x;
+    if (slot >= cap || item.y >= (uint32_t)n) continue;   /* wave-uniform */
+    const PairOps o = pair_ops_ab(P, slot, item.y, pos, vel, omg);
     /* the previous pair's reads are done before the stage is overwritten */
     __builtin_amdgcn_wave_barrier();
-    if (lane < Ag.nv) S.v[0][lane] = Ag.V[lane];
-    if (lane < Bg.nv) S.v[1][lane] = Bg.V[lane];
-    if (lane < Ag.nf) { S.fn[0][lane] = Ag.Fn[lane]; S.fc[0][lane] = Ag.Fc[lane]; }
-    if (lane < Bg.nf) { S.fn[1][lane] = Bg.Fn[lane]; S.fc[1][lane] = Bg.Fc[lane]; }
+    /* turned rows at the table's largest counts: their addresses need no shape row */
+    const float4* ra = rot + (size_t)o.a * L.stride;
+    const float4* rb = rot + (size_t)o.b * L.stride;
+    if (lane < L.pv) { S.v[0][lane] = ra[lane]; S.v[1][lane] = rb[lane]; }
+    if (lane < L.pf) {
+      S.fn[0][lane] = ra[L.pv + lane]; S.fc[0][lane] = ra[L.pv + L.pf + lane];
+      S.fn[1][lane] = rb[L.pv + lane]; S.fc[1][lane] = rb[L.pv + L.pf + lane];
+    }
+    const bdem_poly_shape& pa = shapes[o.sa];
+    const bdem_poly_shape& pb = shapes[o.sb];
+    const int ea = pa.num_edges, eb = pb.num_edges;
     if (lane < ea) S.e[0][lane] = make_uint2(pa.edge[lane].point_vindex[0], pa.edge[lane].point_vindex[1]);
     if (lane < eb) S.e[1][lane] = make_uint2(pb.edge[lane].point_vindex[0], pb.edge[lane].point_vindex[1]);
+    if (lane < pa.num_faces) stage_face(S, 0, lane, pa.face[lane]);
+    if (lane < pb.num_faces) stage_face(S, 1, lane, pb.face[lane]);
     __builtin_amdgcn_wave_barrier();
-    const Shape A{S.v[0], S.fn[0], S.fc[0], Ag.nv, Ag.nf};
-    const Shape B{S.v[1], S.fn[1], S.fc[1], Bg.nv, Bg.nf};
+    const Shape A{S.v[0], S.fn[0], S.fc[0], pa.num_vertex, pa.num_faces, S.vo[0], S.area[0], S.fnv[0]};
+    const Shape B{S.v[1], S.fn[1], S.fc[1], pb.num_vertex, pb.num_faces, S.vo[1], S.area[1], S.fnv[1]};
     const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
     if (separated_wave(A, B, o.PosA, o.PosB, vbA, vbB)) {
       if (lane == 0) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
       continue;
     }
-    PColl c = vertex_faces_pair(o.PosA, A, pa, o.PosB, B, pb);
+    PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
     if (!c.hit) {
       const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
       c = edges_wave(A, S.e[0], ea, B, S.e[1], eb, RelPos, o.PosA, o.PosB, vbA, vbB,
@@ -1290,7 +1362,7 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
   hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt);
   hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.qcount, pb.equeue, pb.pf, pb.pt);
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt);
 }
 
 /* World-frame shapes of the current (unsorted) state into pb.rot: the
