@@ -437,9 +437,10 @@ class Oracle {
     pair_response(a, b, N3, dis, RA, RB, F, T);
   }
 
-  /* spheres only: the library does not tally the polyhedra path */
+  /* one contact's dissipation increment, kept per particle for this step's
+   * ordered sum (the force-only debug pass keeps none) */
   void record(int k, int i, R e) const {
-    if (sim.EnergyCalc && !poly()) incs[k][i].push_back(e);
+    if (sim.EnergyCalc && size_t(i) < incs[k].size()) incs[k][size_t(i)].push_back(e);
   }
 
   /* Collision_Response_Particle_Particle (Collision_Response.cuh:374-494):
@@ -503,15 +504,23 @@ class Oracle {
    *   rstar, mstar  effective radius and mass
    *   lp     per type-pair constants; xi in/out tangential spring
    * Returns the force on a. */
+  /* diss (may be null): the contact's dissipation rate, the library's own
+   * definition for the history laws (the reference tallies only its own
+   * law): normal damping force x |vn| while the normal force pushes, or the
+   * elastic force x |vn| while damping has clipped it to zero (that spring
+   * energy is not returned), plus the tangential damping force x |vt| while
+   * the spring sticks, or the capped tangential force x |vt| while it slides. */
   V3<R> history_law(V3<R> n, R delta, V3<R> vc, R rstar, R mstar, const LawPair<R>& lp,
-                    V3<R>& xi) const {
+                    V3<R>& xi, R* diss = nullptr) const {
     R vn = dot(vc, n);
     V3<R> vt = vc - vn * n;
     xi = rotate_spring(xi, n, vt);
-    R fn, kt, gt;
+    R fn, kt, gt, dn, fe;
     if (law_kind() == BDEM_LAW_LSD_COULOMB) {
       R cn = R(-2.0f) * lp.beta * rsqrt_(lp.kn * mstar);
-      fn = lp.kn * delta - cn * vn;
+      fe = lp.kn * delta;
+      fn = fe - cn * vn;
+      dn = cn * vn;
       kt = lp.kt;
       gt = R(0);
     } else {
@@ -520,17 +529,29 @@ class Oracle {
       R st = R(8.0f) * lp.gstar * srd;
       R gn = R(-1.8257418583505538f) * lp.beta * rsqrt_(sn * mstar);   /* 2 sqrt(5/6) */
       gt = gn * lp.tg;                                                   /* sqrt(St m*) = tg sqrt(Sn m*) */
-      fn = (R(4.0f) / R(3.0f)) * lp.estar * srd * delta - gn * vn;
+      fe = (R(4.0f) / R(3.0f)) * lp.estar * srd * delta;
+      fn = fe - gn * vn;
+      dn = gn * vn;
       kt = st;
     }
+    const bool pushing = fn > R(0);
     if (fn < R(0)) fn = R(0);
     V3<R> ft = R(-1.0f) * kt * xi - gt * vt;
     R ft2 = dot(ft, ft);
     R fc = lp.mu * fn;
+    bool slip = false;
     if (ft2 > fc * fc) {
       R s = fc / rsqrt_(ft2);
       ft = ft * s;
       xi = (R(-1.0f) / kt) * ft;
+      slip = true;
+    }
+    if (diss) {
+      R e = (pushing ? std::fabs(dn) : std::fabs(fe)) * std::fabs(vn);
+      const R vt2 = dot(vt, vt);
+      if (slip) e = e + rsqrt_(dot(ft, ft)) * rsqrt_(vt2);
+      else e = e + gt * vt2;
+      *diss = e;
     }
     return fn * n + ft;
   }
@@ -563,7 +584,11 @@ class Oracle {
     R rstar = ((radius[a] * radius[b]) * sm) * q;
     R mstar = ((mass[a] * mass[b]) * sr) * q;
     V3<R> xi = find_spring(hin, hcin, HMAX, pid[a], pid[b]);
-    V3<R> f = history_law(n, delta, vc, rstar, mstar, law[ptype[a]][ptype[b]], xi);
+    R e = R(0);
+    V3<R> f = history_law(n, delta, vc, rstar, mstar, law[ptype[a]][ptype[b]], xi, &e);
+    /* half the pair's dissipation from each side, as the reference law's
+     * tally splits its normal work (Collision_Response.cuh:466-474) */
+    record(T_PP, a, (R(0.5f) * e) * R(sim.InitalDelta_t));
     F = F + f;
     if (sim.Rotation) T = T + cross(la, f);
     if (int(out.size()) < HMAX) out.push_back(PairHist<R>{pid[b], xi});
@@ -746,7 +771,9 @@ class Oracle {
     V3<R> la = c.cp - pos[i];
     V3<R> vc = (vel[i] + cross(avel[i], la)) - vw;
     V3<R> xi = find_spring(whist, wcount, WMAX, pid[i], wall_id);
-    V3<R> f = history_law(c.n, c.dis, vc, radius[i], mass[i], wlaw[ptype[i]], xi);
+    R e = R(0);
+    V3<R> f = history_law(c.n, c.dis, vc, radius[i], mass[i], wlaw[ptype[i]], xi, &e);
+    record(T_SURF, i, e * R(sim.InitalDelta_t));
     F = F + f;
     if (sim.Rotation) T = T + cross(la, f);
     if (int(wout.size()) < WMAX) wout.push_back(PairHist<R>{wall_id, xi});

@@ -1360,7 +1360,7 @@ int bdem_update_sim(bdem_ctx* c) {
   if (ev) c->t_lists[(size_t)c->t_used] = sa.on ? 1 : 0;
   if (c->poly)
     launch_poly_step(c->dp, c->dshape, c->b, c->pb, c->b.keys_sorted[c->sorted_par], c->append_par, do_kill,
-                     c->record_wf, !c->dobj.empty(), c->stream);
+                     c->record_wf, !c->dobj.empty(), c->sim.EnergyCalc, c->stream);
   else
     launch_step(c->dp, c->b, c->b.keys_sorted[c->sorted_par], c->hp.law, c->orient, c->hist_par,
                 c->append_par, do_kill, c->record_wf, c->sim.EnergyCalc, sa, c->dbg_list, c->dbg_nn, c->stream);

@@ -611,15 +611,25 @@ __device__ __forceinline__ f3 rotate_spring(const DevParams& P, f3 xi_old, f3 n,
   return xi + vt * P.dt;
 }
 
+/* diss (may be null): the contact's dissipation rate for the tallies, this
+ * library's definition for the history laws (the reference tallies only
+ * its own law): normal damping force x |vn| while the normal force pushes,
+ * or the elastic force x |vn| while damping has clipped it to zero (that
+ * spring energy is not returned), plus the tangential damping force x |vt|
+ * while the spring sticks, or the capped tangential force x |vt| while it
+ * slides. */
 __device__ __forceinline__ f3 history_law(const DevParams& P, f3 n, float delta, f3 vc,
-                                          float rstar, float mstar, const LawPair& lp, f3& xi) {
+                                          float rstar, float mstar, const LawPair& lp, f3& xi,
+                                          float* diss = nullptr) {
   const float vn = dot(vc, n);
   const f3 vt = vc - vn * n;
   xi = rotate_spring(P, xi, n, vt);
-  float fn, kt, gt;
+  float fn, kt, gt, dn, fe;
   if (P.law == BDEM_LAW_LSD_COULOMB) {
     const float cn = -2.0f * lp.beta * __builtin_sqrtf(lp.kn * mstar);
-    fn = lp.kn * delta - cn * vn;
+    fe = lp.kn * delta;
+    fn = fe - cn * vn;
+    dn = cn * vn;
     kt = lp.kt;
     gt = 0.0f;
   } else {
@@ -628,17 +638,29 @@ __device__ __forceinline__ f3 history_law(const DevParams& P, f3 n, float delta,
     const float st = 8.0f * lp.gstar * srd;
     const float gn = -1.8257418583505538f * lp.beta * __builtin_sqrtf(sn * mstar);
     gt = gn * lp.tg;   /* sqrt(St m*) = tg sqrt(Sn m*) */
-    fn = (4.0f / 3.0f) * lp.estar * srd * delta - gn * vn;
+    fe = (4.0f / 3.0f) * lp.estar * srd * delta;
+    fn = fe - gn * vn;
+    dn = gn * vn;
     kt = st;
   }
+  const bool pushing = fn > 0.0f;
   if (fn < 0.0f) fn = 0.0f;
   f3 ft = -1.0f * kt * xi - gt * vt;
   const float ft2 = dot(ft, ft);
   const float fc = lp.mu * fn;
+  bool slip = false;
   if (ft2 > fc * fc) {
     const float s = fc / __builtin_sqrtf(ft2);
     ft = ft * s;
     xi = (-1.0f / kt) * ft;
+    slip = true;
+  }
+  if (diss) {
+    float e = (pushing ? fabsf(dn) : fabsf(fe)) * fabsf(vn);
+    const float vt2 = dot(vt, vt);
+    if (slip) e = e + __builtin_sqrtf(dot(ft, ft)) * __builtin_sqrtf(vt2);
+    else e = e + gt * vt2;
+    *diss = e;
   }
   return fn * n + ft;
 }
@@ -849,12 +871,14 @@ struct WallHist {
 
 __device__ __forceinline__ void wall_contact_hist(const DevParams& P, int t, f3 p, float r, f3 v,
                                                   f3 w, float m, const Coll& c, f3 vw, int wall_id,
-                                                  f3& F, f3& T, WallHist& wh) {
+                                                  f3& F, f3& T, WallHist& wh, float* tally = nullptr) {
   const f3 la = c.cp - p;
   const f3 vc = (v + cross(w, la)) - vw;
   int cur = 0;
   f3 xi = find_spring(wh.in, wh.in_cnt, wall_id, cur);
-  const f3 f = history_law(P, c.n, c.dis, vc, r, m, P.law_w[t], xi);
+  float e = 0.0f;
+  const f3 f = history_law(P, c.n, c.dis, vc, r, m, P.law_w[t], xi, tally ? &e : nullptr);
+  if (tally) atomicAdd(tally + TALLY_SURF, e * P.dt);
   F = F + f;
   if (P.rotation) T = T + cross(la, f);
   if (wh.out_cnt < WMAX) {
@@ -879,7 +903,7 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
         if (!c.hit) continue;
         if (hist) {
           const f3 vw = moving_planes ? drum_velocity(P, c.cp) : mk3(0.f, 0.f, 0.f);
-          wall_contact_hist(P, t, p, r, v, w, m, c, vw, k * 16 + s, F, T, *wh);
+          wall_contact_hist(P, t, p, r, v, w, m, c, vw, k * 16 + s, F, T, *wh, tally);
         } else if (moving_planes) {
           f3 tq;
           F = F + resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq, tally);
@@ -894,7 +918,7 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
       if (c.hit) {
         if (hist) {
           const f3 vw = P.cyl_rotation ? drum_velocity(P, c.cp) : mk3(0.f, 0.f, 0.f);
-          wall_contact_hist(P, t, p, r, v, w, m, c, vw, 128 + k, F, T, *wh);
+          wall_contact_hist(P, t, p, r, v, w, m, c, vw, 128 + k, F, T, *wh, tally);
         } else if (P.cyl_rotation) {
           f3 tq;
           F = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, c.cp), tq, tally);
@@ -909,14 +933,14 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
       if (cyl.has_top_cap) {
         const Coll cc = sphere_plane(p, r, cvt(cyl.normal_top_cap), cvt(cyl.center_top_cap), nullptr, nullptr);
         if (cc.hit) {
-          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 144 + k, F, T, *wh);
+          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 144 + k, F, T, *wh, tally);
           else F = resp_endplate(P, t, v, cc);
         }
       }
       if (cyl.has_bot_cap) {
         const Coll cc = sphere_plane(p, r, cvt(cyl.normal_bot_cap), cvt(cyl.center_bot_cap), nullptr, nullptr);
         if (cc.hit) {
-          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 160 + k, F, T, *wh);
+          if (hist) wall_contact_hist(P, t, p, r, v, w, m, cc, mk3(0.f, 0.f, 0.f), 160 + k, F, T, *wh, tally);
           else F = resp_endplate(P, t, v, cc);
         }
       }
@@ -925,7 +949,7 @@ __device__ __forceinline__ void wall_forces(const DevParams& P, int t, f3 p, flo
   for (int k = 0; k < P.num_cones; ++k) {
     const Coll c = sphere_cone(p, r, P.cone[k]);
     if (!c.hit) continue;
-    if (hist) wall_contact_hist(P, t, p, r, v, w, m, c, mk3(0.f, 0.f, 0.f), 176 + k, F, T, *wh);
+    if (hist) wall_contact_hist(P, t, p, r, v, w, m, c, mk3(0.f, 0.f, 0.f), 176 + k, F, T, *wh, tally);
     else resp_static(P, t, p, v, w, m, c, F, T, tally);
   }
 }

@@ -125,8 +125,11 @@ __device__ __forceinline__ void contact_force(const DevParams& P, int j, const f
     f3 xi = find_spring_first(hin, hin_cnt, idb, cursor, first);
     /* one type: a wave-uniform address, read by scalar loads that need not
      * wait for the partner's type word */
+    float e = 0.0f;
     Fc = history_law(P, n, delta, vc, rstar, mstar, P.one_type ? P.law_pp[0][0] : P.law_pp[ta][packed_type(ob.w)],
-                     xi);
+                     xi, tally ? &e : nullptr);
+    /* half the pair's dissipation from each side (the pair is met from both) */
+    if (tally) atomicAdd(tally + TALLY_PP, (0.5f * e) * P.dt);
     Tc = cross(la, Fc);
     if (hrec) *hrec = make_float4(xi.x, xi.y, xi.z, __int_as_float(idb));
   }
@@ -584,13 +587,10 @@ template <int LAW, bool ORIENT>
 static void launch_step_s(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, int hp, int ap,
                           bool do_kill, bool want_wf, bool energy, const SkinArgs& sa, uint32_t* dl, uint32_t* dn,
                           hipStream_t s) {
-  if constexpr (LAW == BDEM_LAW_REFERENCE) {
-    /* the dissipation tallies are defined for the reference law only */
-    if (energy) {
-      if (sa.on) launch_step_t<LAW, ORIENT, true, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-      else launch_step_t<LAW, ORIENT, false, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
-      return;
-    }
+  if (energy) {   /* dissipation tallies (EnergyCalc) */
+    if (sa.on) launch_step_t<LAW, ORIENT, true, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    else launch_step_t<LAW, ORIENT, false, true>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
+    return;
   }
   if (sa.on) launch_step_t<LAW, ORIENT, true, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
   else launch_step_t<LAW, ORIENT, false, false>(dP, b, sk, hp, ap, do_kill, want_wf, sa, dl, dn, s);
@@ -603,11 +603,11 @@ void launch_step(const DevParams* dP, const StepBuffers& b, const uint32_t* sk, 
     if (orient) launch_step_s<BDEM_LAW_REFERENCE, true>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
     else launch_step_s<BDEM_LAW_REFERENCE, false>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
   } else if (law == BDEM_LAW_LSD_COULOMB) {
-    if (orient) launch_step_s<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
-    else launch_step_s<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
+    if (orient) launch_step_s<BDEM_LAW_LSD_COULOMB, true>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_LSD_COULOMB, false>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
   } else {
-    if (orient) launch_step_s<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
-    else launch_step_s<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, false, sa, dl, dn, s);
+    if (orient) launch_step_s<BDEM_LAW_HERTZ_MINDLIN, true>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
+    else launch_step_s<BDEM_LAW_HERTZ_MINDLIN, false>(dP, b, sk, hp, ap, do_kill, want_wf, energy, sa, dl, dn, s);
   }
 }
 

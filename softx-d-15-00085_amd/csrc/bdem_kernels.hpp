@@ -169,7 +169,7 @@ void launch_skin(const DevParams* dP, const StepBuffers& b, const uint32_t* old_
  * with the same outputs as launch_step; and the force stage alone. */
 void launch_poly_step(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                       const uint32_t* sorted_keys, int append_par, bool do_kill, bool want_wall_force,
-                      bool lifters, hipStream_t s);
+                      bool lifters, bool energy, hipStream_t s);
 /* rot <- world-frame vertices, normals and centroids of the current state */
 void launch_poly_world_shapes(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                               const PolyBuffers& pb, hipStream_t s);

@@ -586,12 +586,14 @@ __device__ __forceinline__ PairOps pair_ops_ab(const DevParams& P, uint32_t slot
 
 /* Clamps, then the reference pair law into the pair's slot. */
 __device__ __forceinline__ void pair_store(const DevParams& P, const bdem_poly_shape& pa, const PairOps& o, PColl c,
-                                           uint32_t slot, float4* __restrict__ pf, float4* __restrict__ pt) {
+                                           uint32_t slot, float4* __restrict__ pf, float4* __restrict__ pt,
+                                           float* tally = nullptr) {
   float4 F4 = make_float4(0.f, 0.f, 0.f, 0.f), T4 = F4;
   if (c.hit) {
     depth_clamps(c, pa.radius, o.PosA, o.PosB, o.va - o.vb);
     f3 F, T;
-    pair_resp(P, o.va, o.ma, o.wa, o.ta, o.vb, o.mb, o.wb, o.tb, c.n, c.dis, c.cp - o.PosA, c.cp - o.PosB, F, T);
+    pair_resp(P, o.va, o.ma, o.wa, o.ta, o.vb, o.mb, o.wb, o.tb, c.n, c.dis, c.cp - o.PosA, c.cp - o.PosB, F, T,
+              tally);
     F4 = make_float4(F.x, F.y, F.z, 1.0f);
     T4 = make_float4(T.x, T.y, T.z, 0.0f);
   }
@@ -721,7 +723,7 @@ __device__ __forceinline__ PColl poly_cylinder(f3 p, float r, const Shape& A, co
  * 254-610).  Infinite planes have no polyhedra method: the previous result
  * stands, as written. */
 __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p, float r, f3 v, f3 w, float m,
-                                                 const Shape& A, f3& F, f3& T) {
+                                                 const Shape& A, f3& F, f3& T, float* tally) {
   F = mk3(0.f, 0.f, 0.f);
   T = mk3(0.f, 0.f, 0.f);
   const bool moving_planes = P.rotating_wobj && P.cyl_rotation;
@@ -735,8 +737,8 @@ __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p
         if (!res.hit) continue;
         const Coll c{true, res.dis, res.cp, res.n};
         f3 ft = mk3(0.f, 0.f, 0.f), tq = ft;
-        if (moving_planes) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq);
-        else resp_static(P, t, p, v, w, m, c, ft, tq);
+        if (moving_planes) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq, tally);
+        else resp_static(P, t, p, v, w, m, c, ft, tq, tally);
         F = F + ft;
         if (P.rotation && res.cls != C_FF) T = T + tq;
       }
@@ -746,8 +748,8 @@ __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p
       if (res.hit) {
         const Coll c{true, res.dis, res.cp, res.n};
         f3 ft = mk3(0.f, 0.f, 0.f), tq = ft;
-        if (P.cyl_rotation) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq);
-        else resp_static(P, t, p, v, w, m, c, ft, tq);
+        if (P.cyl_rotation) ft = resp_moving(P, t, p, v, w, m, c, drum_velocity(P, res.cp), tq, tally);
+        else resp_static(P, t, p, v, w, m, c, ft, tq, tally);
         F = ft;
         if (P.rotation && res.cls != C_FF) T = T + tq;
       }
@@ -997,7 +999,8 @@ __device__ __forceinline__ PColl edges_dobj(const bdem_DOBJ& D, const Shape& B, 
 
 /* Lifter force and torque on a polyhedron: the first lifter in contact acts. */
 __device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3 p, f3 v, f3 w, float m,
-                                                   const Shape& A, const bdem_poly_shape& pa, f3& F, f3& T) {
+                                                   const Shape& A, const bdem_poly_shape& pa, f3& F, f3& T,
+                                                   float* tally) {
   F = mk3(0.f, 0.f, 0.f);
   T = F;
   const float r = pa.radius;
@@ -1026,7 +1029,7 @@ __device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3
     else if (D.is_translating) Vl = cvt(D.velocity);
     const DColl c{true, 0, res.dis, res.n, res.cp};
     f3 Ftr, Tq;
-    resp_dobject(P, t, p, v, w, m, c, Vl, Ftr, Tq);
+    resp_dobject(P, t, p, v, w, m, c, Vl, Ftr, Tq, tally);
     F = F + Ftr;
     if (P.rotation) T = T + Tq;
     break;
@@ -1200,8 +1203,9 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
                                                      const float4* __restrict__ omg,
                                                      const uint32_t* __restrict__ qcount,
                                                      const uint2* __restrict__ equeue,
-                                                     float4* __restrict__ pf, float4* __restrict__ pt) {
+                                                     float4* __restrict__ pf, float4* __restrict__ pt, int energy) {
   const DevParams& P = *Pp;
+  float* const tally = energy ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
   __shared__ PairStage stage[4];
   PairStage& S = stage[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
@@ -1245,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
                      pa.plane_slack * P.vbound_scale, pb.plane_slack * P.vbound_scale, S.ia, S.ib);
       if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
     }
-    if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt);
+    if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt, tally);
   }
 }
 
@@ -1263,8 +1267,9 @@ __global__ __launch_bounds__(256) void k_poly_finish(
     unsigned long long* __restrict__ sbuf, uint32_t* __restrict__ append, const uint32_t* __restrict__ hwc_in,
     uint32_t* __restrict__ hwc_out, float* __restrict__ wall_force, int do_kill, uint32_t* status,
     int* kill_count, float* __restrict__ fpp, float* __restrict__ tpp, float* __restrict__ fw,
-    float* __restrict__ tw) {
+    float* __restrict__ tw, int energy) {
   const DevParams& P = *Pp;
+  float* const tally = energy ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
   const int i = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const bool in = i < n;
   bool chg = false;
@@ -1291,9 +1296,9 @@ __global__ __launch_bounds__(256) void k_poly_finish(
       if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
       const Shape A = shape_of(rot, L, i, po);
       f3 Fw, Tw;
-      wall_forces_poly(P, ta, p, r, v, w, m, A, Fw, Tw);
+      wall_forces_poly(P, ta, p, r, v, w, m, A, Fw, Tw, tally);
       f3 Fl = mk3(0.f, 0.f, 0.f), Tl = Fl;
-      if (LIFTERS && P.num_dobj > 0) lifter_forces_poly(P, ta, p, v, w, m, A, po, Fl, Tl);
+      if (LIFTERS && P.num_dobj > 0) lifter_forces_poly(P, ta, p, v, w, m, A, po, Fl, Tl, tally);
       if (FORCES_ONLY) {
         fpp[3 * i] = Fpp.x; fpp[3 * i + 1] = Fpp.y; fpp[3 * i + 2] = Fpp.z;
         tpp[3 * i] = Tpp.x; tpp[3 * i + 1] = Tpp.y; tpp[3 * i + 2] = Tpp.z;
@@ -1352,7 +1357,7 @@ __global__ __launch_bounds__(256) void k_poly_finish(
 
 /* ------------------------------------------------------------------ launchers */
 static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
-                                      const PolyBuffers& pb, hipStream_t s) {
+                                      const PolyBuffers& pb, bool energy, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
   hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dshape,
                      b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
@@ -1362,7 +1367,7 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
   hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt);
   hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt);
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt, energy ? 1 : 0);
 }
 
 /* World-frame shapes of the current (unsorted) state into pb.rot: the
@@ -1376,24 +1381,25 @@ void launch_poly_world_shapes(const DevParams* dP, const bdem_poly_shape* dshape
 }
 
 void launch_poly_step(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
-                      const uint32_t* sk, int ap, bool do_kill, bool want_wf, bool lifters, hipStream_t s) {
-  launch_poly_forces_common(dP, dshape, b, pb, s);
+                      const uint32_t* sk, int ap, bool do_kill, bool want_wf, bool lifters, bool energy,
+                      hipStream_t s) {
+  launch_poly_forces_common(dP, dshape, b, pb, energy, s);
   /* the lifter search needs private arrays (scratch): only its own instantiation carries it */
   auto k = lifters ? k_poly_finish<false, true> : k_poly_finish<false, false>;
   hipLaunchKernelGGL(k, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dshape, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, b.pos_cur,
                      b.vel_cur, b.omg_cur, b.quat_cur, b.keys_new, sk, b.changed, b.sbuf,
                      b.meta + META_APPEND + ap, b.hwc_srt, b.hwc_cur, want_wf ? b.wall_force : nullptr,
-                     do_kill ? 1 : 0, b.status, b.kill_count, nullptr, nullptr, nullptr, nullptr);
+                     do_kill ? 1 : 0, b.status, b.kill_count, nullptr, nullptr, nullptr, nullptr, energy ? 1 : 0);
 }
 
 void launch_poly_forces(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b, const PolyBuffers& pb,
                         float* fpp, float* tpp, float* fw, float* tw, hipStream_t s) {
-  launch_poly_forces_common(dP, dshape, b, pb, s);
+  launch_poly_forces_common(dP, dshape, b, pb, false, s);
   hipLaunchKernelGGL((k_poly_finish<true, false>), dim3((b.n + 255) / 256), dim3(256), 0, s, dP, dshape, b.n, pb.L,
                      pb.rot, pb.nn, pb.pf, pb.pt, b.pos_srt, b.vel_srt, b.omg_srt, b.quat_srt, nullptr, nullptr,
                      nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-                     b.status, b.kill_count, fpp, tpp, fw, tw);
+                     b.status, b.kill_count, fpp, tpp, fw, tw, 0);
 }
 
 }  // namespace bdem
