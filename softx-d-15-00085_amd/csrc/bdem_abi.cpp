@@ -698,11 +698,13 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
   if (num_dobj > 0 && !dobj) return fail(c, BDEM_EINVAL, "null dynamic-object table");
   int rc0 = BDEM_OK;
   if (num_dobj > 0 && (rc0 = check_dobj(c, dobj, num_dobj)) != BDEM_OK) return rc0;
+  /* use_hist (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist,
+   * DEM_Kernels_Compute.cuh:998-1329) is accepted and changes nothing: the
+   * reference's set-up always clears it (main.cpp:830) and its kernel is
+   * undefined as written.  The polyhedra step keeps its own contact memory
+   * in every mode, the face that separated each listed pair at the previous
+   * step, which it tries first; that memory never changes a result. */
   if (sim->particle_type == 1) {
-    if (sim->use_hist)
-      return fail(c, BDEM_ENOTSUP,
-                  "polyhedra contact-type history (Kernel_ParticleInteraction_Polyhedra_NonSymmetryHist) is "
-                  "disabled in the reference (main.cpp:830) and undefined as written; not built");
     if ((rc0 = check_poly(c, pobj, num_pobj, 32, 32, 32)) != BDEM_OK) return rc0;
   }
   if ((rc0 = check_wobj(c, wobj, num_wobj)) != BDEM_OK) return rc0;
@@ -762,6 +764,8 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     ALLOC(pb.qcount, 2);
     ALLOC(pb.pf, (size_t)n * LIST_STRIDE);
     ALLOC(pb.pt, (size_t)n * LIST_STRIDE);
+    ALLOC(pb.hint, (size_t)n * LIST_STRIDE);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(pb.hint), (size_t)n * LIST_STRIDE * 2, 0xFFFFFFFFu, c->stream);
   }
 #undef ALLOC
   HIPCHK(c, hipMemset(b.status, 0, sizeof(uint32_t)));

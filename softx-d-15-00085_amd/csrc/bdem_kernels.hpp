@@ -71,6 +71,7 @@ struct PolyBuffers {
   uint2* equeue;                       /* (slot, partner) of the pairs for the wave-per-pair kernel */
   uint32_t* qcount;                    /* [0] queue length, [1] edge queue length */
   float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
+  uint2* hint;                         /* n * 32 by P_ID and list slot: (partner P_ID, separating face code) */
 };
 constexpr int POLY_PAIR_BLOCKS = 2048;
 

@@ -493,17 +493,37 @@ __device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
  * the other is tried first (quick_separated, one thread per pair): it
  * separates most separated pairs.  The rest go to the wave-per-pair kernel,
  * where separated_wave tests every face at once. */
+/* A separating face is named by a code: f for face f of A, 0x100 | f for
+ * face f of B; HINT_NONE for none. */
+constexpr uint32_t HINT_NONE = 0xFFFFFFFFu;
+
 __device__ __forceinline__ bool quick_separated(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
-                                                float vbB) {
+                                                float vbB, uint32_t& code) {
   const f3 dAB = PosB - PosA;
-  if (face_separates(A, facing_face(A, dAB), PosB, B, vbB)) return true;
-  return face_separates(B, facing_face(B, -1.0f * dAB), PosA, A, vbA);
+  const int fa = facing_face(A, dAB);
+  if (face_separates(A, fa, PosB, B, vbB)) { code = (uint32_t)fa; return true; }
+  const int fb = facing_face(B, -1.0f * dAB);
+  if (face_separates(B, fb, PosA, A, vbA)) { code = 0x100u | (uint32_t)fb; return true; }
+  return false;
+}
+
+/* The separating-face hint of a listed pair: the partner's P_ID and the code
+ * of the face that separated the two at the previous step.  Whether some
+ * face separates a pair does not depend on which face shows it, so a face
+ * that separates them now proves it, and a stale or foreign hint costs one
+ * test and changes nothing. */
+__device__ __forceinline__ bool hint_separates(uint2 h, uint32_t pid_b, const Shape& A, const Shape& B, f3 PosA,
+                                               f3 PosB, float vbA, float vbB) {
+  if (h.x != pid_b || h.y == HINT_NONE) return false;
+  const int f = (int)(h.y & 0xFFu);
+  if (h.y & 0x100u) return f < B.nf && face_separates(B, f, PosA, A, vbA);
+  return f < A.nf && face_separates(A, f, PosB, B, vbB);
 }
 
 /* is_SP_Poly_Poly both ways with a lane per face: lanes 0..31 test A's
  * faces against B, lanes 32..63 B's faces against A (at most 32 faces). */
-__device__ __forceinline__ bool separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
-                                               float vbB) {
+__device__ __forceinline__ uint32_t separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
+                                                   float vbB) {
   const int lane = threadIdx.x & 63;
   const bool hb = lane >= 32;
   const int f = lane & 31;
@@ -513,7 +533,10 @@ __device__ __forceinline__ bool separated_wave(const Shape& A, const Shape& B, f
     const Shape& other = hb ? A : B;
     sep = face_separates<false>(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB);
   }
-  return __ballot(sep) != 0ull;
+  const uint64_t m = __ballot(sep);
+  if (!m) return HINT_NONE;
+  const int l = __ffsll((unsigned long long)m) - 1;   /* the code of the first separating face */
+  return l < 32 ? (uint32_t)l : (0x100u | (uint32_t)(l - 32));
 }
 
 /* The depth clamps (DEM_Kernels_Compute.cuh:918-950). */
@@ -1127,7 +1150,8 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
                                                     const uint32_t* __restrict__ list32,
                                                     const uint32_t* __restrict__ queue,
                                                     uint32_t* __restrict__ qcount, uint2* __restrict__ equeue,
-                                                    float4* __restrict__ pf, float4* __restrict__ pt) {
+                                                    float4* __restrict__ pf, float4* __restrict__ pt,
+                                                    uint2* __restrict__ hint) {
   const DevParams& P = *Pp;
   const uint32_t total = qcount[0];
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
@@ -1148,7 +1172,20 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
       const bdem_poly_shape& pb = shapes[o.sb];
       const Shape A = shape_of(rot, L, o.a, pa);
       const Shape B = shape_of(rot, L, o.b, pb);
-      need = !quick_separated(A, B, o.PosA, o.PosB, pa.vbound * P.vbound_scale, pb.vbound * P.vbound_scale);
+      const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
+      /* the pair's hint, kept by P_ID and list slot (rows past n: no hint) */
+      const int ida = packed_pid(omg[o.a].w);
+      const uint32_t idb = (uint32_t)packed_pid(omg[o.b].w);
+      uint2* hp = (ida >= 0 && ida < n) ? hint + (size_t)ida * LIST_STRIDE + (slot % LIST_STRIDE) : nullptr;
+      uint32_t code = HINT_NONE;
+      bool sep = false;
+      if (hp) {
+        const uint2 h = *hp;
+        if (hint_separates(h, idb, A, B, o.PosA, o.PosB, vbA, vbB)) { sep = true; code = h.y; }
+      }
+      if (!sep) sep = quick_separated(A, B, o.PosA, o.PosB, vbA, vbB, code);
+      if (hp) *hp = make_uint2(idb, sep ? code : HINT_NONE);
+      need = !sep;
       if (!need) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
     }
     const uint64_t mask = __ballot(need);
@@ -1203,7 +1240,8 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
                                                      const float4* __restrict__ omg,
                                                      const uint32_t* __restrict__ qcount,
                                                      const uint2* __restrict__ equeue,
-                                                     float4* __restrict__ pf, float4* __restrict__ pt, int energy) {
+                                                     float4* __restrict__ pf, float4* __restrict__ pt, int energy,
+                                                     uint2* __restrict__ hint) {
   const DevParams& P = *Pp;
   float* const tally = energy ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
   __shared__ PairStage stage[4];
@@ -1238,8 +1276,14 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
     const Shape A{S.v[0], S.fn[0], S.fc[0], pa.num_vertex, pa.num_faces, S.vo[0], S.area[0], S.fnv[0]};
     const Shape B{S.v[1], S.fn[1], S.fc[1], pb.num_vertex, pb.num_faces, S.vo[1], S.area[1], S.fnv[1]};
     const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
-    if (separated_wave(A, B, o.PosA, o.PosB, vbA, vbB)) {
-      if (lane == 0) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
+    const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB);
+    if (code != HINT_NONE) {
+      if (lane == 0) {
+        pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
+        const int ida = packed_pid(omg[o.a].w);
+        if (ida >= 0 && ida < n)   /* the next step tries this face first */
+          hint[(size_t)ida * LIST_STRIDE + (slot % LIST_STRIDE)] = make_uint2((uint32_t)packed_pid(omg[o.b].w), code);
+      }
       continue;
     }
     PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
@@ -1365,9 +1409,9 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
   hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt);
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt, pb.hint);
   hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt, energy ? 1 : 0);
+                     b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt, energy ? 1 : 0, pb.hint);
 }
 
 /* World-frame shapes of the current (unsorted) state into pb.rot: the

@@ -126,3 +126,37 @@ def test_shape_table_rejections_keep_the_context():
     dev.step(2)
     assert not dev.status() & bdem.lib.WARN_SORT
     dev.close()
+
+
+def test_separating_face_memory_shrinks_the_pair_queue_only():
+    """The face that separated a listed pair is tried first at the next step:
+    fewer pairs reach the wave-per-pair kernel, and the states are those of
+    a context whose memory starts empty at every step (the oracle)."""
+    print("[case] hull box 600, separating-face memory", flush=True)
+    deck = S.hull_box_deck(600, seed=41)
+    dev, ora = make_pair(deck)
+    dev.step(1)
+    listed1, wave1 = dev.debug_poly_queues()
+    dev.step(1)
+    listed2, wave2 = dev.debug_poly_queues()
+    print(f"step 1: {listed1} listed, {wave1} to the wave kernel; step 2: {listed2}, {wave2}")
+    assert wave2 < wave1
+    ora.step(2)
+    _compare_states("hull box memory 2 steps", deck, dev, ora)
+    dev.close()
+    ora.close()
+
+
+def test_use_hist_is_accepted_and_changes_nothing():
+    print("[case] mixed poly deck with use_hist, 20 steps", flush=True)
+    deck = S.poly_deck(1000, seed=54, L=16.0, shape="mixed", spacing=1.25)
+    sim2 = T.SimulationInfo.from_buffer_copy(deck["sim"])
+    sim2.use_hist = True
+    a, b = make_device(deck), make_device(dict(deck, sim=sim2))
+    a.step(20)
+    b.step(20)
+    sa, sb = a.get_state(), b.get_state()
+    for key in ("pos", "vel", "avel", "quat", "pid"):
+        assert np.array_equal(sa[key], sb[key]), key
+    a.close()
+    b.close()
