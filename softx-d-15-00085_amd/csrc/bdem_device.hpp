@@ -709,6 +709,10 @@ __device__ __forceinline__ Coll sphere_plane(f3 p, float r, f3 nrm, f3 cen, cons
 __device__ __forceinline__ Coll sphere_cylinder(f3 p, float r, const bdem_Macro_Cylinder& cyl) {
   Coll c{false, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
   const f3 Pvec = p * cvt(cyl.AXIS) - cvt(cyl.center_bot_cap);
+  /* inside by more than r + 10 um: no contact, decided without the square
+   * root (the margin exceeds the rounding of both forms by far) */
+  const float inner = cyl.radius - r - 0.001f;
+  if (inner > 0.0f && dot(Pvec, Pvec) < inner * inner) return c;
   const float Pd = length(Pvec) + r;
   if (Pd > cyl.radius) {
     f3 n = -1.0f * Pvec;
@@ -727,6 +731,11 @@ __device__ __forceinline__ Coll sphere_cone(f3 p, float r, const bdem_cone& cn) 
   const f3 ax = cvt(cn.axis);
   const f3 d = p - cvt(cn.base);
   const float s = dot(d, ax);
+  /* the distance to the meridian segment is at least the axial gap to its
+   * span [min(s0, s1), max(s0, s1)]: beyond r (plus a rounding margin) there
+   * is no contact */
+  const float slack = r + 0.001f + 4.0e-6f * fabsf(s);
+  if (s > fmaxf(cn.s0, cn.s1) + slack || s < fminf(cn.s0, cn.s1) - slack) return c;
   const f3 q = d - s * ax;
   const float rho = length(q);
   if (!(rho > 0.0f)) return c;
