@@ -1214,25 +1214,20 @@ struct PairStage {
   unsigned char ia[BDEM_SHAPE_MAX_EDGE], ib[BDEM_SHAPE_MAX_EDGE];   /* edge lists of the pruned search */
 };
 
-/* Face f's polygon of a shape row into the stage (vertex indices < 32). */
-__device__ __forceinline__ void stage_face(PairStage& S, int r, int f, const bdem_Surface& F) {
-  uint32_t w0 = 0u, w1 = 0u;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    w0 |= ((uint32_t)F.vertex_Order[q] & 0xFFu) << (8 * q);
-    w1 |= ((uint32_t)F.vertex_Order[q + 4] & 0xFFu) << (8 * q);
-  }
-  S.vo[r][f] = make_uint2(w0, w1);
-  S.area[r][f] = F.area;
-  S.fnv[r][f] = F.num_vertex;
-}
-
 /* The pairs the first pass could not separate, one wave per pair,
  * grid-stride over the queue by waves: vertices of A in B, vertices of B in
  * A (normal reversed), then edge-edge with its depth limited to 0.025 rA
  * (DEM_Kernels_Compute.cuh:852-904).  The wave first copies both bodies
- * into its LDS stage with one load per lane and row, so the searches read
- * LDS instead of chains of dependent global loads. */
+ * into its LDS stage, so the searches read LDS instead of chains of
+ * dependent global loads.
+ *
+ * Staging is one round trip to memory per pair.  Each lane fetches one of
+ * the wave's next 64 queue entries and the shape rows of its two bodies
+ * ahead of time; a pair's values are then read from that lane, so they are
+ * wave-uniform (scalar loads for the pair's state and shape counts), and
+ * every row of the stage is loaded without a condition on the shape's
+ * counts (lanes 0-31 body A, 32-63 body B; rows past the counts are inside
+ * the record and never read) before the first LDS write. */
 __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict__ Pp,
                                                      const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
                                                      const float4* __restrict__ rot,
@@ -1247,53 +1242,81 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
   __shared__ PairStage stage[4];
   PairStage& S = stage[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, k = lane & 31;
+  const int kv = k < L.pv ? k : L.pv - 1, kf = k < L.pf ? k : L.pf - 1;
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
   const uint32_t cnt = qcount[1] < cap ? qcount[1] : cap;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
-  for (uint32_t q = blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); q < cnt; q += waves) {
-    const uint2 item = equeue[q];   /* (slot, partner) */
-    const uint32_t slot = item.x;
-    if (slot >= cap || item.y >= (uint32_t)n) continue;   /* wave-uniform */
-    const PairOps o = pair_ops_ab(P, slot, item.y, pos, vel, omg);
-    /* the previous pair's reads are done before the stage is overwritten */
-    __builtin_amdgcn_wave_barrier();
-    /* turned rows at the table's largest counts: their addresses need no shape row */
-    const float4* ra = rot + (size_t)o.a * L.stride;
-    const float4* rb = rot + (size_t)o.b * L.stride;
-    if (lane < L.pv) { S.v[0][lane] = ra[lane]; S.v[1][lane] = rb[lane]; }
-    if (lane < L.pf) {
-      S.fn[0][lane] = ra[L.pv + lane]; S.fc[0][lane] = ra[L.pv + L.pf + lane];
-      S.fn[1][lane] = rb[L.pv + lane]; S.fc[1][lane] = rb[L.pv + L.pf + lane];
-    }
-    const bdem_poly_shape& pa = shapes[o.sa];
-    const bdem_poly_shape& pb = shapes[o.sb];
-    const int ea = pa.num_edges, eb = pb.num_edges;
-    if (lane < ea) S.e[0][lane] = make_uint2(pa.edge[lane].point_vindex[0], pa.edge[lane].point_vindex[1]);
-    if (lane < eb) S.e[1][lane] = make_uint2(pb.edge[lane].point_vindex[0], pb.edge[lane].point_vindex[1]);
-    if (lane < pa.num_faces) stage_face(S, 0, lane, pa.face[lane]);
-    if (lane < pb.num_faces) stage_face(S, 1, lane, pb.face[lane]);
-    __builtin_amdgcn_wave_barrier();
-    const Shape A{S.v[0], S.fn[0], S.fc[0], pa.num_vertex, pa.num_faces, S.vo[0], S.area[0], S.fnv[0]};
-    const Shape B{S.v[1], S.fn[1], S.fc[1], pb.num_vertex, pb.num_faces, S.vo[1], S.area[1], S.fnv[1]};
-    const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
-    const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB);
-    if (code != HINT_NONE) {
-      if (lane == 0) {
-        pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
-        const int ida = packed_pid(omg[o.a].w);
-        if (ida >= 0 && ida < n)   /* the next step tries this face first */
-          hint[(size_t)ida * LIST_STRIDE + (slot % LIST_STRIDE)] = make_uint2((uint32_t)packed_pid(omg[o.b].w), code);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6));
+  /* the wave's pairs are q = wave + j * waves, j = 0, 1, ... */
+  for (uint32_t base = wave; base < cnt; base += 64u * waves) {
+    uint32_t lslot = cap, lpart = 0u, lsa = 0u, lsb = 0u;
+    const uint32_t ql = base + (uint32_t)lane * waves;
+    if (ql < cnt) {
+      const uint2 item = equeue[ql];   /* (slot, partner) */
+      if (item.x < cap && item.y < (uint32_t)n) {
+        lslot = item.x;
+        lpart = item.y;
+        lsa = (uint32_t)shape_index(P, omg[item.x / LIST_STRIDE].w);
+        lsb = (uint32_t)shape_index(P, omg[item.y].w);
       }
-      continue;
     }
-    PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
-    if (!c.hit) {
-      const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
-      c = edges_wave(A, S.e[0], ea, B, S.e[1], eb, RelPos, o.PosA, o.PosB, vbA, vbB,
-                     pa.plane_slack * P.vbound_scale, pb.plane_slack * P.vbound_scale, S.ia, S.ib);
-      if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
+    for (int j = 0; j < 64; ++j) {
+      if (base + (uint32_t)j * waves >= cnt) break;
+      const uint32_t slot = __builtin_amdgcn_readlane(lslot, j);
+      if (slot >= cap) continue;
+      const uint32_t partner = __builtin_amdgcn_readlane(lpart, j);
+      const bdem_poly_shape& pa = shapes[__builtin_amdgcn_readlane(lsa, j)];
+      const bdem_poly_shape& pb = shapes[__builtin_amdgcn_readlane(lsb, j)];
+      /* turned rows at the table's largest counts, edges and face polygons */
+      const float4* rh = rot + (size_t)(h ? partner : slot / LIST_STRIDE) * L.stride;
+      const float4 v4 = rh[kv], n4 = rh[L.pv + kf], c4 = rh[L.pv + L.pf + kf];
+      const bdem_Edge_Def eA = pa.edge[lane], eB = pb.edge[lane];
+      const bdem_Surface& F = (h ? pb : pa).face[k];
+      uint32_t w0 = 0u, w1 = 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        w0 |= ((uint32_t)F.vertex_Order[q] & 0xFFu) << (8 * q);
+        w1 |= ((uint32_t)F.vertex_Order[q + 4] & 0xFFu) << (8 * q);
+      }
+      const float farea = F.area;
+      const int fnv = F.num_vertex;
+      const PairOps o = pair_ops_ab(P, slot, partner, pos, vel, omg);
+      const int ea = pa.num_edges, eb = pb.num_edges;
+      const int nva = pa.num_vertex, nfa = pa.num_faces, nvb = pb.num_vertex, nfb = pb.num_faces;
+      const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
+      /* the previous pair's reads are done before the stage is overwritten */
+      __builtin_amdgcn_wave_barrier();
+      S.v[h][k] = v4;
+      S.fn[h][k] = n4;
+      S.fc[h][k] = c4;
+      S.e[0][lane] = make_uint2(eA.point_vindex[0], eA.point_vindex[1]);
+      S.e[1][lane] = make_uint2(eB.point_vindex[0], eB.point_vindex[1]);
+      S.vo[h][k] = make_uint2(w0, w1);
+      S.area[h][k] = farea;
+      S.fnv[h][k] = fnv;
+      __builtin_amdgcn_wave_barrier();
+      const Shape A{S.v[0], S.fn[0], S.fc[0], nva, nfa, S.vo[0], S.area[0], S.fnv[0]};
+      const Shape B{S.v[1], S.fn[1], S.fc[1], nvb, nfb, S.vo[1], S.area[1], S.fnv[1]};
+      const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB);
+      if (code != HINT_NONE) {
+        if (lane == 0) {
+          pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
+          const int ida = packed_pid(omg[o.a].w);
+          if (ida >= 0 && ida < n)   /* the next step tries this face first */
+            hint[(size_t)ida * LIST_STRIDE + (slot % LIST_STRIDE)] = make_uint2((uint32_t)packed_pid(omg[o.b].w), code);
+        }
+        continue;
+      }
+      PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
+      if (!c.hit) {
+        const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
+        c = edges_wave(A, S.e[0], ea, B, S.e[1], eb, RelPos, o.PosA, o.PosB, vbA, vbB,
+                       pa.plane_slack * P.vbound_scale, pb.plane_slack * P.vbound_scale, S.ia, S.ib);
+        if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
+      }
+      if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt, tally);
     }
-    if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt, tally);
   }
 }
 
