@@ -208,9 +208,15 @@ int upload_params(bdem_ctx* c) {
   if (rc0 != BDEM_OK) return rc0;
   build_params(c);
   HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
-  if (c->dshape)
+  std::vector<ShapeHdr> hdr;
+  if (c->dshape) {
     HIPCHK(c, hipMemcpyAsync(c->dshape, c->shapes.data(), c->shapes.size() * sizeof(bdem_poly_shape),
                              hipMemcpyHostToDevice, c->stream));
+    for (const bdem_poly_shape& s : c->shapes)
+      hdr.push_back(ShapeHdr{s.num_vertex, s.num_faces, s.num_edges, 0, s.vbound, s.plane_slack, s.radius, 0.0f});
+    HIPCHK(c, hipMemcpyAsync(c->pb.hdr, hdr.data(), hdr.size() * sizeof(ShapeHdr), hipMemcpyHostToDevice,
+                             c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BDEM_OK;
 }
@@ -756,12 +762,13 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     for (int t = 0; t < num_pobj; ++t) c->shapes.push_back(shape_of_pobj(pobj[t]));
     pb.L = poly_layout(c->shapes);
     ALLOC(c->dshape, c->shapes.size());
+    ALLOC(pb.hdr, c->shapes.size());
     ALLOC(pb.rot, (size_t)n * pb.L.stride);
     ALLOC(pb.list, (size_t)n * LIST_STRIDE);
     ALLOC(pb.nn, (size_t)n);
     ALLOC(pb.queue, (size_t)n * LIST_STRIDE);
     ALLOC(pb.equeue, (size_t)n * LIST_STRIDE);
-    ALLOC(pb.qcount, 2);
+    ALLOC(pb.qcount, 4);
     ALLOC(pb.pf, (size_t)n * LIST_STRIDE);
     ALLOC(pb.pt, (size_t)n * LIST_STRIDE);
     ALLOC(pb.hint, (size_t)n * LIST_STRIDE);
@@ -1048,18 +1055,26 @@ int bdem_set_poly_shapes(bdem_ctx* c, const bdem_poly_shape* shapes, int count) 
   const PolyLayout L = poly_layout(t);
   drop_graphs(c);
   bdem_poly_shape* d = nullptr;
+  ShapeHdr* dh = nullptr;
   if ((rc = dalloc(c, &d, t.size())) != BDEM_OK) return rc;
+  if ((rc = dalloc(c, &dh, t.size())) != BDEM_OK) {
+    dfree(c, d);
+    return rc;
+  }
   if (L.stride != c->pb.L.stride) {
     float4* nr = nullptr;
     if ((rc = dalloc(c, &nr, (size_t)n * L.stride)) != BDEM_OK) {
       dfree(c, d);
+      dfree(c, dh);
       return rc;
     }
     dfree(c, c->pb.rot);
     c->pb.rot = nr;
   }
   dfree(c, c->dshape);
+  dfree(c, c->pb.hdr);
   c->dshape = d;
+  c->pb.hdr = dh;
   c->pb.L = L;
   c->shapes.swap(t);
   c->shape_by_pid = true;

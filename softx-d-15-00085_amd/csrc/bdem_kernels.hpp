@@ -62,14 +62,22 @@ struct SkinArgs {
  * (pv vertex slots, pf face normals, pf face centroids), the bounding-sphere
  * contact list, its pair queue and one force/torque slot per listed pair. */
 struct PolyLayout { int pv, pf, stride; };
+/* The fields of a shape-table row that the pair tests read for every pair,
+ * one 32-byte record per row (in the 3,148-byte row they lie at its start,
+ * middle and end, three cache lines apart). */
+struct ShapeHdr {
+  int nv, nf, ne, pad0;
+  float vbound, plane_slack, radius, pad1;
+};
 struct PolyBuffers {
   PolyLayout L;
+  ShapeHdr* hdr;                       /* one per shape-table row */
   float4* rot;                         /* n * stride */
   uint32_t* list;                      /* n * 32 partner slots */
   uint32_t* nn;                        /* full contact count per particle */
   uint32_t* queue;                     /* (particle * 32 + k) work items */
   uint2* equeue;                       /* (slot, partner) of the pairs for the wave-per-pair kernel */
-  uint32_t* qcount;                    /* [0] queue length, [1] edge queue length */
+  uint32_t* qcount;                    /* [0] queue length, [1] edge queue length, [2] its chunk counter */
   float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
   uint2* hint;                         /* n * 32 by P_ID and list slot: (partner P_ID, separating face code) */
 };

@@ -73,6 +73,12 @@ __device__ __forceinline__ Shape shape_of(const float4* __restrict__ rot, const 
   return Shape{base, base + L.pv, base + L.pv + L.pf, po.num_vertex, po.num_faces};
 }
 
+__device__ __forceinline__ Shape shape_of(const float4* __restrict__ rot, const PolyLayout& L, int i,
+                                          const ShapeHdr& h) {
+  const float4* base = rot + (size_t)i * L.stride;
+  return Shape{base, base + L.pv, base + L.pv + L.pf, h.nv, h.nf};
+}
+
 /* Face f; f = -1 (no face of B had A's centre in front) is a zero plane. */
 __device__ __forceinline__ f3 fnorm(const Shape& B, int f) { return f >= 0 ? ld3(B.Fn + f) : mk3(0.f, 0.f, 0.f); }
 __device__ __forceinline__ f3 fcent(const Shape& B, int f) { return f >= 0 ? ld3(B.Fc + f) : mk3(0.f, 0.f, 0.f); }
@@ -122,31 +128,64 @@ __device__ __forceinline__ bool point_in_poly(const Shape& A, f3 point) {
 
 struct InPoly { int f0, f1, f2; float d0, d1, d2; };
 
-/* Collision_Detection_Vertex_InPoly_3Faces (Collision_Detection_Polyhedra.cuh:287-375):
- * inside every face of B; the three closest faces that have PosP in front. */
-__device__ __forceinline__ bool vertex_in_poly(f3 PosP, f3 v, const Shape& B, InPoly& r) {
-  r = InPoly{-1, -1, -1, 10.0f, 10.0f, 10.0f};
-  for (int f = 0; f < B.nf; ++f) {
-    const f3 n = ld3(B.Fn + f), c = ld3(B.Fc + f);
-    const float vd = dot(n, v - c);
-    if (!(vd < -0.0000100f)) return false;
-    if (dot(n, PosP - c) > 0.0f) {
-      const float a = fabsf(vd);
-      if (a < r.d0) {
-        if (r.f0 > -1) {
-          if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
-          r.d1 = r.d0; r.f1 = r.f0;
-        }
-        r.d0 = a; r.f0 = f;
-      } else if (a < r.d1) {
-        if (r.f1 > -1) { r.d2 = r.d1; r.f2 = r.f1; }
-        r.d1 = a; r.f1 = f;
-      } else if (a < r.d2) {
-        r.d2 = a; r.f2 = f;
-      }
+/* The inside part of Collision_Detection_Vertex_InPoly_3Faces
+ * (Collision_Detection_Polyhedra.cuh:287-375) for every vertex of P against
+ * Q, a lane per (vertex, face): vertex v is inside when every face of Q has
+ * it behind its plane, vd = n . (v - c) < -1e-5 (the serial scan stops at
+ * the first face that does not).  Lanes 0-31 take face f = lane of Q with
+ * vertex 2r, lanes 32-63 the same face with vertex 2r + 1; bit v of the
+ * result is set for an inside vertex. */
+__device__ __forceinline__ uint32_t vertices_inside(const Shape& P, const Shape& Q) {
+  const int lane = threadIdx.x & 63;
+  const int f = lane & 31, h = lane >> 5;
+  const bool fv = f < Q.nf;
+  const int fr = fv ? f : 0;
+  const f3 n = ld3(Q.Fn + fr), c = ld3(Q.Fc + fr);
+  uint32_t inside = 0u;
+  for (int v0 = 0; v0 < P.nv; v0 += 2) {
+    const int v = v0 + h < P.nv ? v0 + h : v0;
+    const float vd = dot(n, ld3(P.V + v) - c);
+    const uint64_t out = __ballot(fv && !(vd < -0.0000100f));
+    if ((uint32_t)out == 0u) inside |= 1u << v0;
+    if (v0 + 1 < P.nv && (uint32_t)(out >> 32) == 0u) inside |= 1u << (v0 + 1);
+  }
+  return inside;
+}
+
+/* The selection part of the same function for an inside vertex: the three
+ * faces of Q with PosP in front (n . (PosP - c) > 0) that have the smallest
+ * |vd|, the earlier face first on ties, as the serial scan inserts them
+ * (strictly below the third distance so far, which starts at 10); -1 and 10
+ * where fewer qualify.  Lanes 0-31 work for vertex v, lanes 32-63 for vertex
+ * w, one face each; every lane returns its half's vertex's result. */
+__device__ __forceinline__ InPoly closest_faces(f3 PosP, const Shape& P, int v, int w, const Shape& Q) {
+  const int lane = threadIdx.x & 63;
+  const int f = lane & 31;
+  uint64_t key = ~0ull;   /* (bits of |vd|, face): the order of the insertion */
+  if (f < Q.nf) {
+    const f3 n = ld3(Q.Fn + f), c = ld3(Q.Fc + f);
+    const float a = fabsf(dot(n, ld3(P.V + (lane < 32 ? v : w)) - c));
+    if (dot(n, PosP - c) > 0.0f && a < 10.0f) key = ((uint64_t)__float_as_uint(a) << 32) | (uint32_t)f;
+  }
+  InPoly r{-1, -1, -1, 10.0f, 10.0f, 10.0f};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    uint64_t m = key;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {   /* minimum over the half */
+      const uint64_t o = __shfl_xor(m, d, 64);
+      m = o < m ? o : m;
+    }
+    if (m != ~0ull) {   /* the same in every lane of the half */
+      const int fm = (int)(uint32_t)m;
+      const float dm = __uint_as_float((uint32_t)(m >> 32));
+      if (q == 0) { r.f0 = fm; r.d0 = dm; }
+      else if (q == 1) { r.f1 = fm; r.d1 = dm; }
+      else { r.f2 = fm; r.d2 = dm; }
+      if (f == fm) key = ~0ull;
     }
   }
-  return true;
+  return r;
 }
 
 /* The ray from PosA through `through` meets face f of B inside its polygon. */
@@ -173,20 +212,21 @@ __device__ __forceinline__ int face_at(const InPoly& s, int q) { return q == 0 ?
  * taken in ascending order, as the serial loop appends them, so the first
  * two vertices, their closest faces and the ordered sum of the first eight
  * are the serial ones, and the selection runs on wave-uniform values. */
-/* The selection among the inside vertices of A (bit v of mask: vertex v),
- * whose closest faces lane off + v holds in t. */
-__device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, const Shape& B, uint64_t mask, int off,
-                                                     const InPoly& t) {
+/* The selection among the inside vertices of A (bit v of mask: vertex v):
+ * the closest faces of the first (and the second's second-closest face) are
+ * found by closest_faces. */
+__device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, const Shape& B, uint32_t mask) {
   PColl res{false, C_UNDEF, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
-  const int np_all = __popcll(mask);
+  const int np_all = __popc(mask);
   if (np_all == 0) return res;
-  const int v0 = __ffsll((unsigned long long)mask) - 1;
-  const uint64_t rest = mask & (mask - 1ull);
-  const int v1 = rest ? __ffsll((unsigned long long)rest) - 1 : v0;
+  const int v0 = __ffs(mask) - 1;
+  const uint32_t rest = mask & (mask - 1u);
+  const int v1 = rest ? __ffs(rest) - 1 : v0;
+  const InPoly t = closest_faces(PosA, A, v0, v1, B);
   InPoly s0;
-  s0.f0 = __shfl(t.f0, v0 + off, 64); s0.f1 = __shfl(t.f1, v0 + off, 64); s0.f2 = __shfl(t.f2, v0 + off, 64);
-  s0.d0 = __shfl(t.d0, v0 + off, 64); s0.d1 = __shfl(t.d1, v0 + off, 64); s0.d2 = __shfl(t.d2, v0 + off, 64);
-  const int s1f1 = __shfl(t.f1, v1 + off, 64);
+  s0.f0 = __shfl(t.f0, 0, 64); s0.f1 = __shfl(t.f1, 0, 64); s0.f2 = __shfl(t.f2, 0, 64);
+  s0.d0 = __shfl(t.d0, 0, 64); s0.d1 = __shfl(t.d1, 0, 64); s0.d2 = __shfl(t.d2, 0, 64);
+  const int s1f1 = __shfl(t.f1, 32, 64);
   const int np = np_all < 8 ? np_all : 8;
   res.hit = true;
   if (np == 1) {
@@ -227,10 +267,10 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
   }
   res.cls = C_FF;
   f3 sum = mk3(0.0f, 0.0f, 0.0f);
-  uint64_t m = mask;
+  uint32_t m = mask;
   for (int k = 0; k < np; ++k) {
-    const int v = __ffsll((unsigned long long)m) - 1;
-    m &= m - 1ull;
+    const int v = __ffs(m) - 1;
+    m &= m - 1u;
     sum = sum + ld3(A.V + v);
   }
   res.cp = sum / (float)np;
@@ -246,24 +286,11 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
   return res;   /* face logic fall-through: depth 0, zero normal */
 }
 
-/* Vertices of A in B, and if none, vertices of B in A (normal reversed):
- * both inside tests run at once, lanes 0..31 on A's vertices, lanes 32..63
- * on B's (at most 32 vertices each). */
+/* Vertices of A in B, and if none, vertices of B in A (normal reversed). */
 __device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, f3 PosB, const Shape& B) {
-  const int lane = threadIdx.x & 63;
-  const bool hb = lane >= 32;
-  const int v = lane & 31;
-  InPoly t{-1, -1, -1, 10.0f, 10.0f, 10.0f};
-  bool inside = false;
-  if (v < (hb ? B.nv : A.nv)) {
-    const Shape& own = hb ? B : A;
-    const Shape& other = hb ? A : B;
-    inside = vertex_in_poly(hb ? PosB : PosA, ld3(own.V + v), other, t);
-  }
-  const uint64_t mask = __ballot(inside);
-  PColl c = vertex_faces_select(PosA, A, B, mask & 0xFFFFFFFFull, 0, t);
+  PColl c = vertex_faces_select(PosA, A, B, vertices_inside(A, B));
   if (!c.hit) {
-    c = vertex_faces_select(PosB, B, A, mask >> 32, 32, t);
+    c = vertex_faces_select(PosB, B, A, vertices_inside(B, A));
     c.n = c.n * -1.0f;   /* B into A: normal reversed */
   }
   return c;
@@ -1070,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict
                                                      float4* __restrict__ rot, uint32_t* __restrict__ qcount) {
   const DevParams& P = *Pp;
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) { qcount[0] = 0u; qcount[1] = 0u; }   /* this step's queues start empty */
+  if (t == 0) { qcount[0] = 0u; qcount[1] = 0u; qcount[2] = 0u; }   /* this step's queues start empty */
   const size_t i = t / (size_t)L.stride;
   const int e = (int)(t - i * (size_t)L.stride);
   if (i >= (size_t)n) return;
@@ -1143,7 +1170,8 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
  * where a wave per pair runs the full separating-plane test, then the
  * vertex-face and edge-edge searches. */
 __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict__ Pp,
-                                                    const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
+                                                    const bdem_poly_shape* __restrict__ shapes,
+                                                    const ShapeHdr* __restrict__ hdr, int n, PolyLayout L,
                                                     const float4* __restrict__ rot,
                                                     const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                     const float4* __restrict__ omg,
@@ -1168,11 +1196,10 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
       slot = queue[q];
       const PairOps o = pair_ops(P, slot, list32, pos, vel, omg);
       partner = (uint32_t)o.b;
-      const bdem_poly_shape& pa = shapes[o.sa];
-      const bdem_poly_shape& pb = shapes[o.sb];
-      const Shape A = shape_of(rot, L, o.a, pa);
-      const Shape B = shape_of(rot, L, o.b, pb);
-      const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
+      const ShapeHdr ha = hdr[o.sa], hb = hdr[o.sb];
+      const Shape A = shape_of(rot, L, o.a, ha);
+      const Shape B = shape_of(rot, L, o.b, hb);
+      const float vbA = ha.vbound * P.vbound_scale, vbB = hb.vbound * P.vbound_scale;
       /* the pair's hint, kept by P_ID and list slot (rows past n: no hint) */
       const int ida = packed_pid(omg[o.a].w);
       const uint32_t idb = (uint32_t)packed_pid(omg[o.b].w);
@@ -1186,7 +1213,8 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
       if (!sep) sep = quick_separated(A, B, o.PosA, o.PosB, vbA, vbB, code);
       if (hp) *hp = make_uint2(idb, sep ? code : HINT_NONE);
       need = !sep;
-      if (!need) pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
+      if (!need)
+        pair_store(P, shapes[o.sa], o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
     }
     const uint64_t mask = __ballot(need);
     if (mask) {
@@ -1221,19 +1249,27 @@ struct PairStage {
  * into its LDS stage, so the searches read LDS instead of chains of
  * dependent global loads.
  *
+ * A wave takes its pairs from the queue in chunks by an atomic counter
+ * (qcount[2]), guided: a chunk is about 1 / (2 x waves) of what is left, 4
+ * to 64 pairs, so waves that drew expensive pairs (the edge-edge search)
+ * do not hold up the launch.
+ *
  * Staging is one round trip to memory per pair.  Each lane fetches one of
- * the wave's next 64 queue entries and the shape rows of its two bodies
- * ahead of time; a pair's values are then read from that lane, so they are
+ * the chunk's queue entries and the shape rows of its two bodies ahead of
+ * time; a pair's values are then read from that lane, so they are
  * wave-uniform (scalar loads for the pair's state and shape counts), and
  * every row of the stage is loaded without a condition on the shape's
  * counts (lanes 0-31 body A, 32-63 body B; rows past the counts are inside
- * the record and never read) before the first LDS write. */
-__global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict__ Pp,
-                                                     const bdem_poly_shape* __restrict__ shapes, int n, PolyLayout L,
+ * the record and never read) before the first LDS write.
+ *
+ * Five waves per SIMD: 96 VGPRs without spills, 5 x 21 KiB of LDS per CU. */
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_poly_narrow(const DevParams* __restrict__ Pp,
+                                                     const bdem_poly_shape* __restrict__ shapes,
+                                                     const ShapeHdr* __restrict__ hdr, int n, PolyLayout L,
                                                      const float4* __restrict__ rot,
                                                      const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                      const float4* __restrict__ omg,
-                                                     const uint32_t* __restrict__ qcount,
+                                                     uint32_t* __restrict__ qcount,
                                                      const uint2* __restrict__ equeue,
                                                      float4* __restrict__ pf, float4* __restrict__ pt, int energy,
                                                      uint2* __restrict__ hint) {
@@ -1247,13 +1283,19 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
   const uint32_t cnt = qcount[1] < cap ? qcount[1] : cap;
   const uint32_t waves = gridDim.x * (blockDim.x / 64u);
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6));
-  /* the wave's pairs are q = wave + j * waves, j = 0, 1, ... */
-  for (uint32_t base = wave; base < cnt; base += 64u * waves) {
+  uint32_t seen = 0u;   /* the counter's value at this wave's last grab */
+  for (;;) {
+    uint32_t size = (cnt > seen ? cnt - seen : 0u) / (2u * waves);
+    size = size < 4u ? 4u : (size > 64u ? 64u : size);
+    uint32_t base = 0u;
+    if (lane == 0) base = atomicAdd(&qcount[2], size);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base >= cnt) break;
+    seen = base + size;
+    const uint32_t m = cnt - base < size ? cnt - base : size;
     uint32_t lslot = cap, lpart = 0u, lsa = 0u, lsb = 0u;
-    const uint32_t ql = base + (uint32_t)lane * waves;
-    if (ql < cnt) {
-      const uint2 item = equeue[ql];   /* (slot, partner) */
+    if ((uint32_t)lane < m) {
+      const uint2 item = equeue[base + (uint32_t)lane];   /* (slot, partner) */
       if (item.x < cap && item.y < (uint32_t)n) {
         lslot = item.x;
         lpart = item.y;
@@ -1261,13 +1303,14 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
         lsb = (uint32_t)shape_index(P, omg[item.y].w);
       }
     }
-    for (int j = 0; j < 64; ++j) {
-      if (base + (uint32_t)j * waves >= cnt) break;
+    for (int j = 0; j < (int)m; ++j) {
       const uint32_t slot = __builtin_amdgcn_readlane(lslot, j);
       if (slot >= cap) continue;
       const uint32_t partner = __builtin_amdgcn_readlane(lpart, j);
-      const bdem_poly_shape& pa = shapes[__builtin_amdgcn_readlane(lsa, j)];
-      const bdem_poly_shape& pb = shapes[__builtin_amdgcn_readlane(lsb, j)];
+      const int sa = __builtin_amdgcn_readlane(lsa, j), sb = __builtin_amdgcn_readlane(lsb, j);
+      const bdem_poly_shape& pa = shapes[sa];
+      const bdem_poly_shape& pb = shapes[sb];
+      const ShapeHdr ha = hdr[sa], hb = hdr[sb];
       /* turned rows at the table's largest counts, edges and face polygons */
       const float4* rh = rot + (size_t)(h ? partner : slot / LIST_STRIDE) * L.stride;
       const float4 v4 = rh[kv], n4 = rh[L.pv + kf], c4 = rh[L.pv + L.pf + kf];
@@ -1282,9 +1325,7 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
       const float farea = F.area;
       const int fnv = F.num_vertex;
       const PairOps o = pair_ops_ab(P, slot, partner, pos, vel, omg);
-      const int ea = pa.num_edges, eb = pb.num_edges;
-      const int nva = pa.num_vertex, nfa = pa.num_faces, nvb = pb.num_vertex, nfb = pb.num_faces;
-      const float vbA = pa.vbound * P.vbound_scale, vbB = pb.vbound * P.vbound_scale;
+      const float vbA = ha.vbound * P.vbound_scale, vbB = hb.vbound * P.vbound_scale;
       /* the previous pair's reads are done before the stage is overwritten */
       __builtin_amdgcn_wave_barrier();
       S.v[h][k] = v4;
@@ -1296,8 +1337,8 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
       S.area[h][k] = farea;
       S.fnv[h][k] = fnv;
       __builtin_amdgcn_wave_barrier();
-      const Shape A{S.v[0], S.fn[0], S.fc[0], nva, nfa, S.vo[0], S.area[0], S.fnv[0]};
-      const Shape B{S.v[1], S.fn[1], S.fc[1], nvb, nfb, S.vo[1], S.area[1], S.fnv[1]};
+      const Shape A{S.v[0], S.fn[0], S.fc[0], ha.nv, ha.nf, S.vo[0], S.area[0], S.fnv[0]};
+      const Shape B{S.v[1], S.fn[1], S.fc[1], hb.nv, hb.nf, S.vo[1], S.area[1], S.fnv[1]};
       const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB);
       if (code != HINT_NONE) {
         if (lane == 0) {
@@ -1311,9 +1352,9 @@ __global__ __launch_bounds__(256) void k_poly_narrow(const DevParams* __restrict
       PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
       if (!c.hit) {
         const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
-        c = edges_wave(A, S.e[0], ea, B, S.e[1], eb, RelPos, o.PosA, o.PosB, vbA, vbB,
-                       pa.plane_slack * P.vbound_scale, pb.plane_slack * P.vbound_scale, S.ia, S.ib);
-        if (c.hit) c.dis = fminf(0.025f * pa.radius, c.dis);
+        c = edges_wave(A, S.e[0], ha.ne, B, S.e[1], hb.ne, RelPos, o.PosA, o.PosB, vbA, vbB,
+                       ha.plane_slack * P.vbound_scale, hb.plane_slack * P.vbound_scale, S.ia, S.ib);
+        if (c.hit) c.dis = fminf(0.025f * ha.radius, c.dis);
       }
       if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt, tally);
     }
@@ -1431,9 +1472,9 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
   hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
-  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
+  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt, pb.hint);
-  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, b.n, pb.L, pb.rot,
+  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt, energy ? 1 : 0, pb.hint);
 }
 
