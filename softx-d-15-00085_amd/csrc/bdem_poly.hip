@@ -126,6 +126,68 @@ __device__ __forceinline__ bool point_in_poly(const Shape& A, f3 point) {
   return true;
 }
 
+/* Wave reductions by DPP moves inside each row of 16 lanes (swap
+ * neighbours, swap pairs, rotate the row by 4, then by 8: every lane then
+ * holds its row's result) and readlane across the four rows: a few cycles
+ * a step, where a shuffle goes through the LDS crossbar.  Every lane of the
+ * wave must be active; the results are wave-uniform. */
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_ROR4 = 0x124, DPP_ROR8 = 0x128;
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) { return __uint_as_float(dpp_u32<CTRL>(__float_as_uint(v))); }
+__device__ __forceinline__ float lane_f32(float v, int l) {
+  return __uint_as_float((uint32_t)__builtin_amdgcn_readlane((int)__float_as_uint(v), l));
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+  v = fminf(v, dpp_f32<DPP_XOR1>(v));
+  v = fminf(v, dpp_f32<DPP_XOR2>(v));
+  v = fminf(v, dpp_f32<DPP_ROR4>(v));
+  v = fminf(v, dpp_f32<DPP_ROR8>(v));
+  return fminf(fminf(lane_f32(v, 0), lane_f32(v, 16)), fminf(lane_f32(v, 32), lane_f32(v, 48)));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dpp_f32<DPP_XOR1>(v));
+  v = fmaxf(v, dpp_f32<DPP_XOR2>(v));
+  v = fmaxf(v, dpp_f32<DPP_ROR4>(v));
+  v = fmaxf(v, dpp_f32<DPP_ROR8>(v));
+  return fmaxf(fmaxf(lane_f32(v, 0), lane_f32(v, 16)), fmaxf(lane_f32(v, 32), lane_f32(v, 48)));
+}
+
+__device__ __forceinline__ int wave_min_i32(int v) {
+  v = min(v, (int)dpp_u32<DPP_XOR1>((uint32_t)v));
+  v = min(v, (int)dpp_u32<DPP_XOR2>((uint32_t)v));
+  v = min(v, (int)dpp_u32<DPP_ROR4>((uint32_t)v));
+  v = min(v, (int)dpp_u32<DPP_ROR8>((uint32_t)v));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+/* The minimum of a 64-bit key over each half of the wave (lanes 0-31,
+ * 32-63), returned to the lanes of that half. */
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_min_u64(uint64_t k) {
+  const uint64_t o = ((uint64_t)dpp_u32<CTRL>((uint32_t)(k >> 32)) << 32) | dpp_u32<CTRL>((uint32_t)k);
+  return o < k ? o : k;
+}
+__device__ __forceinline__ uint64_t half_min_u64(uint64_t k) {
+  k = dpp_min_u64<DPP_XOR1>(k);
+  k = dpp_min_u64<DPP_XOR2>(k);
+  k = dpp_min_u64<DPP_ROR4>(k);
+  k = dpp_min_u64<DPP_ROR8>(k);
+  auto row = [&](int l) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), l) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, l);
+  };
+  const uint64_t r0 = row(0), r1 = row(16), r2 = row(32), r3 = row(48);
+  const uint64_t h0 = r0 < r1 ? r0 : r1, h1 = r2 < r3 ? r2 : r3;
+  return (threadIdx.x & 32) ? h1 : h0;
+}
+
 struct InPoly { int f0, f1, f2; float d0, d1, d2; };
 
 /* The inside part of Collision_Detection_Vertex_InPoly_3Faces
@@ -170,12 +232,7 @@ __device__ __forceinline__ InPoly closest_faces(f3 PosP, const Shape& P, int v, 
   InPoly r{-1, -1, -1, 10.0f, 10.0f, 10.0f};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    uint64_t m = key;
-#pragma unroll
-    for (int d = 1; d < 32; d <<= 1) {   /* minimum over the half */
-      const uint64_t o = __shfl_xor(m, d, 64);
-      m = o < m ? o : m;
-    }
+    const uint64_t m = half_min_u64(key);
     if (m != ~0ull) {   /* the same in every lane of the half */
       const int fm = (int)(uint32_t)m;
       const float dm = __uint_as_float((uint32_t)(m >> 32));
@@ -223,10 +280,14 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
   const uint32_t rest = mask & (mask - 1u);
   const int v1 = rest ? __ffs(rest) - 1 : v0;
   const InPoly t = closest_faces(PosA, A, v0, v1, B);
-  InPoly s0;
-  s0.f0 = __shfl(t.f0, 0, 64); s0.f1 = __shfl(t.f1, 0, 64); s0.f2 = __shfl(t.f2, 0, 64);
-  s0.d0 = __shfl(t.d0, 0, 64); s0.d1 = __shfl(t.d1, 0, 64); s0.d2 = __shfl(t.d2, 0, 64);
-  const int s1f1 = __shfl(t.f1, 32, 64);
+  InPoly s0;   /* t is uniform over each half: lane 0 holds v0's faces, lane 32 v1's */
+  s0.f0 = __builtin_amdgcn_readlane(t.f0, 0);
+  s0.f1 = __builtin_amdgcn_readlane(t.f1, 0);
+  s0.f2 = __builtin_amdgcn_readlane(t.f2, 0);
+  s0.d0 = lane_f32(t.d0, 0);
+  s0.d1 = lane_f32(t.d1, 0);
+  s0.d2 = lane_f32(t.d2, 0);
+  const int s1f1 = __builtin_amdgcn_readlane(t.f1, 32);
   const int np = np_all < 8 ? np_all : 8;
   res.hit = true;
   if (np == 1) {
@@ -328,19 +389,11 @@ __device__ __forceinline__ bool edge_pair_dis(const Shape& A, uint2 ea, const Sh
   return true;
 }
 
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
 /* The face of S whose normal points most along d (lane f tests face f). */
 __device__ __forceinline__ int facing_face_wave(const Shape& S, f3 d) {
   const int lane = threadIdx.x & 63;
   const float v = lane < S.nf ? dot(ld3(S.Fn + lane), d) : -3.0e38f;
-  float m = v;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  const float m = wave_max(v);
   const uint64_t b = __ballot(lane < S.nf && v == m);
   return b ? __ffsll((unsigned long long)b) - 1 : 0;
 }
@@ -424,9 +477,7 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
   sweep(__ballot(lane < ea && la <= w2), __ballot(lane < eb && lb <= w2), SA1, SB1);
   /* wave minimum of (distance, t) */
   const float m = wave_min(best);
-  int tm = (best == m) ? bt : 0x7FFFFFFF;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) tm = min(tm, __shfl_xor(tm, o, 64));
+  const int tm = wave_min_i32((best == m) ? bt : 0x7FFFFFFF);
   PColl res{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.0f, 0.0f, 0.0f)};
   if (!(m < 5.0f) || tm == 0x7FFFFFFF) return res;
   /* the selected pair, recomputed by every lane */
