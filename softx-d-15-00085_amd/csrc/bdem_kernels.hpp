@@ -81,7 +81,7 @@ struct PolyBuffers {
   float4 *pf, *pt;                     /* n * 32: pair force (w = 1 when in contact), torque */
   uint2* hint;                         /* n * 32 by P_ID and list slot: (partner P_ID, separating face code) */
 };
-constexpr int POLY_PAIR_BLOCKS = 2048;
+constexpr int POLY_PAIR_BLOCKS = 2048;   /* a multiple of the 8 XCDs */
 
 constexpr int SORT_THREADS = 256;
 constexpr int SORT_ITEMS = 16;

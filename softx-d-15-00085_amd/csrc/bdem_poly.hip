@@ -188,28 +188,68 @@ __device__ __forceinline__ uint64_t half_min_u64(uint64_t k) {
   return (threadIdx.x & 32) ? h1 : h0;
 }
 
+/* The face and the vertex of each body of a pair turned most towards the
+ * other: largest n . (other centre - own centre), resp.
+ * (v - own centre) . (other centre - own centre), the lowest index on ties;
+ * lanes 0-31 work on A, 32-63 on B.  They only choose which face or vertex
+ * a test tries first; no result depends on them. */
+struct PairDirs { int fA, fB, vA, vB; };
+__device__ __forceinline__ uint64_t argmax_key(bool valid, float v, int idx) {
+  if (!valid) return ~0ull;
+  uint32_t u = __float_as_uint(v);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);   /* ordered as the floats */
+  return ((uint64_t)(~u) << 32) | (uint32_t)idx;    /* smallest key: largest value, then lowest index */
+}
+__device__ __forceinline__ PairDirs pair_dirs(const Shape& A, const Shape& B, f3 PosA, f3 PosB) {
+  const int lane = threadIdx.x & 63;
+  const bool hb = lane >= 32;
+  const int k = lane & 31;
+  const Shape& own = hb ? B : A;
+  const f3 Pown = hb ? PosB : PosA;
+  const f3 d = hb ? PosA - PosB : PosB - PosA;
+  const bool fk = k < own.nf, vk = k < own.nv;
+  const uint64_t kf = half_min_u64(argmax_key(fk, dot(ld3(own.Fn + (fk ? k : 0)), d), k));
+  const uint64_t kv = half_min_u64(argmax_key(vk, dot(ld3(own.V + (vk ? k : 0)) - Pown, d), k));
+  auto pick = [](uint64_t key, int l) {
+    const int i = __builtin_amdgcn_readlane((int)(uint32_t)key, l);
+    return (i >= 0 && i < 32) ? i : 0;
+  };
+  return PairDirs{pick(kf, 0), pick(kf, 32), pick(kv, 0), pick(kv, 32)};
+}
+
 struct InPoly { int f0, f1, f2; float d0, d1, d2; };
 
 /* The inside part of Collision_Detection_Vertex_InPoly_3Faces
  * (Collision_Detection_Polyhedra.cuh:287-375) for every vertex of P against
- * Q, a lane per (vertex, face): vertex v is inside when every face of Q has
- * it behind its plane, vd = n . (v - c) < -1e-5 (the serial scan stops at
- * the first face that does not).  Lanes 0-31 take face f = lane of Q with
- * vertex 2r, lanes 32-63 the same face with vertex 2r + 1; bit v of the
- * result is set for an inside vertex. */
-__device__ __forceinline__ uint32_t vertices_inside(const Shape& P, const Shape& Q) {
+ * Q: vertex v is inside when every face of Q has it behind its plane,
+ * vd = n . (v - c) < -1e-5 (the serial scan stops at the first face that
+ * does not).  Each vertex is first tried against face fq of Q (the face
+ * turned towards P, which most of P's vertices are in front of), a lane per
+ * vertex; the rest take every face, a lane per (vertex, face): lanes 0-31
+ * face f = lane with one vertex, lanes 32-63 the same face with the next.
+ * Bit v of the result is set for an inside vertex. */
+__device__ __forceinline__ uint32_t vertices_inside(const Shape& P, const Shape& Q, int fq) {
   const int lane = threadIdx.x & 63;
+  uint32_t rest;
+  {
+    const int v = lane < P.nv ? lane : 0;
+    const float vd = dot(ld3(Q.Fn + fq), ld3(P.V + v) - ld3(Q.Fc + fq));
+    rest = (uint32_t)__ballot(lane < P.nv && vd < -0.0000100f);
+  }
   const int f = lane & 31, h = lane >> 5;
   const bool fv = f < Q.nf;
   const int fr = fv ? f : 0;
   const f3 n = ld3(Q.Fn + fr), c = ld3(Q.Fc + fr);
   uint32_t inside = 0u;
-  for (int v0 = 0; v0 < P.nv; v0 += 2) {
-    const int v = v0 + h < P.nv ? v0 + h : v0;
-    const float vd = dot(n, ld3(P.V + v) - c);
+  while (rest) {
+    const int v0 = __ffs(rest) - 1;
+    rest &= rest - 1u;
+    const int v1 = rest ? __ffs(rest) - 1 : -1;
+    if (v1 >= 0) rest &= rest - 1u;
+    const float vd = dot(n, ld3(P.V + (h && v1 >= 0 ? v1 : v0)) - c);
     const uint64_t out = __ballot(fv && !(vd < -0.0000100f));
     if ((uint32_t)out == 0u) inside |= 1u << v0;
-    if (v0 + 1 < P.nv && (uint32_t)(out >> 32) == 0u) inside |= 1u << (v0 + 1);
+    if (v1 >= 0 && (uint32_t)(out >> 32) == 0u) inside |= 1u << v1;
   }
   return inside;
 }
@@ -348,10 +388,11 @@ __device__ __forceinline__ PColl vertex_faces_select(f3 PosA, const Shape& A, co
 }
 
 /* Vertices of A in B, and if none, vertices of B in A (normal reversed). */
-__device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, f3 PosB, const Shape& B) {
-  PColl c = vertex_faces_select(PosA, A, B, vertices_inside(A, B));
+__device__ __forceinline__ PColl vertex_faces_pair(f3 PosA, const Shape& A, f3 PosB, const Shape& B,
+                                                   const PairDirs& D) {
+  PColl c = vertex_faces_select(PosA, A, B, vertices_inside(A, B, D.fB));
   if (!c.hit) {
-    c = vertex_faces_select(PosB, B, A, vertices_inside(B, A));
+    c = vertex_faces_select(PosB, B, A, vertices_inside(B, A, D.fA));
     c.n = c.n * -1.0f;   /* B into A: normal reversed */
   }
   return c;
@@ -387,15 +428,6 @@ __device__ __forceinline__ bool edge_pair_dis(const Shape& A, uint2 ea, const Sh
   if (!(dis > 0.00000f)) return false;
   dis = __builtin_sqrtf(dis);
   return true;
-}
-
-/* The face of S whose normal points most along d (lane f tests face f). */
-__device__ __forceinline__ int facing_face_wave(const Shape& S, f3 d) {
-  const int lane = threadIdx.x & 63;
-  const float v = lane < S.nf ? dot(ld3(S.Fn + lane), d) : -3.0e38f;
-  const float m = wave_max(v);
-  const uint64_t b = __ballot(lane < S.nf && v == m);
-  return b ? __ffsll((unsigned long long)b) - 1 : 0;
 }
 
 /* A lower bound of |pA - pB| over every pA of the edge [p0, p1] extended by
@@ -434,10 +466,8 @@ __device__ __forceinline__ float edge_bound(f3 p0, f3 p1, f3 c, float vb, f3 n, 
  * recomputes with the same arithmetic.  The result is wave-uniform. */
 __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int ea, const Shape& B, const uint2* EB,
                                             int eb, f3 RelPos, f3 PosA, f3 PosB, float vbA, float vbB, float slA,
-                                            float slB, unsigned char* ia, unsigned char* ib) {
+                                            float slB, unsigned char* ia, unsigned char* ib, int fA, int fB) {
   const int lane = threadIdx.x & 63;
-  const int fB = facing_face_wave(B, PosA - PosB);
-  const int fA = facing_face_wave(A, PosB - PosA);
   float la = 3.0e38f, lb = 3.0e38f;
   if (lane < ea)
     la = edge_bound(ld3(A.V + EA[lane].x), ld3(A.V + EA[lane].y), PosB, vbB, ld3(B.Fn + fB), ld3(B.Fc + fB), slB);
@@ -522,12 +552,15 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
  * B's vertex bound: the sphere shortcut changes which face is found first,
  * never whether one is). */
 template <bool BATCHED = true>
-__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb) {
+__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb,
+                                               int first = 0) {
   const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
   const float dist = dot(n, PosB - c);
   if (!(dist > 0.0f)) return false;
   if (beyond_by_sphere(dist, c, PosB, vb)) return true;
   if (!BATCHED) {   /* shapes staged in LDS: one vertex at a time costs fewer instructions */
+    /* vertex `first` (B's vertex turned towards A) is tried before the others */
+    if (dot(n, ld3(B.V + first) - c) <= 0.00000f) return false;
     for (int v = 0; v < B.nv; ++v)
       if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
     return true;
@@ -601,7 +634,7 @@ __device__ __forceinline__ bool hint_separates(uint2 h, uint32_t pid_b, const Sh
 /* is_SP_Poly_Poly both ways with a lane per face: lanes 0..31 test A's
  * faces against B, lanes 32..63 B's faces against A (at most 32 faces). */
 __device__ __forceinline__ uint32_t separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
-                                                   float vbB) {
+                                                   float vbB, const PairDirs& D) {
   const int lane = threadIdx.x & 63;
   const bool hb = lane >= 32;
   const int f = lane & 31;
@@ -609,7 +642,7 @@ __device__ __forceinline__ uint32_t separated_wave(const Shape& A, const Shape& 
   if (f < (hb ? B.nf : A.nf)) {
     const Shape& own = hb ? B : A;
     const Shape& other = hb ? A : B;
-    sep = face_separates<false>(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB);
+    sep = face_separates<false>(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB, hb ? D.vA : D.vB);
   }
   const uint64_t m = __ballot(sep);
   if (!m) return HINT_NONE;
@@ -1236,11 +1269,17 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
   const uint32_t cap = (uint32_t)n * LIST_STRIDE;
   const uint32_t cnt = total < cap ? total : cap;
   const int lane = threadIdx.x & 63;
-  /* every lane of a wave runs the same number of iterations (the bound is
-   * rounded up to whole waves) so that the append below sees the whole wave */
-  const uint32_t stride = gridDim.x * blockDim.x;
+  /* Each XCD (workgroup w runs on XCD w mod 8) takes one contiguous eighth
+   * of the queue, and its blocks stride through it together, so the turned
+   * shapes of neighbouring particles are read through that XCD's L2.  Every
+   * lane of a wave runs the same number of iterations (the bounds are whole
+   * waves) so that the append below sees the whole wave. */
+  const uint32_t xcd = blockIdx.x % NUM_XCD, per = gridDim.x / NUM_XCD;
   const uint32_t cnt_w = (cnt + 63u) & ~63u;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt_w; q += stride) {
+  const uint32_t chunk = ((cnt_w / 64u + NUM_XCD - 1u) / NUM_XCD) * 64u;
+  const uint32_t lo = xcd * chunk, hi = lo + chunk < cnt_w ? lo + chunk : cnt_w;
+  const uint32_t stride = per * blockDim.x;
+  for (uint32_t q = lo + (blockIdx.x / NUM_XCD) * blockDim.x + threadIdx.x; q < hi; q += stride) {
     bool need = false;
     uint32_t slot = 0u, partner = 0u;
     if (q < cnt) {
@@ -1282,6 +1321,10 @@ __global__ __launch_bounds__(256) void k_poly_pairs(const DevParams* __restrict_
 
 /* One pair's bodies staged in LDS by its wave: turned vertices, face
  * normals and centroids, and edge lists of A (row 0) and B (row 1). */
+/* A pair's contact, kept until the chunk's pair law runs. */
+struct PairHit { uint32_t hit; float dis, nx, ny, nz, cx, cy, cz; };
+constexpr uint32_t NARROW_CHUNK = 32;   /* largest chunk of the narrow kernel's queue */
+
 struct PairStage {
   float4 v[2][BDEM_SHAPE_MAX_VERTEX];
   float4 fn[2][BDEM_SHAPE_MAX_FACE];
@@ -1302,8 +1345,9 @@ struct PairStage {
  *
  * A wave takes its pairs from the queue in chunks by an atomic counter
  * (qcount[2]), guided: a chunk is about 1 / (2 x waves) of what is left, 4
- * to 64 pairs, so waves that drew expensive pairs (the edge-edge search)
- * do not hold up the launch.
+ * to 32 pairs, so waves that drew expensive pairs (the edge-edge search)
+ * do not hold up the launch.  The searches leave each pair's contact in
+ * LDS, and the pair law then runs for the whole chunk, a lane per pair.
  *
  * Staging is one round trip to memory per pair.  Each lane fetches one of
  * the chunk's queue entries and the shape rows of its two bodies ahead of
@@ -1327,7 +1371,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const DevParams& P = *Pp;
   float* const tally = energy ? P.tally : nullptr;   /* dissipation tallies (EnergyCalc) */
   __shared__ PairStage stage[4];
+  __shared__ PairHit hits[4][NARROW_CHUNK];
   PairStage& S = stage[threadIdx.x >> 6];
+  PairHit* const R = hits[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5, k = lane & 31;
   const int kv = k < L.pv ? k : L.pv - 1, kf = k < L.pf ? k : L.pf - 1;
@@ -1337,7 +1383,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   uint32_t seen = 0u;   /* the counter's value at this wave's last grab */
   for (;;) {
     uint32_t size = (cnt > seen ? cnt - seen : 0u) / (2u * waves);
-    size = size < 4u ? 4u : (size > 64u ? 64u : size);
+    size = size < 4u ? 4u : (size > NARROW_CHUNK ? NARROW_CHUNK : size);
     uint32_t base = 0u;
     if (lane == 0) base = atomicAdd(&qcount[2], size);
     base = __builtin_amdgcn_readfirstlane(base);
@@ -1390,25 +1436,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       __builtin_amdgcn_wave_barrier();
       const Shape A{S.v[0], S.fn[0], S.fc[0], ha.nv, ha.nf, S.vo[0], S.area[0], S.fnv[0]};
       const Shape B{S.v[1], S.fn[1], S.fc[1], hb.nv, hb.nf, S.vo[1], S.area[1], S.fnv[1]};
-      const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB);
+      const PairDirs D = pair_dirs(A, B, o.PosA, o.PosB);
+      const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB, D);
+      PColl c{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
       if (code != HINT_NONE) {
         if (lane == 0) {
-          pair_store(P, pa, o, PColl{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)}, slot, pf, pt);
           const int ida = packed_pid(omg[o.a].w);
           if (ida >= 0 && ida < n)   /* the next step tries this face first */
             hint[(size_t)ida * LIST_STRIDE + (slot % LIST_STRIDE)] = make_uint2((uint32_t)packed_pid(omg[o.b].w), code);
         }
-        continue;
+      } else {
+        c = vertex_faces_pair(o.PosA, A, o.PosB, B, D);
+        if (!c.hit) {
+          const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
+          c = edges_wave(A, S.e[0], ha.ne, B, S.e[1], hb.ne, RelPos, o.PosA, o.PosB, vbA, vbB,
+                         ha.plane_slack * P.vbound_scale, hb.plane_slack * P.vbound_scale, S.ia, S.ib, D.fA, D.fB);
+          if (c.hit) c.dis = fminf(0.025f * ha.radius, c.dis);
+        }
       }
-      PColl c = vertex_faces_pair(o.PosA, A, o.PosB, B);
-      if (!c.hit) {
-        const f3 RelPos = (o.PosA - o.PosB) / length(o.PosA - o.PosB);
-        c = edges_wave(A, S.e[0], ha.ne, B, S.e[1], hb.ne, RelPos, o.PosA, o.PosB, vbA, vbB,
-                       ha.plane_slack * P.vbound_scale, hb.plane_slack * P.vbound_scale, S.ia, S.ib);
-        if (c.hit) c.dis = fminf(0.025f * ha.radius, c.dis);
-      }
-      if (lane == 0) pair_store(P, pa, o, c, slot, pf, pt, tally);
+      /* the pair law runs for the whole chunk at once below, a lane per pair */
+      if (lane == 0) R[j] = PairHit{c.hit ? 1u : 0u, c.dis, c.n.x, c.n.y, c.n.z, c.cp.x, c.cp.y, c.cp.z};
     }
+    __builtin_amdgcn_wave_barrier();
+    if ((uint32_t)lane < m && lslot < cap) {
+      const PairHit r = R[lane];
+      const PairOps o = pair_ops_ab(P, lslot, lpart, pos, vel, omg);
+      pair_store(P, shapes[lsa], o, PColl{r.hit != 0u, C_UNDEF, r.dis, mk3(r.nx, r.ny, r.nz), mk3(r.cx, r.cy, r.cz)},
+                 lslot, pf, pt, tally);
+    }
+    __builtin_amdgcn_wave_barrier();   /* the records are read before the next chunk writes them */
   }
 }
 
@@ -1523,6 +1579,7 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
   hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
+  static_assert(POLY_PAIR_BLOCKS % NUM_XCD == 0, "the pair kernel splits its queue by XCD");
   hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt, pb.hint);
   hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
