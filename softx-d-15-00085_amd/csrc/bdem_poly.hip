@@ -119,13 +119,6 @@ __device__ __forceinline__ bool beyond_by_sphere(float dist, f3 c, f3 PosB, floa
   return dist > vb + 0.001f + 4.0e-6f * size;
 }
 
-/* isPointinPoly (Utilities.cuh:63-98) */
-__device__ __forceinline__ bool point_in_poly(const Shape& A, f3 point) {
-  for (int f = 0; f < A.nf; ++f)
-    if (!(dot(ld3(A.Fn + f), point - ld3(A.Fc + f)) <= 0.000100f)) return false;
-  return true;
-}
-
 /* Wave reductions by DPP moves inside each row of 16 lanes (swap
  * neighbours, swap pairs, rotate the row by 4, then by 8: every lane then
  * holds its row's result) and readlane across the four rows: a few cycles
@@ -478,20 +471,23 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
   float best = 5.0f;
   int bt = 0x7FFFFFFF;
   const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  /* the pairs of SA x SB not in XA x XB, from compacted index lists */
-  auto sweep = [&](uint64_t SA, uint64_t SB, uint64_t XA, uint64_t XB) {
+  /* the pairs of SA x SB, from compacted index lists; lane u takes pairs
+   * u, u + 64, ... in row-major order (64 = qa nb + rb) */
+  auto sweep = [&](uint64_t SA, uint64_t SB) {
     __builtin_amdgcn_wave_barrier();
     if ((SA >> lane) & 1ull) ia[__popcll(SA & lt)] = (unsigned char)lane;
     if ((SB >> lane) & 1ull) ib[__popcll(SB & lt)] = (unsigned char)lane;
     __builtin_amdgcn_wave_barrier();
     const int na = __popcll(SA), nb = __popcll(SB);
     const int total = na * nb;
-    int a = nb > 0 ? lane / nb : 0, b = lane - a * nb;
+    if (total == 0) return;
+    const int qa = 64 / nb, rb = 64 - qa * nb;
+    int a = lane / nb, b = lane - a * nb;
     for (int u = lane; u < total; u += 64) {
       const int i = ia[a], j = ib[b];
-      b += 64;
-      while (b >= nb) { b -= nb; ++a; }
-      if (((XA >> i) & 1ull) && ((XB >> j) & 1ull)) continue;   /* evaluated in pass 1 */
+      a += qa;
+      b += rb;
+      if (b >= nb) { b -= nb; ++a; }
       float dis;
       if (edge_pair_dis(A, EA[i], B, EB[j], dis) && dis < 5.0f) {
         const int t = i * eb + j;
@@ -502,9 +498,13 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
   const float t0 = fmaxf(wave_min(la), wave_min(lb));
   const float w1 = fmaxf(t0, 0.0f) + 0.05f * fminf(vbA, vbB) + margin;
   const uint64_t SA1 = __ballot(lane < ea && la <= w1), SB1 = __ballot(lane < eb && lb <= w1);
-  sweep(SA1, SB1, 0ull, 0ull);
+  sweep(SA1, SB1);
+  /* pass 2: the pairs of SA2 x SB2 that pass 1 left, as (SA2 \ SA1) x SB2
+   * and (SA2 n SA1) x (SB2 \ SB1) */
   const float w2 = wave_min(best) + margin;
-  sweep(__ballot(lane < ea && la <= w2), __ballot(lane < eb && lb <= w2), SA1, SB1);
+  const uint64_t SA2 = __ballot(lane < ea && la <= w2), SB2 = __ballot(lane < eb && lb <= w2);
+  if ((SA2 & ~SA1) && SB2) sweep(SA2 & ~SA1, SB2);
+  if ((SA2 & SA1) && (SB2 & ~SB1)) sweep(SA2 & SA1, SB2 & ~SB1);
   /* wave minimum of (distance, t) */
   const float m = wave_min(best);
   const int tm = wave_min_i32((best == m) ? bt : 0x7FFFFFFF);
@@ -531,7 +531,9 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
   res.n = cross(DA, DB);
   res.n = res.n * (1.0f / length(res.n));
   if (length(PosB - selB) < length(PosB - selA)) {
-    if (!point_in_poly(B, selA)) return res;
+    /* isPointinPoly (Utilities.cuh:63-98), a lane per face of B */
+    const int f = lane < B.nf ? lane : 0;
+    if (__ballot(lane < B.nf && !(dot(ld3(B.Fn + f), selA - ld3(B.Fc + f)) <= 0.000100f))) return res;
   }
   res.hit = true;
   res.cls = C_EE;
