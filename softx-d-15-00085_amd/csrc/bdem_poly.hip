@@ -553,20 +553,11 @@ __device__ __forceinline__ PColl edges_wave(const Shape& A, const uint2* EA, int
  * front (the per-face test of is_SP_Poly_Poly, Utilities.cuh:105-156; vb is
  * B's vertex bound: the sphere shortcut changes which face is found first,
  * never whether one is). */
-template <bool BATCHED = true>
-__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb,
-                                               int first = 0) {
+__device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, const Shape& B, float vb) {
   const f3 n = ld3(A.Fn + f), c = ld3(A.Fc + f);
   const float dist = dot(n, PosB - c);
   if (!(dist > 0.0f)) return false;
   if (beyond_by_sphere(dist, c, PosB, vb)) return true;
-  if (!BATCHED) {   /* shapes staged in LDS: one vertex at a time costs fewer instructions */
-    /* vertex `first` (B's vertex turned towards A) is tried before the others */
-    if (dot(n, ld3(B.V + first) - c) <= 0.00000f) return false;
-    for (int v = 0; v < B.nv; ++v)
-      if (dot(n, ld3(B.V + v) - c) <= 0.00000f) return false;
-    return true;
-  }
   /* four vertices per round, their loads in flight together (indices past
    * the count re-read the last vertex and are not tested) */
   const int last = B.nv - 1;
@@ -633,23 +624,47 @@ __device__ __forceinline__ bool hint_separates(uint2 h, uint32_t pid_b, const Sh
   return f < A.nf && face_separates(A, f, PosB, B, vbB);
 }
 
-/* is_SP_Poly_Poly both ways with a lane per face: lanes 0..31 test A's
- * faces against B, lanes 32..63 B's faces against A (at most 32 faces). */
-__device__ __forceinline__ uint32_t separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB, float vbA,
-                                                   float vbB, const PairDirs& D) {
+/* is_SP_Poly_Poly both ways (Utilities.cuh:105-156): some face of A has
+ * B's centre and every vertex of B strictly in front, or some face of B so
+ * for A.  A face is a candidate when the centre and the other body's vertex
+ * turned towards it (D) are in front, a lane per face (0-31 A's, 32-63
+ * B's); the candidates then take all vertices, two faces per round and a
+ * lane per vertex, in ascending order, so the first separating face found
+ * is the lowest-numbered one.  Returns its code, or HINT_NONE. */
+__device__ __forceinline__ uint32_t separated_wave(const Shape& A, const Shape& B, f3 PosA, f3 PosB,
+                                                   const PairDirs& D) {
   const int lane = threadIdx.x & 63;
-  const bool hb = lane >= 32;
-  const int f = lane & 31;
-  bool sep = false;
-  if (f < (hb ? B.nf : A.nf)) {
+  uint64_t cand;
+  {
+    const bool hb = lane >= 32;
+    const int f = lane & 31;
     const Shape& own = hb ? B : A;
     const Shape& other = hb ? A : B;
-    sep = face_separates<false>(own, f, hb ? PosA : PosB, other, hb ? vbA : vbB, hb ? D.vA : D.vB);
+    const bool fv = f < own.nf;
+    const int fr = fv ? f : 0;
+    const f3 n = ld3(own.Fn + fr), c = ld3(own.Fc + fr);
+    cand = __ballot(fv && dot(n, (hb ? PosA : PosB) - c) > 0.0f &&
+                    !(dot(n, ld3(other.V + (hb ? D.vA : D.vB)) - c) <= 0.00000f));
   }
-  const uint64_t m = __ballot(sep);
-  if (!m) return HINT_NONE;
-  const int l = __ffsll((unsigned long long)m) - 1;   /* the code of the first separating face */
-  return l < 32 ? (uint32_t)l : (0x100u | (uint32_t)(l - 32));
+  const int h = lane >> 5, v = lane & 31;
+  while (cand) {
+    const int c0 = __ffsll((unsigned long long)cand) - 1;
+    cand &= cand - 1ull;
+    const int c1 = cand ? __ffsll((unsigned long long)cand) - 1 : -1;
+    if (c1 >= 0) cand &= cand - 1ull;
+    const int cf = (h && c1 >= 0) ? c1 : c0;   /* this half's face: code cf (B's faces from 32) */
+    const bool fb = cf >= 32;
+    const Shape& own = fb ? B : A;
+    const Shape& other = fb ? A : B;
+    const int f = cf & 31;
+    const f3 n = ld3(own.Fn + f), c = ld3(own.Fc + f);
+    const bool vv = v < other.nv;
+    const bool behind = vv && dot(n, ld3(other.V + (vv ? v : 0)) - c) <= 0.00000f;
+    const uint64_t m = __ballot(behind);
+    if ((uint32_t)m == 0u) return c0 < 32 ? (uint32_t)c0 : (0x100u | (uint32_t)(c0 - 32));
+    if (c1 >= 0 && (uint32_t)(m >> 32) == 0u) return c1 < 32 ? (uint32_t)c1 : (0x100u | (uint32_t)(c1 - 32));
+  }
+  return HINT_NONE;
 }
 
 /* The depth clamps (DEM_Kernels_Compute.cuh:918-950). */
@@ -1439,7 +1454,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       const Shape A{S.v[0], S.fn[0], S.fc[0], ha.nv, ha.nf, S.vo[0], S.area[0], S.fnv[0]};
       const Shape B{S.v[1], S.fn[1], S.fc[1], hb.nv, hb.nf, S.vo[1], S.area[1], S.fnv[1]};
       const PairDirs D = pair_dirs(A, B, o.PosA, o.PosB);
-      const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, vbA, vbB, D);
+      const uint32_t code = separated_wave(A, B, o.PosA, o.PosB, D);
       PColl c{false, C_SP, 0.0f, mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
       if (code != HINT_NONE) {
         if (lane == 0) {
