@@ -28,6 +28,8 @@
  * Every float expression follows the restatement's operation order, so the
  * results equal the oracle's bit for bit away from sin/cos in the quaternion.
  */
+#include <atomic>
+
 #include "bdem_kernels.hpp"
 
 namespace bdem {
@@ -1588,6 +1590,27 @@ __global__ __launch_bounds__(256) void k_poly_finish(
 }
 
 /* ------------------------------------------------------------------ launchers */
+/* Blocks of 256 threads of `kernel` that the device holds at once (every
+ * CU full), a multiple of the 8 XCDs: the grid of the two pair kernels, so
+ * that no partial second round of blocks starts late.  Queried once per
+ * device (slot: which kernel). */
+template <class K>
+static unsigned resident_blocks(K kernel, int slot) {
+  static std::atomic<unsigned> cache[2][64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return (unsigned)POLY_PAIR_BLOCKS;
+  unsigned nb = cache[slot][dev].load(std::memory_order_relaxed);
+  if (nb == 0u) {
+    int per = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess || per <= 0) per = 8;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    nb = (unsigned)(per * cus) / NUM_XCD * NUM_XCD;
+    if (nb < (unsigned)NUM_XCD) nb = NUM_XCD;
+    cache[slot][dev].store(nb, std::memory_order_relaxed);
+  }
+  return nb;
+}
+
 static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                                       const PolyBuffers& pb, bool energy, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
@@ -1595,11 +1618,14 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
                      b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
   hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
-  /* grid-stride over the queue: 8 blocks of 256 per CU on 256 CUs */
+  /* both pair kernels loop over their queues with the resident grid; the
+   * first splits its queue by XCD (a grid of whole XCD multiples) */
   static_assert(POLY_PAIR_BLOCKS % NUM_XCD == 0, "the pair kernel splits its queue by XCD");
-  hipLaunchKernelGGL(k_poly_pairs, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
-                     b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf, pb.pt, pb.hint);
-  hipLaunchKernelGGL(k_poly_narrow, dim3(POLY_PAIR_BLOCKS), dim3(256), 0, s, dP, dshape, pb.hdr, b.n, pb.L, pb.rot,
+  hipLaunchKernelGGL(k_poly_pairs, dim3(resident_blocks(k_poly_pairs, 0)), dim3(256), 0, s, dP, dshape, pb.hdr, b.n,
+                     pb.L, pb.rot, b.pos_srt, b.vel_srt, b.omg_srt, pb.list, pb.queue, pb.qcount, pb.equeue, pb.pf,
+                     pb.pt, pb.hint);
+  hipLaunchKernelGGL(k_poly_narrow, dim3(resident_blocks(k_poly_narrow, 1)), dim3(256), 0, s, dP, dshape, pb.hdr, b.n,
+                     pb.L, pb.rot,
                      b.pos_srt, b.vel_srt, b.omg_srt, pb.qcount, pb.equeue, pb.pf, pb.pt, energy ? 1 : 0, pb.hint);
 }
 
