@@ -209,12 +209,30 @@ int upload_params(bdem_ctx* c) {
   build_params(c);
   HIPCHK(c, hipMemcpyAsync(c->dp, &c->hp, sizeof(DevParams), hipMemcpyHostToDevice, c->stream));
   std::vector<ShapeHdr> hdr;
+  std::vector<float4> local;
   if (c->dshape) {
     HIPCHK(c, hipMemcpyAsync(c->dshape, c->shapes.data(), c->shapes.size() * sizeof(bdem_poly_shape),
                              hipMemcpyHostToDevice, c->stream));
     for (const bdem_poly_shape& s : c->shapes)
       hdr.push_back(ShapeHdr{s.num_vertex, s.num_faces, s.num_edges, 0, s.vbound, s.plane_slack, s.radius, 0.0f});
     HIPCHK(c, hipMemcpyAsync(c->pb.hdr, hdr.data(), hdr.size() * sizeof(ShapeHdr), hipMemcpyHostToDevice,
+                             c->stream));
+    const PolyLayout& L = c->pb.L;
+    local.assign(c->shapes.size() * (size_t)L.stride, float4{0.f, 0.f, 0.f, 0.f});
+    for (size_t r = 0; r < c->shapes.size(); ++r) {
+      const bdem_poly_shape& s = c->shapes[r];
+      float4* row = local.data() + r * (size_t)L.stride;
+      for (int v = 0; v < s.num_vertex; ++v)
+        row[v] = float4{s.Local_Vertex[v].x, s.Local_Vertex[v].y, s.Local_Vertex[v].z, 1.0f};
+      for (int f = 0; f < s.num_faces; ++f) {
+        const bdem_Surface& F = s.face[f];
+        row[L.pv + f] = float4{F.normal.x, F.normal.y, F.normal.z, 1.0f};
+        /* the float differences the turned centroid is made from */
+        const float dx = F.centroid.x - s.COM.x, dy = F.centroid.y - s.COM.y, dz = F.centroid.z - s.COM.z;
+        row[L.pv + L.pf + f] = float4{dx, dy, dz, 1.0f};
+      }
+    }
+    HIPCHK(c, hipMemcpyAsync(c->pb.local, local.data(), local.size() * sizeof(float4), hipMemcpyHostToDevice,
                              c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -763,6 +781,7 @@ int bdem_set_simulation_data(bdem_ctx* c, const bdem_WOBJ* wobj, const bdem_POBJ
     pb.L = poly_layout(c->shapes);
     ALLOC(c->dshape, c->shapes.size());
     ALLOC(pb.hdr, c->shapes.size());
+    ALLOC(pb.local, c->shapes.size() * (size_t)pb.L.stride);
     ALLOC(pb.rot, (size_t)n * pb.L.stride);
     ALLOC(pb.list, (size_t)n * LIST_STRIDE);
     ALLOC(pb.nn, (size_t)n);
@@ -1056,9 +1075,11 @@ int bdem_set_poly_shapes(bdem_ctx* c, const bdem_poly_shape* shapes, int count) 
   drop_graphs(c);
   bdem_poly_shape* d = nullptr;
   ShapeHdr* dh = nullptr;
+  float4* dl = nullptr;
   if ((rc = dalloc(c, &d, t.size())) != BDEM_OK) return rc;
-  if ((rc = dalloc(c, &dh, t.size())) != BDEM_OK) {
+  if ((rc = dalloc(c, &dh, t.size())) != BDEM_OK || (rc = dalloc(c, &dl, t.size() * (size_t)L.stride)) != BDEM_OK) {
     dfree(c, d);
+    dfree(c, dh);
     return rc;
   }
   if (L.stride != c->pb.L.stride) {
@@ -1066,6 +1087,7 @@ int bdem_set_poly_shapes(bdem_ctx* c, const bdem_poly_shape* shapes, int count) 
     if ((rc = dalloc(c, &nr, (size_t)n * L.stride)) != BDEM_OK) {
       dfree(c, d);
       dfree(c, dh);
+      dfree(c, dl);
       return rc;
     }
     dfree(c, c->pb.rot);
@@ -1073,8 +1095,10 @@ int bdem_set_poly_shapes(bdem_ctx* c, const bdem_poly_shape* shapes, int count) 
   }
   dfree(c, c->dshape);
   dfree(c, c->pb.hdr);
+  dfree(c, c->pb.local);
   c->dshape = d;
   c->pb.hdr = dh;
+  c->pb.local = dl;
   c->pb.L = L;
   c->shapes.swap(t);
   c->shape_by_pid = true;

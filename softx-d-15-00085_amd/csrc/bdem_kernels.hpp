@@ -72,6 +72,8 @@ struct ShapeHdr {
 struct PolyBuffers {
   PolyLayout L;
   ShapeHdr* hdr;                       /* one per shape-table row */
+  float4* local;                       /* per row, `stride` float4 in the rot layout, local frame: vertices,
+                                          normals, centroid - COM; w = 1 where the slot is used */
   float4* rot;                         /* n * stride */
   uint32_t* list;                      /* n * 32 partner slots */
   uint32_t* nn;                        /* full contact count per particle */

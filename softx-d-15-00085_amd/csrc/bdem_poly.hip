@@ -1193,7 +1193,7 @@ __device__ __forceinline__ void lifter_forces_poly(const DevParams& P, int t, f3
 
 /* ------------------------------------------------------------------ kernels */
 __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict__ Pp,
-                                                     const bdem_poly_shape* __restrict__ shapes, int n,
+                                                     const float4* __restrict__ local, int n,
                                                      PolyLayout L, const float4* __restrict__ pos,
                                                      const float4* __restrict__ omg,
                                                      const float4* __restrict__ quat,
@@ -1204,23 +1204,17 @@ __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict
   const size_t i = t / (size_t)L.stride;
   const int e = (int)(t - i * (size_t)L.stride);
   if (i >= (size_t)n) return;
-  const bdem_poly_shape& po = shapes[shape_index(P, omg[i].w)];
+  /* the shape row's slot e in the same layout, local frame (w = 1: used) */
+  const float4 l4 = local[(size_t)shape_index(P, omg[i].w) * L.stride + e];
+  const f3 l = xyz(l4);
   const f3 c = xyz(pos[i]);
   const float4 q4 = quat[i];
   const QuatF Q{q4.x, q4.y, q4.z, q4.w};
   const QuatF QT{Q.w, -Q.x, -Q.y, -Q.z};
   f3 out = mk3(0.f, 0.f, 0.f);
-  if (e < L.pv) {
-    if (e < po.num_vertex) out = (P.rotation ? qrot(Q, QT, cvt(po.Local_Vertex[e])) : cvt(po.Local_Vertex[e])) + c;
-  } else if (e < L.pv + L.pf) {
-    const int f = e - L.pv;
-    if (f < po.num_faces) out = P.rotation ? qrot(Q, QT, cvt(po.face[f].normal)) : cvt(po.face[f].normal);
-  } else {
-    const int f = e - L.pv - L.pf;
-    if (f < po.num_faces) {
-      const f3 d = cvt(po.face[f].centroid) - cvt(po.COM);
-      out = (P.rotation ? qrot(Q, QT, d) : d) + c;
-    }
+  if (l4.w != 0.0f) {
+    out = P.rotation ? qrot(Q, QT, l) : l;
+    if (e < L.pv || e >= L.pv + L.pf) out = out + c;   /* vertices and centroids; normals turn only */
   }
   rot[t] = make_float4(out.x, out.y, out.z, 0.0f);
 }
@@ -1614,7 +1608,7 @@ static unsigned resident_blocks(K kernel, int slot) {
 static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape* dshape, const StepBuffers& b,
                                       const PolyBuffers& pb, bool energy, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
-  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dshape,
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, pb.local,
                      b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
   hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
@@ -1635,7 +1629,7 @@ void launch_poly_world_shapes(const DevParams* dP, const bdem_poly_shape* dshape
                               const PolyBuffers& pb, hipStream_t s) {
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
   if (rot_threads == 0) return;
-  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, dshape,
+  hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, pb.local,
                      b.n, pb.L, b.pos_cur, b.omg_cur, b.quat_cur, pb.rot, pb.qcount);
 }
 
