@@ -560,16 +560,16 @@ __device__ __forceinline__ bool face_separates(const Shape& A, int f, f3 PosB, c
   const float dist = dot(n, PosB - c);
   if (!(dist > 0.0f)) return false;
   if (beyond_by_sphere(dist, c, PosB, vb)) return true;
-  /* four vertices per round, their loads in flight together (indices past
+  /* eight vertices per round, their loads in flight together (indices past
    * the count re-read the last vertex and are not tested) */
   const int last = B.nv - 1;
-  for (int v = 0; v < B.nv; v += 4) {
-    const f3 p0 = ld3(B.V + v), p1 = ld3(B.V + min(v + 1, last));
-    const f3 p2 = ld3(B.V + min(v + 2, last)), p3 = ld3(B.V + min(v + 3, last));
-    if (dot(n, p0 - c) <= 0.00000f) return false;
-    if (v + 1 < B.nv && dot(n, p1 - c) <= 0.00000f) return false;
-    if (v + 2 < B.nv && dot(n, p2 - c) <= 0.00000f) return false;
-    if (v + 3 < B.nv && dot(n, p3 - c) <= 0.00000f) return false;
+  for (int v = 0; v < B.nv; v += 8) {
+    f3 p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = ld3(B.V + min(v + u, last));
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (v + u < B.nv && dot(n, p[u] - c) <= 0.00000f) return false;
   }
   return true;
 }
@@ -579,15 +579,13 @@ __device__ __forceinline__ int facing_face(const Shape& A, f3 dirAB) {
   int best = 0;
   float bv = -3.0e38f;
   const int last = A.nf - 1;
-  for (int f = 0; f < A.nf; f += 4) {   /* four normals per round, as above */
-    const float d0 = dot(ld3(A.Fn + f), dirAB);
-    const float d1 = dot(ld3(A.Fn + min(f + 1, last)), dirAB);
-    const float d2 = dot(ld3(A.Fn + min(f + 2, last)), dirAB);
-    const float d3 = dot(ld3(A.Fn + min(f + 3, last)), dirAB);
-    if (d0 > bv) { bv = d0; best = f; }
-    if (f + 1 < A.nf && d1 > bv) { bv = d1; best = f + 1; }
-    if (f + 2 < A.nf && d2 > bv) { bv = d2; best = f + 2; }
-    if (f + 3 < A.nf && d3 > bv) { bv = d3; best = f + 3; }
+  for (int f = 0; f < A.nf; f += 8) {   /* eight normals per round, as above */
+    float d[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) d[u] = dot(ld3(A.Fn + min(f + u, last)), dirAB);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (f + u < A.nf && d[u] > bv) { bv = d[u]; best = f + u; }
   }
   return best;
 }
