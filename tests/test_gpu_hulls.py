@@ -62,6 +62,40 @@ def test_hull_contact_lists_and_forces(hull_box):
     ora.close()
 
 
+@pytest.mark.parametrize("kmin,kmax", [(4, 6), (17, 18)])
+def test_smallest_and_largest_hulls_match_oracle(kmin, kmax):
+    """Hulls of 4-6 points (tetrahedra and up) and of 17-18 points, whose 32
+    faces fill the shape table's face rows (48 edges): the lane-per-face and
+    lane-per-vertex tests of the pair kernels at their extremes.  Contact
+    lists bit-exact, forces and 20-step states within 1e-5."""
+    print(f"[case] hull box 300 hulls of {kmin}-{kmax} points", flush=True)
+    deck = S.hull_box_deck(300, seed=43, kmin=kmin, kmax=kmax)
+    if kmax == 18:
+        assert max(s.num_faces for s in deck["shapes"]) == T.SHAPE_MAX_FACE
+    dev, ora = make_pair(deck)
+    lg, ng = dev.debug_contacts()
+    lo, no = ora.contacts()
+    assert np.array_equal(ng, no)
+    assert no.sum() > 0
+    for i in np.nonzero(no)[0]:
+        k = min(int(no[i]), 32)
+        assert np.array_equal(lg[i, :k], lo[i, :k]), f"particle {i}"
+    g = dev.debug_forces()
+    o = ora.forces()
+    assert np.array_equal(g[4], o[4])
+    assert float(np.abs(o[0]).max()) > 0.0
+    for a, b, what in zip(g[:4], o[:4], ("F_pp", "T_pp", "F_wall", "T_wall")):
+        scale = max(float(np.max(np.abs(b))), 1e-30)
+        err = rel_err(a, b, 1e-3 * scale)
+        print(f"hulls of {kmin}-{kmax} points {what}: max rel err {err:.3e}")
+        assert err <= TOL, f"{what}: {err}"
+    dev.step(20)
+    ora.step(20)
+    _compare_states(f"hulls of {kmin}-{kmax} points, 20 steps", deck, dev, ora)
+    dev.close()
+    ora.close()
+
+
 @pytest.mark.parametrize("steps", [1, 20])
 def test_hull_box_steps_match_oracle(hull_box, steps):
     dev, ora = make_pair(hull_box)
