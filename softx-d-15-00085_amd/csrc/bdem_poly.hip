@@ -873,8 +873,18 @@ __device__ __forceinline__ PColl poly_cylinder(f3 p, float r, const Shape& A, co
 /* WorldObject_InteractionParticle, polyhedra branches (DEM_Kernels_Compute.cuh:
  * 254-610).  Infinite planes have no polyhedra method: the previous result
  * stands, as written. */
+/* No vertex of a body with centre p and vertex bound vb (every turned
+ * vertex within vb of p) lies behind the plane (n, c) when p is farther in
+ * front of it than |n| vb plus a margin (10 um and 4e-6 of the coordinates'
+ * size) beyond the float error of the vertex heights: the vertex test of
+ * the plane would find nothing. */
+__device__ __forceinline__ bool clear_of_plane(f3 n, f3 c, f3 p, float vb) {
+  const float size = fabsf(p.x) + fabsf(p.y) + fabsf(p.z) + fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+  return dot(n, p - c) > length(n) * vb + 0.001f + 4.0e-6f * size;
+}
+
 __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p, float r, f3 v, f3 w, float m,
-                                                 const Shape& A, f3& F, f3& T, float* tally) {
+                                                 const Shape& A, float vb, f3& F, f3& T, float* tally) {
   F = mk3(0.f, 0.f, 0.f);
   T = mk3(0.f, 0.f, 0.f);
   const bool moving_planes = P.rotating_wobj && P.cyl_rotation;
@@ -904,11 +914,11 @@ __device__ __forceinline__ void wall_forces_poly(const DevParams& P, int t, f3 p
         F = ft;
         if (P.rotation && res.cls != C_FF) T = T + tq;
       }
-      if (cyl.has_top_cap) {
+      if (cyl.has_top_cap && !clear_of_plane(cvt(cyl.normal_top_cap), cvt(cyl.center_top_cap), p, vb)) {
         res = vertices_plane(A, cvt(cyl.normal_top_cap), cvt(cyl.center_top_cap));
         if (res.hit) F = resp_endplate(P, t, v, Coll{true, res.dis, res.cp, res.n});
       }
-      if (cyl.has_bot_cap) {
+      if (cyl.has_bot_cap && !clear_of_plane(cvt(cyl.normal_bot_cap), cvt(cyl.center_bot_cap), p, vb)) {
         res = vertices_plane(A, cvt(cyl.normal_bot_cap), cvt(cyl.center_bot_cap));
         if (res.hit) F = resp_endplate(P, t, v, Coll{true, res.dis, res.cp, res.n});
       }
@@ -1522,7 +1532,7 @@ __global__ __launch_bounds__(256) void k_poly_finish(
       if (!P.rotation) Tpp = mk3(0.f, 0.f, 0.f);
       const Shape A = shape_of(rot, L, i, po);
       f3 Fw, Tw;
-      wall_forces_poly(P, ta, p, r, v, w, m, A, Fw, Tw, tally);
+      wall_forces_poly(P, ta, p, r, v, w, m, A, po.vbound * P.vbound_scale, Fw, Tw, tally);
       f3 Fl = mk3(0.f, 0.f, 0.f), Tl = Fl;
       if (LIFTERS && P.num_dobj > 0) lifter_forces_poly(P, ta, p, v, w, m, A, po, Fl, Tl, tally);
       if (FORCES_ONLY) {
