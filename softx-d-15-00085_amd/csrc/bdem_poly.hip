@@ -145,14 +145,6 @@ __device__ __forceinline__ float wave_min(float v) {
   return fminf(fminf(lane_f32(v, 0), lane_f32(v, 16)), fminf(lane_f32(v, 32), lane_f32(v, 48)));
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-  v = fmaxf(v, dpp_f32<DPP_XOR1>(v));
-  v = fmaxf(v, dpp_f32<DPP_XOR2>(v));
-  v = fmaxf(v, dpp_f32<DPP_ROR4>(v));
-  v = fmaxf(v, dpp_f32<DPP_ROR8>(v));
-  return fmaxf(fmaxf(lane_f32(v, 0), lane_f32(v, 16)), fmaxf(lane_f32(v, 32), lane_f32(v, 48)));
-}
-
 __device__ __forceinline__ int wave_min_i32(int v) {
   v = min(v, (int)dpp_u32<DPP_XOR1>((uint32_t)v));
   v = min(v, (int)dpp_u32<DPP_XOR2>((uint32_t)v));
