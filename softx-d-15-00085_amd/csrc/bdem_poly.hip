@@ -1370,7 +1370,8 @@ struct PairStage {
  * counts (lanes 0-31 body A, 32-63 body B; rows past the counts are inside
  * the record and never read) before the first LDS write.
  *
- * Five waves per SIMD: 96 VGPRs without spills, 5 x 21 KiB of LDS per CU. */
+ * Five waves per SIMD: at most 96 VGPRs without spills, and five blocks of
+ * 25 KiB of LDS (four stages and their chunk's pair records) per CU. */
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_poly_narrow(const DevParams* __restrict__ Pp,
                                                      const bdem_poly_shape* __restrict__ shapes,
                                                      const ShapeHdr* __restrict__ hdr, int n, PolyLayout L,
