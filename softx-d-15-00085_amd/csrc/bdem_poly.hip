@@ -1219,6 +1219,8 @@ __global__ __launch_bounds__(256) void k_poly_rotate(const DevParams* __restrict
   rot[t] = make_float4(out.x, out.y, out.z, 0.0f);
 }
 
+constexpr int LIST_LANES = 32;   /* particles per wave in the polyhedra list kernel (a power of two) */
+
 /* Bounding-sphere contact list (the sphere broad phase: Kernel_BroadCollision-
  * Detection_NonSymmetry, 32 slots, full count) and the pair work queue. */
 __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__ Pp, int n,
@@ -1230,8 +1232,13 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
                                                    uint32_t* status) {
   const DevParams& P = *Pp;
   __shared__ __align__(16) unsigned char wlds[4][sizeof(WaveStage)];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = i < n && packed_pid(omg[i].w) >= 0;
+  /* 32 consecutive particles per wave (lanes 32-63 idle): twice the waves
+   * of one particle per lane, to hide the scan's memory latency at the
+   * particle counts of a polyhedra run */
+  const int lane = threadIdx.x & 63;
+  const bool own = lane < LIST_LANES;
+  const int i = (blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * LIST_LANES + (lane & (LIST_LANES - 1));
+  const bool live = own && i < n && packed_pid(omg[i].w) >= 0;
   const float4 pr = live ? pos_r[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   /* the staged stencil scan of a wave of consecutive particles */
   int c = for_each_contact_wave(P, cells, pos_r, (uint32_t)n, i, xyz(pr), pr.w, live,
@@ -1240,10 +1247,9 @@ __global__ __launch_bounds__(256) void k_poly_list(const DevParams* __restrict__
                                 });
   if (!live) c = 0;
   if (c > LIST_STRIDE) atomicOr(status, ST_LIST_OVERFLOW);
-  if (i < n) nn[i] = (uint32_t)c;
+  if (own && i < n) nn[i] = (uint32_t)c;
   const int m = c < LIST_STRIDE ? c : LIST_STRIDE;
   /* wave prefix of the per-lane pair counts, one atomic per wave */
-  const int lane = threadIdx.x & 63;
   int incl = m;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -1611,7 +1617,8 @@ static void launch_poly_forces_common(const DevParams* dP, const bdem_poly_shape
   const size_t rot_threads = (size_t)b.n * pb.L.stride;
   hipLaunchKernelGGL(k_poly_rotate, dim3((unsigned)((rot_threads + 255) / 256)), dim3(256), 0, s, dP, pb.local,
                      b.n, pb.L, b.pos_srt, b.omg_srt, b.quat_srt, pb.rot, pb.qcount);
-  hipLaunchKernelGGL(k_poly_list, dim3((b.n + 255) / 256), dim3(256), 0, s, dP, b.n, b.cells, b.pos_srt,
+  hipLaunchKernelGGL(k_poly_list, dim3((b.n + 4 * LIST_LANES - 1) / (4 * LIST_LANES)), dim3(256), 0, s, dP, b.n,
+                     b.cells, b.pos_srt,
                      b.omg_srt, pb.list, pb.nn, pb.queue, pb.qcount, b.status);
   /* both pair kernels loop over their queues with the resident grid; the
    * first splits its queue by XCD (a grid of whole XCD multiples) */
